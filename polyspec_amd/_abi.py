@@ -1,0 +1,158 @@
+"""ctypes bindings to libpolyspec_hip.so (declared in include/polyspec_hip.h).
+
+The product path has no CPU fallback: if the HIP library is missing, importing
+the bindings raises.  torch tensors are used only as device containers; every
+compute call goes through the C ABI on torch's current HIP stream.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpolyspec_hip.so")
+
+_P, _I, _LL, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+
+# name -> (restype, argtypes); every entry point of include/polyspec_hip.h
+SIGNATURES = {
+    "ps_last_error": (ctypes.c_char_p, []),
+    "ps_version": (_I, []),
+    "ps_init": (_I, [_I]),
+    "ps_sht_plan_create": (_I, [_I, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "ps_sht_plan_destroy": (_I, [_P]),
+    "ps_sht_plan_info": (_I, [_P, ctypes.POINTER(_LL), ctypes.POINTER(_LL), ctypes.POINTER(_I)]),
+    "ps_sht_workspace_bytes": (_LL, [_P, _I, _I]),
+    "ps_sht_lmajor_multiple": (_I, [_P]),
+    "ps_sht_synthesis": (_I, [_P, _I, _P, _P, _P, _LL, _P]),
+    "ps_sht_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P, _LL, _P]),
+    "ps_sht_adjoint": (_I, [_P, _I, _P, _P, _D, _P, _LL, _P]),
+}
+
+_lib = None
+
+
+class PolySpecHIPError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PolySpecHIPError(
+            "libpolyspec_hip.so not found at %s: build it with `python -m polyspec_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().ps_last_error()
+        raise PolySpecHIPError("polyspec_hip error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_device(t, dtype, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise AssertionError("%s must be a device tensor" % name)
+    if t.dtype != dtype:
+        raise AssertionError("%s must have dtype %s (got %s)" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise AssertionError("%s must be contiguous" % name)
+
+
+_workspaces = {}
+
+
+def workspace(nbytes, device):
+    """A cached uint8 device buffer of at least nbytes (one per device)."""
+    key = torch.device(device).index or 0
+    buf = _workspaces.get(key)
+    if buf is None or buf.numel() < nbytes:
+        _workspaces[key] = None
+        buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+        _workspaces[key] = buf
+    return buf
+
+
+class ShtPlan:
+    """Immutable HEALPix SHT plan (nside, lmax, spin, precision) on the current device."""
+
+    def __init__(self, nside, lmax, spin=0, precision=64):
+        lib = load()
+        self.nside, self.lmax, self.spin, self.precision = int(nside), int(lmax), int(spin), int(precision)
+        h = ctypes.c_void_p()
+        check(lib.ps_sht_plan_create(self.nside, self.lmax, self.lmax, self.spin, self.precision, ctypes.byref(h)))
+        self._h = h
+        npix, nlm, nc = _LL(), _LL(), _I()
+        check(lib.ps_sht_plan_info(h, ctypes.byref(npix), ctypes.byref(nlm), ctypes.byref(nc)))
+        self.npix, self.nlm, self.ncomp = npix.value, nlm.value, nc.value
+        self.rdtype = torch.float32 if precision == 32 else torch.float64
+        self.cdtype = torch.complex64 if precision == 32 else torch.complex128
+        self.lmajor_multiple = lib.ps_sht_lmajor_multiple(h)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.ps_sht_plan_destroy(h)
+            self._h = None
+
+    def work_bytes(self, nmaps, lmajor):
+        return int(load().ps_sht_workspace_bytes(self._h, int(nmaps), int(lmajor)))
+
+    def synthesis(self, alm, out=None):
+        """alm (nmaps, ncomp, nlm) complex -> maps (nmaps, ncomp, npix) real."""
+        require_device(alm, self.cdtype, "alm")
+        assert alm.dim() == 3 and alm.shape[1] == self.ncomp and alm.shape[2] == self.nlm, "Wrong alm shape"
+        nm = alm.shape[0]
+        if out is None:
+            out = torch.empty((nm, self.ncomp, self.npix), dtype=self.rdtype, device=alm.device)
+        require_device(out, self.rdtype, "map")
+        wb = self.work_bytes(nm, 0)
+        ws = workspace(wb, alm.device)
+        check(load().ps_sht_synthesis(self._h, nm, ptr(alm), ptr(out), ptr(ws), wb, stream_handle()))
+        return out
+
+    def synthesis_lmajor(self, alm_t, nmaps, out=None):
+        """alm_t (nlm, ld) complex, columns map*ncomp+comp -> maps (nmaps, ncomp, npix)."""
+        require_device(alm_t, self.cdtype, "alm_t")
+        assert alm_t.dim() == 2 and alm_t.shape[0] == self.nlm
+        ld = alm_t.shape[1]
+        if out is None:
+            out = torch.empty((nmaps, self.ncomp, self.npix), dtype=self.rdtype, device=alm_t.device)
+        require_device(out, self.rdtype, "map")
+        wb = self.work_bytes(nmaps, 1)
+        ws = workspace(wb, alm_t.device)
+        check(load().ps_sht_synthesis_lmajor(self._h, int(nmaps), ptr(alm_t), ld, ptr(out), ptr(ws), wb,
+                                             stream_handle()))
+        return out
+
+    def adjoint(self, maps, scale, out=None):
+        """maps (nmaps, ncomp, npix) real -> scale * adjoint (nmaps, ncomp, nlm) complex."""
+        require_device(maps, self.rdtype, "map")
+        assert maps.dim() == 3 and maps.shape[1] == self.ncomp and maps.shape[2] == self.npix, "Wrong map shape"
+        nm = maps.shape[0]
+        if out is None:
+            out = torch.empty((nm, self.ncomp, self.nlm), dtype=self.cdtype, device=maps.device)
+        require_device(out, self.cdtype, "alm")
+        wb = self.work_bytes(nm, 1)
+        ws = workspace(wb, maps.device)
+        check(load().ps_sht_adjoint(self._h, nm, ptr(maps), ptr(out), float(scale), ptr(ws), wb, stream_handle()))
+        return out

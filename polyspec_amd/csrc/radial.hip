@@ -1,0 +1,450 @@
+// Radial-node pixel and harmonic kernels of the gNL-loc / tauNL-loc estimators
+// (the n = 0 loop nests of polyspec/cython/tauNL_utils.pyx and ideal_fisher.pyx).
+//
+// All of these are HBM-bound streams; the design rules are: one pass over
+// each input, coalesced 1-D grid-stride access, FP64 accumulation, and a
+// fixed-order two-stage reduction (per-block partials, then one block) so
+// that results are bit-reproducible run to run.
+#include <math.h>
+
+#include "polyspec_hip.h"
+#include "ps_common.h"
+
+namespace ps {
+
+static constexpr int RED_BLOCKS = 1024;  // fixed partial count => deterministic order
+static constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    // fixed-order tree over 256 threads
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int n, double* out,
+                                                      int accumulate) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = accumulate ? (*out + s) : s;
+}
+
+// ---------------------------------------------------------- radial weighting
+// tauNL_utils.apply_fl_weights (tauNL_utils.pyx:80-98) writing the l-major
+// synthesis input: out_t[idx(l,m) * ld + r] = w sum_{X<npol} f[l, X, r] h[X, idx(l,m)]
+// for lmin <= l <= lmax, zero otherwise (and zero in padding columns r >= nr).
+// f: (lmax_f+1, nslot, nr) real; h: (npol, nlm) complex (full m-major set).
+template <typename T>
+__global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, const cplx<T>* __restrict__ h,
+                                    int npol, int lmax, int lmin, T w, int ld, cplx<T>* __restrict__ out,
+                                    int col_offset) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlm * ld) return;
+    const long lm = i / ld;
+    const int c = (int)(i % ld) - col_offset;
+    if (c < 0) return;  // columns owned by another family
+    if (c >= nr) {
+        if (col_offset == 0 || c < ld) { /* padding */ }
+    }
+    // recover l from lm (m-major): m = largest with m(2lmax+1-m)/2 <= lm
+    const double B = 2.0 * lmax + 3.0;
+    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
+    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
+    while ((long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm && m < lmax) m++;
+    const int l = (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+    cplx<T> v{T(0), T(0)};
+    if (c < nr && l >= lmin) {
+        const cplx<T> h0 = h[lm];
+        const T f0 = f[((long)l * nslot + 0) * nr + c];
+        v.re = f0 * h0.re;
+        v.im = f0 * h0.im;
+        if (npol > 1) {
+            const cplx<T> h1 = h[nlm + lm];
+            const T f1 = f[((long)l * nslot + 1) * nr + c];
+            v.re += f1 * h1.re;
+            v.im += f1 * h1.im;
+        }
+        v.re *= w;
+        v.im *= w;
+    }
+    if (c < nr || col_offset == 0) out[lm * ld + col_offset + c] = v;
+}
+
+// ------------------------------------------------------------- gNL-loc sums
+// sum_r w_r sum_p [c4 P^3 Q + c2 (P^2 U + P Q V)], P, Q, U, V: (nr, npix).
+// c2 = 0 gives tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270) times c4; U = sum_s P_s Q_s and
+// V = sum_s P_s^2 give the collapsed disconnected sum of gnl_loc_disc_sum
+// (tauNL_utils.pyx:275-285) over the simulations s.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gnl_fused(const T* __restrict__ P, const T* __restrict__ Q,
+                                                   const T* __restrict__ U, const T* __restrict__ V,
+                                                   const double* __restrict__ w, int nr, long npix, double c4,
+                                                   double c2, double* __restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long n = (long)nr * npix;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int r = (int)(i / npix);
+        const double p = (double)P[i], q = (double)Q[i];
+        double t = c4 * p * p * p * q;
+        if (U) t += c2 * (p * p * (double)U[i] + p * q * (double)V[i]);
+        acc += w[r] * t;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285):
+// sum_r w_r sum_p (P1^2 P2 Q2 + P2^2 P1 Q1)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gnl_disc(const T* __restrict__ P1, const T* __restrict__ P2,
+                                                  const T* __restrict__ Q1, const T* __restrict__ Q2,
+                                                  const double* __restrict__ w, int nr, long npix,
+                                                  double* __restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long n = (long)nr * npix;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int r = (int)(i / npix);
+        const double p1 = (double)P1[i], p2 = (double)P2[i];
+        acc += w[r] * (p1 * p1 * p2 * (double)Q2[i] + p2 * p2 * p1 * (double)Q1[i]);
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// U += sum over the batch of P_s Q_s, V += sum of P_s^2; P, Q: (nsim, n), U, V: (n)
+template <typename T>
+__global__ void k_mc_accumulate(const T* __restrict__ P, const T* __restrict__ Q, int nsim, long n,
+                                T* __restrict__ U, T* __restrict__ V) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double u = (double)U[i], v = (double)V[i];
+        for (int s = 0; s < nsim; s++) {
+            const double p = (double)P[(long)s * n + i], q = (double)Q[(long)s * n + i];
+            u += p * q;
+            v += p * p;
+        }
+        U[i] = (T)u;
+        V[i] = (T)v;
+    }
+}
+
+// ------------------------------------------------------- pixel products
+// mode 0: out = a*b                   (ideal_fisher.multiplyRR, ideal_fisher.pyx:100-108)
+// mode 1: out = a*b + c*d             (multiplyRR_sym, ideal_fisher.pyx:112-120)
+// mode 2: out = a*b*c                 (tauNL_utils.multiply, tauNL_utils.pyx:103-111)
+// mode 3: out = a*(2 b c + a d)       (tauNL_utils.multiply_asym, tauNL_utils.pyx:116-124)
+template <typename T>
+__global__ void k_products(int mode, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+                           const T* __restrict__ d, long n, T scale, T* __restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        T v;
+        switch (mode) {
+            case 0: v = a[i] * b[i]; break;
+            case 1: v = a[i] * b[i] + c[i] * d[i]; break;
+            case 2: v = a[i] * b[i] * c[i]; break;
+            default: v = a[i] * (T(2) * b[i] * c[i] + a[i] * d[i]); break;
+        }
+        out[i] = v * scale;
+    }
+}
+
+// ------------------------------------------------------------- radial sums
+// tauNL_utils.radial_sum (tauNL_utils.pyx:220-235) over the full m-major set with
+// l < lmin zeroed: out[X, lm] = scale sum_r lm_map[r, lm] f[l, X, r] w_r (X < npol),
+// lm_map: (nr, nlm); f: (lmax+1, nslot, nr).  accumulate != 0 adds into out.
+template <typename T>
+__global__ void k_radial_sum(const cplx<T>* __restrict__ lmmap, int nr, int lmax, int lmin,
+                             const T* __restrict__ f, int nslot, int npol, const double* __restrict__ w,
+                             double scale, int accumulate, cplx<T>* __restrict__ out) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const long lm = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lm >= nlm) return;
+    const double B = 2.0 * lmax + 3.0;
+    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
+    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
+    while ((long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm && m < lmax) m++;
+    const int l = (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+    for (int X = 0; X < npol; X++) {
+        double re = 0.0, im = 0.0;
+        if (l >= lmin) {
+            for (int r = 0; r < nr; r++) {
+                const cplx<T> v = lmmap[(long)r * nlm + lm];
+                const double fw = (double)f[((long)l * nslot + X) * nr + r] * w[r];
+                re += (double)v.re * fw;
+                im += (double)v.im * fw;
+            }
+        }
+        cplx<T>* o = out + (long)X * nlm + lm;
+        if (accumulate) {
+            o->re += (T)(re * scale);
+            o->im += (T)(im * scale);
+        } else {
+            *o = cplx<T>{(T)(re * scale), (T)(im * scale)};
+        }
+    }
+}
+
+// -------------------------------------------------------------- tau_sum_n0
+// tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646) restated as per-L Gram
+// matrices: tau = sum_L sum_{j,k} F_L(j,k) w_j w_k / (sqrt(4pi) sqrt(2n1+1))
+//                 sum_{M} (1+[M>0]) Re(A_j(L,M) conj(B_k(L,M))).
+// A, B: (nmu, nr, nlm(Lmaxs)) complex in m-major order over 0..Lmaxs.
+// One block per L; each thread owns Gram entries (j, k).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tau_gram(const cplx<T>* __restrict__ A, const cplx<T>* __restrict__ Bm,
+                                                  int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
+                                                  const double* __restrict__ FLL, const double* __restrict__ w,
+                                                  double pref, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int L = Lmin + blockIdx.x;
+    double acc = 0.0;
+    if (L <= Lmax) {
+        const long nlm = (long)(Lmaxs + 1) * (Lmaxs + 2) / 2;
+        for (int jk = threadIdx.x; jk < nr * nr; jk += blockDim.x) {
+            const int j = jk / nr, k = jk % nr;
+            if (w[j] == 0.0 || w[k] == 0.0) continue;
+            double g = 0.0;
+            for (int mu = 0; mu < nmu; mu++) {
+                const cplx<T>* a = A + ((long)mu * nr + j) * nlm;
+                const cplx<T>* b = Bm + ((long)mu * nr + k) * nlm;
+                for (int M = 0; M <= L; M++) {
+                    const long id = (long)M * (2 * Lmaxs + 1 - M) / 2 + L;
+                    const double re = (double)a[id].re * (double)b[id].re + (double)a[id].im * (double)b[id].im;
+                    g += (M == 0 ? 1.0 : 2.0) * re;
+                }
+            }
+            acc += FLL[((long)L * nr + j) * nr + k] * w[j] * w[k] * g;
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc * pref;
+}
+
+// ---------------------------------------------------------- outer product
+// ideal_fisher.outer_product (ideal_fisher.pyx:423-459), sym = False:
+// fish[a,b] = Re sum_j [Q0a* Q0b + Q1a* Q1b + 9(Q2a* Q2b + Q3a* Q3b)
+//                       - 3(Q0a* Q3b + Q1a* Q2b + Q3a* Q0b + Q2a* Q1b)] / (24 * 48)
+// Qa: (4, nta, nin), Qb: (4, ntb, nin).  grid (RED_BLOCKS, nta*ntb) -> partials.
+template <typename T>
+__global__ __launch_bounds__(256) void k_outer(const cplx<T>* __restrict__ Qa, const cplx<T>* __restrict__ Qb,
+                                               int nta, int ntb, long nin, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int ab = blockIdx.y, ia = ab / ntb, ib = ab % ntb;
+    double acc = 0.0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < nin; j += stride) {
+        cplx<T> a[4], b[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            a[c] = Qa[((long)c * nta + ia) * nin + j];
+            b[c] = Qb[((long)c * ntb + ib) * nin + j];
+        }
+        auto rdot = [](cplx<T> x, cplx<T> y) { return (double)x.re * (double)y.re + (double)x.im * (double)y.im; };
+        acc += rdot(a[0], b[0]) + rdot(a[1], b[1]) + 9.0 * (rdot(a[2], b[2]) + rdot(a[3], b[3])) -
+               3.0 * (rdot(a[0], b[3]) + rdot(a[1], b[2]) + rdot(a[3], b[0]) + rdot(a[2], b[1]));
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(long)ab * gridDim.x + blockIdx.x] = acc / (24.0 * 48.0);
+}
+
+__global__ void k_sum_rows(const double* __restrict__ part, int nrow, int ncolp, double* __restrict__ out,
+                           int accumulate) {
+    __shared__ double sh[256];
+    const int row = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < ncolp; i += 256) s += part[(long)row * ncolp + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[row] = accumulate ? out[row] + s : s;
+}
+
+}  // namespace ps
+
+using namespace ps;
+
+// =============================================================== C ABI
+template <typename T>
+static int weight_alm_impl(const void* f, int nslot, int nr, const void* h, int npol, int lmax, int lmin, double w,
+                           int ld, void* out, int col_offset, void* stream) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const long tot = nlm * ld;
+    hipLaunchKernelGGL(k_weight_alm_lmajor<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const T*)f, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)w, ld,
+                       (cplx<T>*)out, col_offset);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
+static int reduce_launch_fused(const void* P, const void* Q, const void* U, const void* V, const double* w, int nr,
+                               long long npix, double c4, double c2, double* part, double* out, int accumulate,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_gnl_fused<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const T*)P, (const T*)Q,
+                       (const T*)U, (const T*)V, w, nr, (long)npix, c4, c2, part);
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part, RED_BLOCKS, out, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" {
+
+long long ps_reduce_scratch_bytes(void) { return (long long)RED_BLOCKS * 64 * sizeof(double); }
+
+int ps_weight_alm_lmajor(int precision, const void* f, int nslot, int nr, const void* h, int npol, int lmax,
+                         int lmin, double weight, int ld, int col_offset, void* out_t, void* stream) {
+    PS_REQUIRE(f && h && out_t, "ps_weight_alm_lmajor: NULL argument");
+    PS_REQUIRE(nr >= 1 && col_offset >= 0 && col_offset + nr <= ld, "ps_weight_alm_lmajor: bad nr/ld/col_offset");
+    PS_REQUIRE(npol >= 1 && npol <= 2 && nslot >= npol, "ps_weight_alm_lmajor: npol must be 1 or 2 and <= nslot");
+    PS_REQUIRE(lmin >= 0 && lmax >= lmin, "ps_weight_alm_lmajor: bad l range");
+    if (precision == 32) return weight_alm_impl<float>(f, nslot, nr, h, npol, lmax, lmin, weight, ld, out_t, col_offset, stream);
+    PS_REQUIRE(precision == 64, "ps_weight_alm_lmajor: precision must be 32 or 64");
+    return weight_alm_impl<double>(f, nslot, nr, h, npol, lmax, lmin, weight, ld, out_t, col_offset, stream);
+}
+
+int ps_gnl_loc_fused_sum(int precision, const void* P, const void* Q, const void* U, const void* V,
+                         const double* w, int nr, long long npix, double c4, double c2, double* scratch,
+                         double* out, int accumulate, void* stream) {
+    PS_REQUIRE(P && Q && w && scratch && out, "ps_gnl_loc_fused_sum: NULL argument");
+    PS_REQUIRE((U == nullptr) == (V == nullptr), "ps_gnl_loc_fused_sum: U and V must both be given or both NULL");
+    PS_REQUIRE(nr >= 1 && npix >= 1, "ps_gnl_loc_fused_sum: empty input");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == 32) return reduce_launch_fused<float>(P, Q, U, V, w, nr, npix, c4, U ? c2 : 0.0, scratch, out, accumulate, st);
+    PS_REQUIRE(precision == 64, "ps_gnl_loc_fused_sum: precision must be 32 or 64");
+    return reduce_launch_fused<double>(P, Q, U, V, w, nr, npix, c4, U ? c2 : 0.0, scratch, out, accumulate, st);
+}
+
+int ps_gnl_loc_disc_sum(int precision, const void* P1, const void* P2, const void* Q1, const void* Q2,
+                        const double* w, int nr, long long npix, double* scratch, double* out, int accumulate,
+                        void* stream) {
+    PS_REQUIRE(P1 && P2 && Q1 && Q2 && w && scratch && out, "ps_gnl_loc_disc_sum: NULL argument");
+    PS_REQUIRE(nr >= 1 && npix >= 1, "ps_gnl_loc_disc_sum: empty input");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == 32)
+        hipLaunchKernelGGL(k_gnl_disc<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const float*)P1,
+                           (const float*)P2, (const float*)Q1, (const float*)Q2, w, nr, (long)npix, scratch);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_gnl_disc<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const double*)P1,
+                           (const double*)P2, (const double*)Q1, (const double*)Q2, w, nr, (long)npix, scratch);
+    else
+        PS_REQUIRE(false, "ps_gnl_loc_disc_sum: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)scratch, RED_BLOCKS, out, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_mc_accumulate(int precision, const void* P, const void* Q, int nsim, long long n, void* U, void* V,
+                     void* stream) {
+    PS_REQUIRE(P && Q && U && V && nsim >= 1 && n >= 1, "ps_mc_accumulate: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_mc_accumulate<float>, dim3(nb), dim3(256), 0, st, (const float*)P, (const float*)Q, nsim,
+                           (long)n, (float*)U, (float*)V);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_mc_accumulate<double>, dim3(nb), dim3(256), 0, st, (const double*)P, (const double*)Q,
+                           nsim, (long)n, (double*)U, (double*)V);
+    else
+        PS_REQUIRE(false, "ps_mc_accumulate: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_pixel_products(int precision, int mode, const void* a, const void* b, const void* c, const void* d,
+                      long long n, double scale, void* out, void* stream) {
+    PS_REQUIRE(mode >= 0 && mode <= 3, "ps_pixel_products: mode must be 0..3");
+    PS_REQUIRE(a && b && out, "ps_pixel_products: NULL argument");
+    PS_REQUIRE(!(mode == 1 || mode == 3) || (c && d), "ps_pixel_products: c and d required for this mode");
+    PS_REQUIRE(!(mode == 2) || c, "ps_pixel_products: c required for mode 2");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 16384);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_products<float>, dim3(nb), dim3(256), 0, st, mode, (const float*)a, (const float*)b,
+                           (const float*)c, (const float*)d, (long)n, (float)scale, (float*)out);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_products<double>, dim3(nb), dim3(256), 0, st, mode, (const double*)a, (const double*)b,
+                           (const double*)c, (const double*)d, (long)n, scale, (double*)out);
+    else
+        PS_REQUIRE(false, "ps_pixel_products: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_radial_sum(int precision, const void* lm_map, int nr, int lmax, int lmin, const void* f, int nslot, int npol,
+                  const double* w, double scale, int accumulate, void* out, void* stream) {
+    PS_REQUIRE(lm_map && f && w && out, "ps_radial_sum: NULL argument");
+    PS_REQUIRE(nr >= 1 && npol >= 1 && npol <= nslot, "ps_radial_sum: bad nr/npol");
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned nb = (unsigned)((nlm + 255) / 256);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_radial_sum<float>, dim3(nb), dim3(256), 0, st, (const cplx<float>*)lm_map, nr, lmax, lmin,
+                           (const float*)f, nslot, npol, w, scale, accumulate, (cplx<float>*)out);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_radial_sum<double>, dim3(nb), dim3(256), 0, st, (const cplx<double>*)lm_map, nr, lmax,
+                           lmin, (const double*)f, nslot, npol, w, scale, accumulate, (cplx<double>*)out);
+    else
+        PS_REQUIRE(false, "ps_radial_sum: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_tau_sum_n0(int precision, const void* A, const void* B, int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
+                  const double* FLL, const double* w, int n1, double* scratch, double* out, int accumulate,
+                  void* stream) {
+    PS_REQUIRE(A && B && FLL && w && scratch && out, "ps_tau_sum_n0: NULL argument");
+    PS_REQUIRE(Lmin >= 0 && Lmax >= Lmin && Lmax <= Lmaxs, "ps_tau_sum_n0: need 0 <= Lmin <= Lmax <= Lmaxs");
+    PS_REQUIRE(Lmax - Lmin + 1 <= RED_BLOCKS * 64, "ps_tau_sum_n0: L range too large for the scratch");
+    hipStream_t st = (hipStream_t)stream;
+    const int nL = Lmax - Lmin + 1;
+    const double pref = 1.0 / (sqrt(4.0 * M_PI) * sqrt(2.0 * n1 + 1.0));
+    if (precision == 32)
+        hipLaunchKernelGGL(k_tau_gram<float>, dim3(nL), dim3(256), 0, st, (const cplx<float>*)A, (const cplx<float>*)B,
+                           nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_tau_gram<double>, dim3(nL), dim3(256), 0, st, (const cplx<double>*)A,
+                           (const cplx<double>*)B, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch);
+    else
+        PS_REQUIRE(false, "ps_tau_sum_n0: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)scratch, nL, out, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_outer_product(int precision, const void* Qa, const void* Qb, int nta, int ntb, long long nin, double* scratch,
+                     double* fish, int accumulate, void* stream) {
+    PS_REQUIRE(Qa && Qb && scratch && fish, "ps_outer_product: NULL argument");
+    PS_REQUIRE(nta >= 1 && ntb >= 1 && nta * ntb <= 64, "ps_outer_product: need 1 <= nta*ntb <= 64");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(RED_BLOCKS, nta * ntb);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_outer<float>, grid, dim3(256), 0, st, (const cplx<float>*)Qa, (const cplx<float>*)Qb, nta,
+                           ntb, (long)nin, scratch);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_outer<double>, grid, dim3(256), 0, st, (const cplx<double>*)Qa, (const cplx<double>*)Qb,
+                           nta, ntb, (long)nin, scratch);
+    else
+        PS_REQUIRE(false, "ps_outer_product: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_rows, dim3(nta * ntb), dim3(256), 0, st, (const double*)scratch, nta * ntb, RED_BLOCKS,
+                       fish, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,40 @@
+// Internal interface of the HEALPix SHT kernels (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace ps {
+
+struct RingInfo {
+    long start;  // first pixel of the ring (RING order)
+    int nphi;    // pixels in the ring
+    int phi_p;   // phi0 = pi * phi_p / phi_q
+    int phi_q;
+    int M;       // FFT length (nphi if a power of two, else the Bluestein length)
+    int logM;
+    int voff;    // offset of the Bluestein kernel in the table, -1 for power-of-two rings
+};
+
+struct ShtPlan {
+    int nside = 0, lmax = 0, mmax = 0, spin = 0, prec = 4;  // prec: bytes per real (4 or 8)
+    int nrings = 0, npairs = 0, ncomp = 1;
+    long npix = 0, nlm = 0, maxM = 1;
+    int device = 0;
+    RingInfo* d_rings = nullptr;
+    double *d_l2s = nullptr, *d_l2sh = nullptr, *d_l2ch = nullptr, *d_seedL = nullptr, *d_seedS = nullptr;
+    int *d_seedA = nullptr, *d_seedB = nullptr;
+    void *d_cth = nullptr, *d_tw = nullptr, *d_blue = nullptr, *d_coef = nullptr;
+    int build();
+    ~ShtPlan();
+};
+
+int sht_synth_lmajor_ld_multiple(const ShtPlan& p);
+size_t sht_phase_bytes(const ShtPlan& p, int nmap);
+int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
+                         void* phase, hipStream_t st);
+int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
+                      hipStream_t st);
+int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, void* alm, long alm_stride,
+                double scale, void* phase, hipStream_t st);
+
+}  // namespace ps
