@@ -1,0 +1,278 @@
+#!/usr/bin/env python
+"""Benchmark: gNL-loc numerators/s at Nside 512 on one MI355X.
+
+Workload (the Nside=512 T+E configuration): gNL-loc numerator, T+E (TQU data),
+Nside=512, lmin=2, lmax=1024, N_r=40 Gauss-Legendre radial nodes on
+[13400, 14600] Mpc, synthetic transfer T_l(k) = j_l(14000 k) on 200 log-spaced
+k in [1e-4, 0.2]/Mpc, C_l^TT = 2 pi 1e3/(l(l+1)) with a 30' beam and
+30 uK-arcmin noise (EE = 0.05 TT, TE 0.3 correlated), a |b| < 20 deg cut plus
+200 seeded 0.5 deg holes apodized over 1 deg, ideal harmonic S^-1, 100
+Monte-Carlo simulation pairs.
+
+A step is one Tl_numerator(data) call with the data map resident in HBM:
+S^-1 (TQU -> TEB adjoint at lmax 1535) -> 80 radial-node syntheses (one
+batched launch) -> fused t4 + t2 reduction against the collapsed Monte-Carlo
+sums (t0 cached, as the reference caches it).  value = numerators/s of this
+batched FP32 path.  Also reported: the plain per-simulation FP64 GPU path (the
+reference's algorithm, simulations streamed as with preload=False), the
+amplitude difference between the two in units of the numerator scatter, the
+Legendre-kernel roofline, and the FP64 CPU oracle on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gNL-loc numerators/s at Nside 512 (batched vs plain GPU); CG iters to 1e-6"
+PEAK_FP32 = 157.3e12
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spectra(nside, fwhm=30., noise=30.):
+    L = 3 * nside - 1
+    l = np.arange(L + 1)
+    cl = np.zeros(L + 1)
+    cl[2:] = 2 * np.pi * 1e3 / (l[2:] * (l[2:] + 1.))
+    sig = np.radians(fwhm / 60.) / np.sqrt(8 * np.log(2))
+    b = np.exp(-l * (l + 1) * sig ** 2 / 2)
+    nl = (noise * np.pi / 10800.) ** 2 * np.ones(L + 1)
+    raw = {'TT': cl, 'EE': 0.05 * cl, 'TE': 0.3 * np.sqrt(0.05) * cl, 'BB': 0. * cl}
+    tot = {'TT': cl * b ** 2 + nl, 'EE': 0.05 * cl * b ** 2 + 2 * nl, 'TE': 0.3 * np.sqrt(0.05) * cl * b ** 2,
+           'BB': 2 * nl}
+    return raw, tot, [b, b]
+
+
+def pixel_z_phi(nside):
+    """z = cos(theta) and phi of every RING pixel (vectorised HEALPix geometry)."""
+    N = nside
+    npix = 12 * N * N
+    z = np.empty(npix)
+    ph = np.empty(npix)
+    p = 0
+    for i in range(1, 4 * N):
+        ir = min(i, 4 * N - i)
+        if ir < N:
+            zz, n, p0 = 1 - ir * ir / (3. * N * N), 4 * ir, np.pi / (4. * ir)
+        else:
+            zz, n = 2. * (2 * N - ir) / (3. * N), 4 * N
+            p0 = np.pi / (4. * N) if ((ir - N) & 1) == 0 else 0.
+        if i > 2 * N:
+            zz = -zz
+        z[p:p + n] = zz
+        ph[p:p + n] = p0 + 2 * np.pi * np.arange(n) / n
+        p += n
+    return z, ph
+
+
+def synthetic_mask(nside, nholes=200, seed=1, cut_deg=20., rhole_deg=0.5, apo_deg=1.):
+    z, ph = pixel_z_phi(nside)
+    b = np.degrees(np.arcsin(np.clip(z, -1, 1)))
+    d = np.abs(b) - cut_deg
+    m = np.clip(d / apo_deg, 0, 1)
+    m = np.where(d <= 0, 0., 0.5 * (1 - np.cos(np.pi * m)))
+    rng = np.random.default_rng(seed)
+    zc = rng.uniform(-1, 1, nholes)
+    pc = rng.uniform(0, 2 * np.pi, nholes)
+    st = np.sqrt(1 - z * z)
+    for zi, pi in zip(zc, pc):
+        cosd = z * zi + st * np.sqrt(1 - zi * zi) * np.cos(ph - pi)
+        near = cosd > np.cos(np.radians(rhole_deg + apo_deg))
+        dd = np.degrees(np.arccos(np.clip(cosd[near], -1, 1))) - rhole_deg
+        taper = np.where(dd <= 0, 0., 0.5 * (1 - np.cos(np.pi * np.clip(dd / apo_deg, 0, 1))))
+        m[near] *= taper
+    return m
+
+
+def build(nside, lmax, nr, precision, batch, nk=200):
+    from scipy.special import roots_legendre, spherical_jn
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.tspec_template import TSpecTemplate
+    from polyspec_amd.weighting import Weightings
+    raw, tot, beam = spectra(nside)
+    base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=precision)
+    base.grf_device = True
+    k = np.geomspace(1e-4, 0.2, nk)
+    Tl = np.zeros((lmax + 1, nk))
+    for l in range(2, lmax + 1):
+        Tl[l] = spherical_jn(l, 14000. * k)
+    x, wg = roots_legendre(nr)
+    r = 600. * x + 14000.
+    w = r ** 2 * wg * 600.
+    return base, raw, tot, (k, np.asarray([Tl, Tl]), r, w)
+
+
+def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None):
+    from polyspec_amd.tspec_template import TSpecTemplate
+    from polyspec_amd.weighting import Weightings
+    k, Tl, r, w = tables
+    W = Weightings(base, mask, dict(raw))
+    return TSpecTemplate(base, mask, W.applySinv_ideal, ['gNL-loc'], k, Tl, 2, lmax, r_values=r,
+                         r_weights={'gNL-loc': w}, batch_sims=batch), W
+
+
+def cpu_baseline(data_np, base, tm, nr_sample, lmax):
+    """FP64 oracle on the host: S^-1 of the data plus `nr_sample` of the 40 radial
+    nodes (2 syntheses each + gnl sums), extrapolated to one numerator."""
+    import oracle
+    import torch
+    ncores = len(os.sched_getaffinity(0))
+    nside = base.Nside
+    L = base.lmax
+    oracle.lib()
+    t0 = time.perf_counter()
+    alm = oracle.to_lm(data_np, nside, L)
+    lf = base.l_arr <= lmax
+    ll = base.l_arr[lf]
+    h = base.beam[:, ll] * np.einsum('abl,bl->al', base.inv_Cl_tot_mat[:, :, ll], alm[:, lf])
+    t_sinv = time.perf_counter() - t0
+    h = np.ascontiguousarray(h[:2])
+    ls = ll.astype(np.int32)
+    fP = np.ascontiguousarray(tm.plLXs[0][:, :, :nr_sample])
+    fQ = np.ascontiguousarray(tm.qlXs[:, :, :nr_sample])
+    t0 = time.perf_counter()
+    summP = oracle.apply_fl_weights(fP, h, ls, 1. / np.sqrt(4 * np.pi))
+    summQ = oracle.apply_fl_weights(fQ, h, ls, 1.)
+    P = oracle.to_map_vec(summP, nside, lmax)
+    Q = oracle.to_map_vec(summQ, nside, lmax)
+    s = oracle.gnl_loc_sum(tm.r_weights['gNL-loc'][:nr_sample], P, Q)
+    t_nodes = time.perf_counter() - t0
+    per_num = t_sinv + (tm.N_r / nr_sample) * t_nodes
+    return {"value": 1.0 / per_num, "unit": "numerators/s", "cores": ncores, "kind": "port",
+            "sample": ("FP64 C/OpenMP oracle on %d host threads: S^-1 of one TQU data map at lmax %d "
+                       "(%.2f s) + %d of %d radial nodes (2 syntheses each at lmax %d + gnl sums, %.2f s), "
+                       "extrapolated linearly in nodes to one numerator (%.1f s)") %
+                      (ncores, L, t_sinv, nr_sample, tm.N_r, lmax, t_nodes, per_num)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nside", type=int, default=512)
+    ap.add_argument("--lmax", type=int, default=1024)
+    ap.add_argument("--nr", type=int, default=40)
+    ap.add_argument("--nsims", type=int, default=100)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-simulation FP64 path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-nodes", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run its own "
+              "share of simulations/data maps; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    t_setup = time.time()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    nside, lmax, nr = args.nside, args.lmax, args.nr
+    log("setup: mask ...")
+    mask = synthetic_mask(nside)
+    log("setup: fp32 template (k-integrals) ...")
+    base32, raw, tot, tables = build(nside, lmax, nr, 32, True)
+    tm32, W32 = make_template(base32, raw, mask, tables, lmax, True)
+    log("setup: %.1fs; generating %d simulation pairs (batched) ..." % (time.time() - t_setup, args.nsims))
+    t1 = time.time()
+    tm32.generate_sims(args.nsims, preload=True)
+    torch.cuda.synchronize()
+    t_mc = time.time() - t1
+    log("  MC precompute %.1fs" % t_mc)
+
+    # data maps resident in HBM (masked TQU GRFs)
+    ndata = max(args.steps, 8)
+    mask_d = torch.from_numpy(mask).to(dev, torch.float32)
+    datas = []
+    for i in range(ndata):
+        m = base32.generate_data(seed=int(7e6) + i)
+        datas.append((m * mask_d).contiguous())
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        tm32.Tl_numerator(datas[i % ndata])
+    torch.cuda.synchronize()
+    tm32.kernel_events = []
+    vals = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        vals.append(tm32.Tl_numerator(datas[i % ndata])[0])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ev = tm32.kernel_events
+    tm32.kernel_events = None
+    ms_step = dt / args.steps * 1e3
+    value = args.steps / dt
+    leg_ms = np.mean([e0.elapsed_time(e1) for _, e0, e1 in ev])
+    ncol = ev[0][0]
+    R = 2 * nside
+    nlm = (lmax + 1) * (lmax + 2) // 2
+    flops = 4. * R * nlm * ncol
+    achieved = flops / (leg_ms * 1e-3)
+    log("batched fp32: %.2f ms/numerator, %.1f numerators/s; Legendre kernel %.3f ms (%.1f TFLOP/s)" %
+        (ms_step, value, leg_ms, achieved / 1e12))
+    vals = np.array(vals)
+    sigma = float(np.std(vals[:ndata])) if args.steps >= 4 else None
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "numerators/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "gNL-loc numerator, T+E TQU data, Nside=%d, lmin=2, lmax=%d, N_r=%d, %d MC sim pairs "
+                               "(collapsed), |b|<20deg + 200x0.5deg holes apodized 1deg, ideal S^-1; "
+                               "batched FP32 path (Legendre recursion FP64)" % (nside, lmax, nr, args.nsims),
+                   "nside": nside, "lmax": lmax, "n_r": nr, "n_sim_pairs": args.nsims},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_leg_synth_s0<float,16> (Legendre synthesis, %d maps per launch, %.3f ms avg; "
+                               "VALU FP32 FMAs, priced against the FP32 peak)" % (ncol, leg_ms),
+                     "algorithmic_flops_per_launch": flops},
+        "mc_precompute_s": t_mc,
+    }
+
+    if not args.no_plain:
+        log("plain per-simulation FP64 path ...")
+        base64, raw64, tot64, tables64 = build(nside, lmax, nr, 64, False)
+        tm64, _ = make_template(base64, raw64, mask, tables64, lmax, False)
+        tm64.generate_sims(args.nsims, preload=False)
+        d64 = datas[0].to(torch.float64)
+        t1 = time.perf_counter()
+        ref = tm64.Tl_numerator(d64)[0]  # first call also computes t0
+        torch.cuda.synchronize()
+        t_first = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        ref2 = tm64.Tl_numerator(d64)[0]  # steady state: t0 cached
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t1
+        diff = abs(vals[0] - ref2)
+        result["plain_fp64"] = {"numerators_per_s": 1. / t_plain, "s_per_numerator": t_plain,
+                                "s_first_call_with_t0": t_first, "numerator": ref2,
+                                "batched_fp32_numerator": float(vals[0]),
+                                "speedup_batched_over_plain": value * t_plain}
+        if sigma:
+            result["plain_fp64"]["abs_diff_over_numerator_scatter"] = diff / sigma
+            result["plain_fp64"]["numerator_scatter"] = sigma
+        log("plain fp64: %.2f s/numerator; speedup %.0fx; |diff|/scatter %s" %
+            (t_plain, value * t_plain, (diff / sigma) if sigma else None))
+        del tm64, base64
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu:
+        log("CPU oracle baseline ...")
+        dnp = datas[0].to(torch.float64).cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(dnp, base32, tm32, args.cpu_nodes, lmax)
+        log("cpu: %s" % result["cpu_baseline"]["sample"])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

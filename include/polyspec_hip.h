@@ -67,10 +67,113 @@ int ps_sht_synthesis(const ps_sht_plan *plan, int nmaps, const void *alm, void *
 int ps_sht_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_t, int ld, void *map,
                             void *work, long long work_bytes, void *stream);
 
+/* The two stages of ps_sht_synthesis_lmajor, for callers that schedule or time them
+ * separately: the Legendre stage writes ring phases F_m(ring) laid out
+ * [nmaps*ncomp][mmax+1][4 nside - 1] complex (ps_sht_phase_bytes bytes), the ring
+ * stage turns them into maps by per-ring FFTs. */
+int ps_sht_phase_bytes(const ps_sht_plan *plan, int nmaps, long long *bytes);
+int ps_sht_legendre_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_t, int ld, void *phase,
+                                     void *stream);
+int ps_sht_ring_synthesis(const ps_sht_plan *plan, int nmaps, const void *phase, void *map, void *stream);
+
 /* map -> alm: scale * sum_p map_p sY*_lm(p) (ducc0.sht.adjoint_synthesis times
  * `scale`; PolySpec passes scale = pi/(3 nside^2) = 4 pi/Npix, base.py:147-151, 193, 206, 253). */
 int ps_sht_adjoint(const ps_sht_plan *plan, int nmaps, const void *map, void *alm, double scale, void *work,
                    long long work_bytes, void *stream);
+
+
+/* ------------------------------------------------------- radial kernels
+ * n = 0 loop nests of polyspec/cython/tauNL_utils.pyx and ideal_fisher.pyx.
+ * `precision` (32 | 64) selects float/complex64 or double/complex128 data;
+ * weights w and F tables are always double.  Reductions write one double to
+ * the device pointer `out` (added to it when accumulate != 0) and need a
+ * device scratch of ps_reduce_scratch_bytes() bytes.
+ * Radial-profile tables f are laid out (lmax+1, nslot, nr) as plLXs[nmax] /
+ * qlXs in the reference (tspec_template.py:623-630). */
+long long ps_reduce_scratch_bytes(void);
+
+/* tauNL_utils.apply_fl_weights (tauNL_utils.pyx:80-98) fused with the zero fill
+ * below lmin of TSpecTemplate._compute_weighted_maps (tspec_template.py:799-808),
+ * emitting the l-major synthesis input of ps_sht_synthesis_lmajor:
+ * out_t[idx(l,m)*ld + col_offset + r] = weight * sum_{X<npol} f[l,X,r] h[X,idx(l,m)].
+ * h: (npol, nlm(lmax)) complex. */
+int ps_weight_alm_lmajor(int precision, const void *f, int nslot, int nr, const void *h, int npol, int lmax,
+                         int lmin, double weight, int ld, int col_offset, void *out_t, void *stream);
+
+/* c4 * tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270) plus, when U and V are
+ * given, c2 * sum_r w_r sum_p (P^2 U + P Q V): with U = sum_s P_s Q_s and
+ * V = sum_s P_s^2 this equals c2 * sum_s gnl_loc_disc_sum(w, P, P_s, Q, Q_s)
+ * (tauNL_utils.pyx:275-285, summed at tspec_template.py:1565-1575). */
+int ps_gnl_loc_fused_sum(int precision, const void *P, const void *Q, const void *U, const void *V,
+                         const double *w, int nr, long long npix, double c4, double c2, double *scratch,
+                         double *out, int accumulate, void *stream);
+
+/* tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285). */
+int ps_gnl_loc_disc_sum(int precision, const void *P1, const void *P2, const void *Q1, const void *Q2,
+                        const double *w, int nr, long long npix, double *scratch, double *out, int accumulate,
+                        void *stream);
+
+/* U += sum_s P_s Q_s, V += sum_s P_s^2 over nsim stacked (P_s, Q_s) of n values
+ * (the Monte-Carlo sums that collapse the per-simulation loop of
+ * tspec_template.py:1542-1580). */
+int ps_mc_accumulate(int precision, const void *P, const void *Q, int nsim, long long n, void *U, void *V,
+                     void *stream);
+
+/* Elementwise map products times scale: mode 0 a*b (ideal_fisher.multiplyRR,
+ * ideal_fisher.pyx:100-108), 1 a*b + c*d (multiplyRR_sym, 112-120),
+ * 2 a*b*c (tauNL_utils.multiply, tauNL_utils.pyx:103-111),
+ * 3 a*(2 b c + a d) (tauNL_utils.multiply_asym, 116-124). */
+int ps_pixel_products(int precision, int mode, const void *a, const void *b, const void *c, const void *d,
+                      long long n, double scale, void *out, void *stream);
+
+/* tauNL_utils.radial_sum (tauNL_utils.pyx:220-235) over the full m-major set,
+ * zero below lmin: out[X, lm] (+)= scale sum_r lm_map[r, lm] f[l,X,r] w_r, X < npol. */
+int ps_radial_sum(int precision, const void *lm_map, int nr, int lmax, int lmin, const void *f, int nslot, int npol,
+                  const double *w, double scale, int accumulate, void *out, void *stream);
+
+/* tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646) as per-L Gram matrices.
+ * A, B: (nmu, nr, nlm(Lmaxs)) complex m-major over L = 0..Lmaxs (the transpose
+ * of the reference's (nmu, nLM, nr) layout); FLL (Lmaxs+1, nr, nr) = FLLs[nmax_F]. */
+int ps_tau_sum_n0(int precision, const void *A, const void *B, int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
+                  const double *FLL, const double *w, int n1, double *scratch, double *out, int accumulate,
+                  void *stream);
+
+/* ideal_fisher.outer_product (ideal_fisher.pyx:423-459, sym = 0): Qa (4, nta, nin),
+ * Qb (4, ntb, nin) complex -> fish (nta, ntb) double (added when accumulate != 0). */
+int ps_outer_product(int precision, const void *Qa, const void *Qb, int nta, int ntb, long long nin, double *scratch,
+                     double *fish, int accumulate, void *stream);
+
+/* tauNL_utils.convolve_F_n000 (tauNL_utils.pyx:966-992, all radial nodes) as used by
+ * TSpecTemplate._convolve_F (tspec_template.py:2155-2162), written l-major for the
+ * following synthesis: out_t[idx(L,M)*ld + j] = sum_{k: w_k != 0} F_L(j,k) w_k PQ[k, idx(L,M)] / sqrt(4 pi)
+ * for Lmin <= L <= Lmax, else 0.  PQ: (nr, nlm(Lmaxs)) complex; FLL (Lmaxs+1, nr, nr). */
+int ps_convolve_F_lmajor(int precision, const void *PQ, int nr, int Lmaxs, int Lmin, int Lmax, const double *FLL,
+                         const double *w, int ld, void *out_t, void *stream);
+
+/* ------------------------------------------------- S^-1 filter vectors
+ * Harmonic vectors are (nf, nlm(lmax)) complex (nf = 1 or 3: T or T,E,B). */
+
+/* out[a] (+)= alpha * post[a,l] sum_b mat[a,b,l] pre[b,l] in[b] for every (l, m):
+ * the per-l 3x3 products of weighting.py (_apply_Cinv 279-281, _apply_M harmonic
+ * 304-305, x0 174, applySinv_ideal 200, 214) and PolySpec.applyAinv (base.py:444).
+ * mat (nf,nf,lmax+1), pre/post (nf,lmax+1) double, any of them NULL = identity.
+ * In-place (in == out) is allowed. Row strides in complex elements. */
+int ps_lmat_apply(int precision, int nf, int lmax, const double *mat, const double *pre, const double *post,
+                  const void *in, long long in_stride, void *out, long long out_stride, double alpha, int accumulate,
+                  void *stream);
+
+/* CGD.dot (weighting.py:255): Re sum x conj(y) (1 + [m > 0]) -> *out (device double).
+ * scratch: ps_mdot_scratch_bytes() bytes. */
+long long ps_mdot_scratch_bytes(void);
+int ps_mdot(int precision, int nf, int lmax, const void *x, const void *y, double *scratch, double *out,
+            void *stream);
+
+/* CGD.step updates (weighting.py:329-339) with device scalars:
+ * x += (rz/pAp) p, r -= (rz/pAp) Ap;   p = z + (num/den) p. n = complex elements. */
+int ps_cg_update_xr(int precision, long long n, const double *rz, const double *pAp, const void *p, const void *Ap,
+                    void *x, void *r, void *stream);
+int ps_cg_update_p(int precision, long long n, const double *num, const double *den, const void *z, void *p,
+                   void *stream);
 
 #ifdef __cplusplus
 }

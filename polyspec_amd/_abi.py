@@ -27,6 +27,24 @@ SIGNATURES = {
     "ps_sht_synthesis": (_I, [_P, _I, _P, _P, _P, _LL, _P]),
     "ps_sht_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P, _LL, _P]),
     "ps_sht_adjoint": (_I, [_P, _I, _P, _P, _D, _P, _LL, _P]),
+    "ps_sht_phase_bytes": (_I, [_P, _I, ctypes.POINTER(_LL)]),
+    "ps_sht_legendre_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P]),
+    "ps_sht_ring_synthesis": (_I, [_P, _I, _P, _P, _P]),
+    "ps_reduce_scratch_bytes": (_LL, []),
+    "ps_weight_alm_lmajor": (_I, [_I, _P, _I, _I, _P, _I, _I, _I, _D, _I, _I, _P, _P]),
+    "ps_gnl_loc_fused_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _D, _D, _P, _P, _I, _P]),
+    "ps_gnl_loc_disc_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _P, _P, _I, _P]),
+    "ps_mc_accumulate": (_I, [_I, _P, _P, _I, _LL, _P, _P, _P]),
+    "ps_pixel_products": (_I, [_I, _I, _P, _P, _P, _P, _LL, _D, _P, _P]),
+    "ps_radial_sum": (_I, [_I, _P, _I, _I, _I, _P, _I, _I, _P, _D, _I, _P, _P]),
+    "ps_tau_sum_n0": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
+    "ps_outer_product": (_I, [_I, _P, _P, _I, _I, _LL, _P, _P, _I, _P]),
+    "ps_convolve_F_lmajor": (_I, [_I, _P, _I, _I, _I, _I, _P, _P, _I, _P, _P]),
+    "ps_lmat_apply": (_I, [_I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _D, _I, _P]),
+    "ps_mdot_scratch_bytes": (_LL, []),
+    "ps_mdot": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    "ps_cg_update_xr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
+    "ps_cg_update_p": (_I, [_I, _LL, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -142,6 +160,25 @@ class ShtPlan:
         ws = workspace(wb, alm_t.device)
         check(load().ps_sht_synthesis_lmajor(self._h, int(nmaps), ptr(alm_t), ld, ptr(out), ptr(ws), wb,
                                              stream_handle()))
+        return out
+
+    def phase_buffer(self, nmaps):
+        n = _LL()
+        check(load().ps_sht_phase_bytes(self._h, int(nmaps), ctypes.byref(n)))
+        return torch.empty(n.value, dtype=torch.uint8, device=self.device)
+
+    def legendre_synthesis_lmajor(self, alm_t, nmaps, phase):
+        """Legendre stage only: alm_t (nlm, ld) -> ring phases (uint8 buffer of phase_buffer())."""
+        require_device(alm_t, self.cdtype, "alm_t")
+        check(load().ps_sht_legendre_synthesis_lmajor(self._h, int(nmaps), ptr(alm_t), alm_t.shape[1], ptr(phase),
+                                                      stream_handle()))
+        return phase
+
+    def ring_synthesis(self, phase, nmaps, out=None):
+        """Ring-FFT stage only: phases -> maps (nmaps, ncomp, npix)."""
+        if out is None:
+            out = torch.empty((nmaps, self.ncomp, self.npix), dtype=self.rdtype, device=phase.device)
+        check(load().ps_sht_ring_synthesis(self._h, int(nmaps), ptr(phase), ptr(out), stream_handle()))
         return out
 
     def adjoint(self, maps, scale, out=None):

@@ -121,6 +121,27 @@ int ps_sht_synthesis_lmajor(const ps_sht_plan* plan, int nmaps, const void* alm_
     return sht_synthesis_lmajor(p, nmaps, alm_t, ld, map, p.npix, work, (hipStream_t)stream);
 }
 
+int ps_sht_phase_bytes(const ps_sht_plan* plan, int nmaps, long long* bytes) {
+    PS_REQUIRE(plan && bytes && nmaps >= 1, "ps_sht_phase_bytes: bad argument");
+    *bytes = (long long)sht_phase_bytes(plan->p, nmaps);
+    return 0;
+}
+
+int ps_sht_legendre_synthesis_lmajor(const ps_sht_plan* plan, int nmaps, const void* alm_t, int ld, void* phase,
+                                     void* stream) {
+    PS_REQUIRE(plan && alm_t && phase, "ps_sht_legendre_synthesis_lmajor: NULL argument");
+    const ShtPlan& p = plan->p;
+    int mult = sht_synth_lmajor_ld_multiple(p);
+    PS_REQUIRE(nmaps >= 1 && ld >= nmaps * p.ncomp && ld % mult == 0,
+               "ps_sht_legendre_synthesis_lmajor: bad nmaps / ld");
+    return sht_legendre_synthesis_lmajor(p, nmaps, alm_t, ld, phase, (hipStream_t)stream);
+}
+
+int ps_sht_ring_synthesis(const ps_sht_plan* plan, int nmaps, const void* phase, void* map, void* stream) {
+    PS_REQUIRE(plan && phase && map && nmaps >= 1, "ps_sht_ring_synthesis: bad argument");
+    return sht_ring_synthesis(plan->p, nmaps, phase, map, plan->p.npix, (hipStream_t)stream);
+}
+
 int ps_sht_adjoint(const ps_sht_plan* plan, int nmaps, const void* map, void* alm, double scale, void* work,
                    long long work_bytes, void* stream) {
     PS_REQUIRE(plan && alm && map && work, "ps_sht_adjoint: NULL argument");

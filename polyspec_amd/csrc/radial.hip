@@ -42,27 +42,27 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 // synthesis input: out_t[idx(l,m) * ld + r] = w sum_{X<npol} f[l, X, r] h[X, idx(l,m)]
 // for lmin <= l <= lmax, zero otherwise (and zero in padding columns r >= nr).
 // f: (lmax_f+1, nslot, nr) real; h: (npol, nlm) complex (full m-major set).
+__device__ __forceinline__ int l_of_lm(long lm, int lmax) {
+    // m-major index lm = m(2 lmax + 1 - m)/2 + l  ->  l
+    const double B = 2.0 * lmax + 3.0;
+    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
+    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
+    while (m < lmax && (long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm) m++;
+    return (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+}
+
 template <typename T>
 __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, const cplx<T>* __restrict__ h,
                                     int npol, int lmax, int lmin, T w, int ld, cplx<T>* __restrict__ out,
                                     int col_offset) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlm * ld) return;
-    const long lm = i / ld;
-    const int c = (int)(i % ld) - col_offset;
-    if (c < 0) return;  // columns owned by another family
-    if (c >= nr) {
-        if (col_offset == 0 || c < ld) { /* padding */ }
-    }
-    // recover l from lm (m-major): m = largest with m(2lmax+1-m)/2 <= lm
-    const double B = 2.0 * lmax + 3.0;
-    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
-    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
-    while ((long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm && m < lmax) m++;
-    const int l = (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+    if (i >= nlm * nr) return;
+    const long lm = i / nr;
+    const int c = (int)(i % nr);
+    const int l = l_of_lm(lm, lmax);
     cplx<T> v{T(0), T(0)};
-    if (c < nr && l >= lmin) {
+    if (l >= lmin) {
         const cplx<T> h0 = h[lm];
         const T f0 = f[((long)l * nslot + 0) * nr + c];
         v.re = f0 * h0.re;
@@ -76,7 +76,7 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
         v.re *= w;
         v.im *= w;
     }
-    if (c < nr || col_offset == 0) out[lm * ld + col_offset + c] = v;
+    out[lm * ld + col_offset + c] = v;
 }
 
 // ------------------------------------------------------------- gNL-loc sums
@@ -173,11 +173,7 @@ __global__ void k_radial_sum(const cplx<T>* __restrict__ lmmap, int nr, int lmax
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long lm = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (lm >= nlm) return;
-    const double B = 2.0 * lmax + 3.0;
-    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
-    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
-    while ((long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm && m < lmax) m++;
-    const int l = (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+    const int l = l_of_lm(lm, lmax);
     for (int X = 0; X < npol; X++) {
         double re = 0.0, im = 0.0;
         if (l >= lmin) {
@@ -271,6 +267,38 @@ __global__ void k_sum_rows(const double* __restrict__ part, int nrow, int ncolp,
     if (threadIdx.x == 0) out[row] = accumulate ? out[row] + s : s;
 }
 
+
+// ------------------------------------------------------ F_L convolution
+// tauNL_utils.convolve_F_n000 (tauNL_utils.pyx:966-992, inds = all nodes) feeding
+// TSpecTemplate._convolve_F (tspec_template.py:2155-2162), written as the
+// l-major synthesis input of the following to_map_vec:
+// out_t[lm * ld + j] = sum_{k: w_k != 0} F_L(j, k) w_k PQ[k, lm] / sqrt(4 pi)
+// for Lmin <= L <= Lmax, else 0.  PQ: (nr, nlm(Lmaxs)) complex; FLL (Lmaxs+1, nr, nr).
+template <typename T>
+__global__ void k_convolve_F(const cplx<T>* __restrict__ PQ, int nr, int Lmaxs, int Lmin, int Lmax,
+                             const double* __restrict__ FLL, const double* __restrict__ w, int ld,
+                             cplx<T>* __restrict__ out) {
+    const long nlm = (long)(Lmaxs + 1) * (Lmaxs + 2) / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlm * nr) return;
+    const long lm = i / nr;
+    const int j = (int)(i % nr);
+    const int L = l_of_lm(lm, Lmaxs);
+    double re = 0.0, im = 0.0;
+    if (L >= Lmin && L <= Lmax) {
+        const double* F = FLL + ((long)L * nr + j) * nr;
+        for (int k = 0; k < nr; k++) {
+            if (w[k] == 0.0) continue;
+            const double fw = F[k] * w[k];
+            const cplx<T> v = PQ[(long)k * nlm + lm];
+            re += fw * (double)v.re;
+            im += fw * (double)v.im;
+        }
+    }
+    const double c = 0.28209479177387814;  // 1/sqrt(4 pi)
+    out[lm * ld + j] = cplx<T>{(T)(re * c), (T)(im * c)};
+}
+
 }  // namespace ps
 
 using namespace ps;
@@ -280,7 +308,7 @@ template <typename T>
 static int weight_alm_impl(const void* f, int nslot, int nr, const void* h, int npol, int lmax, int lmin, double w,
                            int ld, void* out, int col_offset, void* stream) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    const long tot = nlm * ld;
+    const long tot = nlm * nr;
     hipLaunchKernelGGL(k_weight_alm_lmajor<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, (const T*)f, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)w, ld,
                        (cplx<T>*)out, col_offset);
@@ -447,4 +475,25 @@ int ps_outer_product(int precision, const void* Qa, const void* Qb, int nta, int
     return 0;
 }
 
+int ps_convolve_F_lmajor(int precision, const void* PQ, int nr, int Lmaxs, int Lmin, int Lmax, const double* FLL,
+                         const double* w, int ld, void* out_t, void* stream) {
+    PS_REQUIRE(PQ && FLL && w && out_t, "ps_convolve_F_lmajor: NULL argument");
+    PS_REQUIRE(nr >= 1 && ld >= nr, "ps_convolve_F_lmajor: need ld >= nr >= 1");
+    const long nlm = (long)(Lmaxs + 1) * (Lmaxs + 2) / 2;
+    const long tot = nlm * nr;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned nb = (unsigned)((tot + 255) / 256);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_convolve_F<float>, dim3(nb), dim3(256), 0, st, (const cplx<float>*)PQ, nr, Lmaxs, Lmin,
+                           Lmax, FLL, w, ld, (cplx<float>*)out_t);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_convolve_F<double>, dim3(nb), dim3(256), 0, st, (const cplx<double>*)PQ, nr, Lmaxs,
+                           Lmin, Lmax, FLL, w, ld, (cplx<double>*)out_t);
+    else
+        PS_REQUIRE(false, "ps_convolve_F_lmajor: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 }  // extern "C"
+

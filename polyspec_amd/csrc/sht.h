@@ -32,6 +32,9 @@ int sht_synth_lmajor_ld_multiple(const ShtPlan& p);
 size_t sht_phase_bytes(const ShtPlan& p, int nmap);
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
                          void* phase, hipStream_t st);
+int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
+                                  hipStream_t st);
+int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st);
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
                       hipStream_t st);
 int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, void* alm, long alm_stride,

@@ -1013,6 +1013,22 @@ int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, 
     return launch_ring_synth<double>(p, ncol, phase, map, map_stride, st);
 }
 
+int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
+                                  hipStream_t st) {
+    const int ncol = nmap * p.ncomp;
+    if (p.prec == 4)
+        return p.spin == 0 ? launch_synth_s0<float, SYN_BC_S0_F>(p, ncol, ld, alm_t, phase, st)
+                           : launch_synth_sp<float, SYN_BC_SP_F>(p, nmap, ld, alm_t, phase, st);
+    return p.spin == 0 ? launch_synth_s0<double, SYN_BC_S0_D>(p, ncol, ld, alm_t, phase, st)
+                       : launch_synth_sp<double, SYN_BC_SP_D>(p, nmap, ld, alm_t, phase, st);
+}
+
+int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st) {
+    const int ncol = nmap * p.ncomp;
+    if (p.prec == 4) return launch_ring_synth<float>(p, ncol, phase, map, map_stride, st);
+    return launch_ring_synth<double>(p, ncol, phase, map, map_stride, st);
+}
+
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
                       hipStream_t st) {
     long tot = p.nlm * (long)ld;

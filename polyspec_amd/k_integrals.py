@@ -1,0 +1,84 @@
+"""Host (FP64) precompute of the radial profiles used by the local templates.
+
+Restates polyspec/cython/k_integrals.pyx for the n = 0 (local) case:
+  * p_integral (k_integrals.pyx:262-309):  p_l^X(r) = (-1)^l (2/pi) int dk k^2 P_zeta(k) T^X_l(k) j_l(k r)
+  * q_integral (k_integrals.pyx:312-352):  q_l^X(r) = (-1)^l (2/pi) int dk k^2 T^X_l(k) j_l(k r)
+    (trapezoid rule on the input k grid, exactly as the Cython sums
+    sum_k (k_i - k_{i-1}) (f_i + f_{i-1}) with the 1/2 folded into kprod);
+  * compute_F_integral (k_integrals.pyx:453-494) for n = 0: the closed form
+    F_L(r, r') = pref 2^(ns-3) Gamma(L+(ns-1)/2) / (Gamma(1-(ns-2)/2) Gamma(L+3/2))
+                 2F1(L+(ns-1)/2, (ns-2)/2; L+3/2; r_<^2/r_>^2) (r_</r_>)^L r_>^(1-ns).
+Spherical Bessel functions are evaluated directly at k r with
+scipy.special.spherical_jn (the reference tabulates j_l on an x grid with GSL's
+Steed algorithm and interpolates linearly, k_integrals.pyx:28-110; direct
+evaluation removes that interpolation error).
+"""
+import numpy as np
+from scipy.special import gammaln, hyp2f1, spherical_jn
+
+
+def Pzeta(k, As=2.1e-9, ns=0.96, k_pivot=0.05):
+    """Primordial spectrum 2 pi^2/k^3 A_s (k/k_p)^(ns-1) (tspec_template.py:71)."""
+    return 2. * np.pi ** 2 / k ** 3 * As * (k / k_pivot) ** (ns - 1)
+
+
+def jl_kr(ls, k_arr, r_arr):
+    """j_l(k r) as (len(ls), len(r), len(k))."""
+    ls = np.asarray(ls)
+    x = np.asarray(r_arr)[:, None] * np.asarray(k_arr)[None, :]
+    out = np.empty((len(ls), len(r_arr), len(k_arr)))
+    for i, l in enumerate(ls):
+        out[i] = spherical_jn(int(l), x)
+    return out
+
+
+def _trapz_sum(k_arr, integrand):
+    """sum_i (k_i - k_{i-1}) (f_i + f_{i-1}) / 2 along the last axis."""
+    dk = np.diff(k_arr)
+    return np.sum(dk * (integrand[..., 1:] + integrand[..., :-1]), axis=-1) / 2.
+
+
+def q_integral(k_arr, Tl_arr, r_arr, lmin, lmax, nslot):
+    """qlXs (lmax+1, nslot, N_r): zero outside lmin <= l <= lmax and X >= len(Tl_arr)."""
+    k_arr = np.asarray(k_arr, dtype=np.float64)
+    ls = np.arange(lmin, lmax + 1)
+    jl = jl_kr(ls, k_arr, r_arr)  # (nl, nr, nk)
+    out = np.zeros((lmax + 1, nslot, len(r_arr)))
+    kp = 2. / np.pi * k_arr ** 2
+    sgn = (-1.) ** ls
+    for X in range(len(Tl_arr)):
+        f = kp[None, None, :] * np.asarray(Tl_arr[X])[lmin:lmax + 1, None, :] * jl
+        out[lmin:lmax + 1, X, :] = sgn[:, None] * _trapz_sum(k_arr, f)
+    return out
+
+
+def p_integral_n0(k_arr, Pz, Tl_arr, r_arr, lmin, lmax, lmax_arr, nslot):
+    """plLXs[nmax=0] (lmax_arr+1, nslot, N_r): zero outside lmin <= l <= lmax."""
+    k_arr = np.asarray(k_arr, dtype=np.float64)
+    ls = np.arange(lmin, lmax + 1)
+    jl = jl_kr(ls, k_arr, r_arr)
+    out = np.zeros((lmax_arr + 1, nslot, len(r_arr)))
+    kp = 2. / np.pi * k_arr ** 2 * np.asarray(Pz)
+    sgn = (-1.) ** ls
+    for X in range(len(Tl_arr)):
+        f = kp[None, None, :] * np.asarray(Tl_arr[X])[lmin:lmax + 1, None, :] * jl
+        out[lmin:lmax + 1, X, :] = sgn[:, None] * _trapz_sum(k_arr, f)
+    return out
+
+
+def F_integral_n0(r_arr, Lmin, Lmax, ns, As, k_pivot):
+    """FLLs[nmax_F = 0] (Lmax+1, N_r, N_r), zero for L < max(Lmin, 1)."""
+    r = np.asarray(r_arr, dtype=np.float64)
+    nr = len(r)
+    out = np.zeros((Lmax + 1, nr, nr))
+    pref = 2. ** (ns - 3) * 2. * np.pi ** 2 * As / k_pivot ** (ns - 1)
+    i2 = (ns - 2.) / 2.
+    rl = np.minimum(r[:, None], r[None, :])
+    rg = np.maximum(r[:, None], r[None, :])
+    ratio = rl / rg
+    for L in range(max(Lmin, 1), Lmax + 1):
+        i1 = L + (ns - 1.) / 2.
+        i3 = L + 1.5
+        pre = pref * np.exp(gammaln(i1) - gammaln(1. - i2) - gammaln(i3))
+        out[L] = pre * hyp2f1(i1, i2, i3, ratio ** 2) * ratio ** L * rg ** (1. - ns)
+    return out

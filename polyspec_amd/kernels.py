@@ -1,0 +1,220 @@
+"""Device-tensor wrappers of the radial-node kernels in libpolyspec_hip.so.
+
+Each function mirrors one Cython routine of the reference
+(polyspec/cython/tauNL_utils.pyx, ideal_fisher.pyx) but takes torch device
+tensors and enqueues on torch's current stream.  Scalar results are returned
+as 0-d float64 device tensors (no host synchronisation inside).
+"""
+import torch
+
+from . import _abi
+from ._abi import check, load, ptr, stream_handle
+
+_scratch = {}
+
+
+def _prec(t):
+    if t.dtype in (torch.float32, torch.complex64):
+        return 32
+    if t.dtype in (torch.float64, torch.complex128):
+        return 64
+    raise AssertionError("unsupported dtype %s" % t.dtype)
+
+
+def _dev_f64(x, device):
+    if isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.is_contiguous():
+        return x
+    return torch.as_tensor(x, dtype=torch.float64).to(device).contiguous()
+
+
+def reduce_scratch(device):
+    key = torch.device(device).index or 0
+    s = _scratch.get(key)
+    if s is None:
+        n = int(load().ps_reduce_scratch_bytes()) // 8
+        s = torch.empty(n, dtype=torch.float64, device=device)
+        _scratch[key] = s
+    return s
+
+
+def _chk(*ts):
+    for t in ts:
+        if t is not None:
+            assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "device, contiguous tensors only"
+
+
+def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
+    """apply_fl_weights (tauNL_utils.pyx:80-98) into l-major synthesis input columns.
+
+    f: (lmax+1, nslot, nr) real, h: (npol, nlm(lmax)) complex, out_t: (nlm, ld) complex.
+    """
+    _chk(f, h, out_t)
+    nr = f.shape[2]
+    npol = min(f.shape[1], h.shape[0], 2) if npol is None else npol
+    check(load().ps_weight_alm_lmajor(_prec(h), ptr(f), f.shape[1], nr, ptr(h), npol, int(lmax), int(lmin),
+                                      float(weight), out_t.shape[1], int(col_offset), ptr(out_t), stream_handle()))
+    return out_t
+
+
+def gnl_loc_fused_sum(P, Q, w, c4=1.0, U=None, V=None, c2=0.0, out=None, accumulate=False):
+    """c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] as a device scalar."""
+    _chk(P, Q, U, V)
+    w = _dev_f64(w, P.device)
+    if out is None:
+        out = torch.zeros((), dtype=torch.float64, device=P.device)
+    nr, npix = P.shape[-2], P.shape[-1]
+    check(load().ps_gnl_loc_fused_sum(_prec(P), ptr(P), ptr(Q), ptr(U) if U is not None else None,
+                                      ptr(V) if V is not None else None, ptr(w), nr, npix, float(c4), float(c2),
+                                      ptr(reduce_scratch(P.device)), ptr(out), int(accumulate), stream_handle()))
+    return out
+
+
+def gnl_loc_sum(w, P, Q):
+    """tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270)."""
+    return gnl_loc_fused_sum(P, Q, w, 1.0)
+
+
+def gnl_loc_disc_sum(w, P1, P2, Q1, Q2, out=None, accumulate=False):
+    """tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285)."""
+    _chk(P1, P2, Q1, Q2)
+    w = _dev_f64(w, P1.device)
+    if out is None:
+        out = torch.zeros((), dtype=torch.float64, device=P1.device)
+    check(load().ps_gnl_loc_disc_sum(_prec(P1), ptr(P1), ptr(P2), ptr(Q1), ptr(Q2), ptr(w), P1.shape[-2],
+                                     P1.shape[-1], ptr(reduce_scratch(P1.device)), ptr(out), int(accumulate),
+                                     stream_handle()))
+    return out
+
+
+def mc_accumulate(P, Q, U, V):
+    """U += sum_s P_s Q_s, V += sum_s P_s^2 over the leading (simulation) axis of P, Q."""
+    _chk(P, Q, U, V)
+    n = U.numel()
+    nsim = P.numel() // n
+    check(load().ps_mc_accumulate(_prec(P), ptr(P), ptr(Q), nsim, n, ptr(U), ptr(V), stream_handle()))
+
+
+def _products(mode, a, b, c=None, d=None, scale=1.0, out=None):
+    _chk(a, b, c, d)
+    if out is None:
+        out = torch.empty_like(a)
+    check(load().ps_pixel_products(_prec(a), mode, ptr(a), ptr(b), ptr(c) if c is not None else None,
+                                   ptr(d) if d is not None else None, a.numel(), float(scale), ptr(out),
+                                   stream_handle()))
+    return out
+
+
+def multiplyRR(a, b, out=None):
+    """ideal_fisher.multiplyRR (ideal_fisher.pyx:100-108)."""
+    return _products(0, a, b, out=out)
+
+
+def multiplyRR_sym(a, b, c, d, out=None):
+    """ideal_fisher.multiplyRR_sym (ideal_fisher.pyx:112-120): a b + c d."""
+    return _products(1, a, b, c, d, out=out)
+
+
+def multiply(a, b, c, scale=1.0, out=None):
+    """tauNL_utils.multiply (tauNL_utils.pyx:103-111): a b c."""
+    return _products(2, a, b, c, scale=scale, out=out)
+
+
+def multiply_asym(a1A, a1B, a3A, a3B, scale=1.0, out=None):
+    """tauNL_utils.multiply_asym (tauNL_utils.pyx:116-124): A (2 B C + A D)."""
+    return _products(3, a1A, a1B, a3A, a3B, scale=scale, out=out)
+
+
+def radial_sum(lm_map, w, f, lmax, lmin, npol, out=None, scale=1.0, accumulate=False):
+    """tauNL_utils.radial_sum (tauNL_utils.pyx:220-235): (nr, nlm) -> (npol, nlm)."""
+    _chk(lm_map, f)
+    w = _dev_f64(w, lm_map.device)
+    if out is None:
+        out = torch.zeros((npol, lm_map.shape[-1]), dtype=lm_map.dtype, device=lm_map.device)
+    check(load().ps_radial_sum(_prec(lm_map), ptr(lm_map), lm_map.shape[0], int(lmax), int(lmin), ptr(f),
+                               f.shape[1], int(npol), ptr(w), float(scale), int(accumulate), ptr(out),
+                               stream_handle()))
+    return out
+
+
+def tau_sum_n0(A, B, FLL, w, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False):
+    """tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646); A, B: (nmu, nr, nlm(Lmaxs))."""
+    _chk(A, B)
+    FLL = _dev_f64(FLL, A.device)
+    w = _dev_f64(w, A.device)
+    if out is None:
+        out = torch.zeros((), dtype=torch.float64, device=A.device)
+    check(load().ps_tau_sum_n0(_prec(A), ptr(A), ptr(B), A.shape[0], A.shape[1], int(Lmaxs), int(Lmin), int(Lmax),
+                               ptr(FLL), ptr(w), int(n1), ptr(reduce_scratch(A.device)), ptr(out), int(accumulate),
+                               stream_handle()))
+    return out
+
+
+def outer_product(Qa, Qb, out=None, accumulate=False):
+    """ideal_fisher.outer_product (ideal_fisher.pyx:423-459), sym=False."""
+    _chk(Qa, Qb)
+    if out is None:
+        out = torch.zeros((Qa.shape[1], Qb.shape[1]), dtype=torch.float64, device=Qa.device)
+    check(load().ps_outer_product(_prec(Qa), ptr(Qa), ptr(Qb), Qa.shape[1], Qb.shape[1], Qa.shape[2],
+                                  ptr(reduce_scratch(Qa.device)), ptr(out), int(accumulate), stream_handle()))
+    return out
+
+
+def convolve_F_lmajor(PQ, FLL, w, Lmaxs, Lmin, Lmax, out_t):
+    """convolve_F_n000 (tauNL_utils.pyx:966-992) into l-major synthesis columns 0..nr-1 of out_t."""
+    _chk(PQ, out_t)
+    FLL = _dev_f64(FLL, PQ.device)
+    w = _dev_f64(w, PQ.device)
+    check(load().ps_convolve_F_lmajor(_prec(PQ), ptr(PQ), PQ.shape[0], int(Lmaxs), int(Lmin), int(Lmax), ptr(FLL),
+                                      ptr(w), out_t.shape[1], ptr(out_t), stream_handle()))
+    return out_t
+
+
+def lmat_apply(x, lmax, mat=None, pre=None, post=None, out=None, alpha=1.0, accumulate=False):
+    """out[a] (+)= alpha post[a,l] sum_b mat[a,b,l] pre[b,l] x[b] on (nf, nlm) vectors."""
+    _chk(x)
+    nf = x.shape[0]
+    dev = x.device
+    m = _dev_f64(mat, dev) if mat is not None else None
+    pr = _dev_f64(pre, dev) if pre is not None else None
+    po = _dev_f64(post, dev) if post is not None else None
+    if out is None:
+        out = torch.empty_like(x)
+    _chk(out)
+    check(load().ps_lmat_apply(_prec(x), nf, int(lmax), ptr(m) if m is not None else None,
+                               ptr(pr) if pr is not None else None, ptr(po) if po is not None else None, ptr(x),
+                               x.shape[-1], ptr(out), out.shape[-1], float(alpha), int(accumulate), stream_handle()))
+    return out
+
+
+_dot_scratch = {}
+
+
+def mdot(x, y, lmax, out=None):
+    """CGD.dot (weighting.py:255) as a 0-d float64 device tensor."""
+    _chk(x, y)
+    key = x.device.index or 0
+    s = _dot_scratch.get(key)
+    if s is None:
+        s = torch.empty(int(load().ps_mdot_scratch_bytes()) // 8, dtype=torch.float64, device=x.device)
+        _dot_scratch[key] = s
+    if out is None:
+        out = torch.empty((), dtype=torch.float64, device=x.device)
+    nf = x.numel() // (((lmax + 1) * (lmax + 2)) // 2)
+    check(load().ps_mdot(_prec(x), nf, int(lmax), ptr(x), ptr(y), ptr(s), ptr(out), stream_handle()))
+    return out
+
+
+def cg_update_xr(rz, pAp, p, Ap, x, r):
+    _chk(rz, pAp, p, Ap, x, r)
+    check(load().ps_cg_update_xr(_prec(x), x.numel(), ptr(rz), ptr(pAp), ptr(p), ptr(Ap), ptr(x), ptr(r),
+                                 stream_handle()))
+
+
+def cg_update_p(num, den, z, p):
+    _chk(num, den, z, p)
+    check(load().ps_cg_update_p(_prec(p), p.numel(), ptr(num), ptr(den), ptr(z), ptr(p), stream_handle()))
+
+
+__all__ = ["lmat_apply", "mdot", "cg_update_xr", "cg_update_p", "convolve_F_lmajor", "weight_alm_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
+           "multiplyRR", "multiplyRR_sym", "multiply", "multiply_asym", "radial_sum", "tau_sum_n0",
+           "outer_product", "_abi"]

@@ -1,0 +1,595 @@
+"""Local trispectrum templates (gNL-loc, tauNL-loc) on the MI355X backend.
+
+Mirrors the gNL-loc / tauNL-loc branches of polyspec/tspec_template.py
+(class TSpecTemplate): same constructor arguments, the same numerator
+(Tl_numerator, tspec_template.py:1338-1954), Fisher contribution
+(compute_fisher_contribution, 2590-3052), compute_fisher (3054-3072) and
+Tl_unwindowed, with the same prefactors.  Everything after applySinv runs on
+the device: radial weighting -> batched synthesis -> fused pixel reductions,
+and product maps -> batched adjoint transforms -> radial / F_L sums.
+
+Two evaluation modes for the Monte-Carlo (disconnected) terms:
+  * batch_sims=True (default): the simulation pairs are processed in batched
+    launches once, at generate_sims / load_sims time, and collapsed into
+    U_r = sum_s P_s Q_s, V_r = sum_s P_s^2 (gNL-loc) and sum_s A_ss
+    (tauNL-loc); per numerator the gNL-loc two-field term is then one fused
+    pass (exact algebra of gnl_loc_disc_sum, tauNL_utils.pyx:275-285, summed
+    over s).  The tauNL-loc A_sd terms remain per simulation (they are
+    quadratic in the simulation) and are batched over simulations.
+  * batch_sims=False: the reference's per-simulation loop
+    (tspec_template.py:1542-1687) verbatim, one simulation at a time.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from .k_integrals import F_integral_n0, Pzeta as _Pzeta, p_integral_n0, q_integral
+
+SUPPORTED = ('gNL-loc', 'tauNL-loc')
+FOURPI15 = (4. * np.pi) ** 1.5
+
+
+def _round_up(x, m):
+    return (x + m - 1) // m * m
+
+
+class TSpecTemplate:
+    """Template trispectrum estimator restricted to the local templates (tspec_template.py:41-240)."""
+
+    def __init__(self, base, mask, applySinv, templates, k_arr, Tl_arr, lmin, lmax, Lmin=None, Lmax=None,
+                 Lmin_lens=None, Lmax_lens=None, ns=0.96, As=2.1e-9, k_pivot=0.05, r_values=[], r_weights={},
+                 rtau_values=[], rtau_weights={}, C_phi=[], C_lens_weight={}, K_coll=0.1, k_coll=0.1, r_star=None,
+                 r_hor=None, batch_sims=True, sim_batch=None):
+        for t in templates:
+            if t not in SUPPORTED:
+                raise NotImplementedError("Template %s is not provided by the hip backend (only %s)" % (t, SUPPORTED))
+        self.base = base
+        self.mask = mask
+        self.applySinv = applySinv
+        self.pol = self.base.pol
+        self.templates = list(templates)
+        self.k_arr = np.asarray(k_arr, dtype=np.float64)
+        self.lmin = lmin
+        self.lmax = lmax
+        self.ns, self.As, self.k_pivot = ns, As, k_pivot
+        self.batch_sims = batch_sims
+        self.sim_batch = sim_batch
+        self.kernel_events = None  # set to a list to time the Legendre stage of every filter call
+        self.Pzeta = lambda k: _Pzeta(k, self.As, self.ns, self.k_pivot)
+
+        assert self.lmax <= base.lmax, "Maximum l can't be larger than HEALPix resolution!"
+        assert self.lmin >= 2, "Minimum l can't be less than 2"
+        self.Lmin = 1 if Lmin is None else Lmin
+        self.Lmax = min([base.lmax, 2 * self.lmax]) if Lmax is None else Lmax
+        assert self.Lmin >= 1, "Minimum L can't be less than 1"
+        assert self.Lmax <= 2 * self.lmax, "Lmax <= 2*lmax by the triangle conditions"
+        assert self.Lmax <= base.lmax, "Maximum L can't be larger than HEALPix resolution"
+
+        # l / L bookkeeping (tspec_template.py:115-125, 437-451) for nmax = nmax_F = 0
+        self.lfilt = (self.base.l_arr >= self.lmin) & (self.base.l_arr <= self.lmax)
+        self.ls, self.ms = self.base.l_arr[self.lfilt], self.base.m_arr[self.lfilt]
+        self.m_weight = np.asarray(self.base.m_weight[self.lfilt], order='C')
+        self.Cinv = np.asarray(self.base.inv_Cl_tot_lm_mat[:, :, self.lfilt], order='C')
+        self.beam_lm = np.asarray(self.base.beam_lm)[:, self.lfilt]
+        self.nmax = 0
+        self.nmax_F = 0
+        self.nLmax = self.Lmax
+        self.nLmin = self.Lmin
+        self.nLfilt = (self.base.l_arr >= self.nLmin) & (self.base.l_arr <= self.nLmax)
+        self.Ls, self.Ms = self.base.l_arr[self.nLfilt], self.base.m_arr[self.nLfilt]
+        self.nLminfilt = self.base.l_arr[self.base.l_arr <= self.nLmax] >= self.nLmin
+        self.lminfilt = self.base.l_arr[self.base.l_arr <= self.lmax] >= self.lmin
+        self.p_inds = [0]
+
+        m = np.asarray(mask, dtype=np.float64)
+        if m.ndim > 1:
+            assert len(m[0]) == self.base.Npix, f'Mask has incorrect shape: {m.shape}'
+        else:
+            assert len(m) == self.base.Npix, f'Mask has incorrect shape: {m.shape}'
+        self.ones_mask = bool(np.std(m) < 1e-12 and np.abs(np.mean(m) - 1) < 1e-12)
+        self.r_star = 13882.607764400758 if r_star is None else r_star
+        self.r_hor = 14163.032903769747 if r_hor is None else r_hor
+
+        self.Tl_arr = np.asarray(Tl_arr, dtype=np.float64)
+        if not self.pol:
+            assert len(self.Tl_arr) == 1, "Transfer function should contain only T components"
+        else:
+            assert len(self.Tl_arr) == 2, "Transfer function should contain both T and E components"
+        assert len(self.Tl_arr[0]) >= self.lmax + 1, "Transfer function must contain all ell modes of interest"
+        assert self.Tl_arr[0][:2].sum() == 0., "Transfer function should return zero for ell = 0, 1"
+        assert len(self.Tl_arr[0][0]) == len(k_arr), "Transfer function must be computed on the input k-grid"
+
+        self.timers = {x: 0. for x in ['precomputation', 'numerator', 'fisher', 'fish_grfs', 'fish_outer',
+                                       'fish_weighting', 'fish_convolve', 'fish_deriv', 'fish_products', 'Sinv', 'Ainv',
+                                       'map_transforms', 'gNL_summation', 'tauNL_products', 'tauNL_summation']}
+        self.base.time_sht = 0.
+        if len(r_values) == 0:
+            print("# No input radial sampling points supplied\n")
+            self.N_r = 0
+        else:
+            for t in self.templates:
+                assert t in r_weights.keys(), "Must supply weight for template %s" % t
+            self.r_arr = np.asarray(r_values, dtype=np.float64)
+            self.N_r = len(self.r_arr)
+            self.r_weights = {k: np.asarray(v, dtype=np.float64) for k, v in r_weights.items()}
+            self._prepare_templates()
+
+    # ------------------------------------------------------------ precompute
+    def _prepare_templates(self):
+        """p_l^X(r), q_l^X(r) and F_L(r, r') (tspec_template.py:520-708, n = 0 terms)."""
+        t0 = time.time()
+        if hasattr(self, 't0_num'):
+            delattr(self, 't0_num')
+        nslot = 1 + 2 * self.pol
+        self.plLXs = p_integral_n0(self.k_arr, self.Pzeta(self.k_arr), self.Tl_arr, self.r_arr, self.lmin, self.lmax,
+                                   self.lmax + 1, nslot)[None]
+        self.qlXs = q_integral(self.k_arr, self.Tl_arr, self.r_arr, self.lmin, self.lmax, nslot)
+        if 'tauNL-loc' in self.templates:
+            self.FLLs = F_integral_n0(self.r_arr, self.Lmin, self.Lmax, self.ns, self.As, self.k_pivot)[None]
+        self._upload_tables()
+        self.timers['precomputation'] += time.time() - t0
+
+    def _upload_tables(self):
+        dev, rdt = self.base.device, self.base.rdtype
+        self.fP_d = torch.from_numpy(np.ascontiguousarray(self.plLXs[self.nmax])).to(dev, rdt)
+        self.fQ_d = torch.from_numpy(np.ascontiguousarray(self.qlXs)).to(dev, rdt)
+        if hasattr(self, 'FLLs'):
+            self.FLL_d = torch.from_numpy(np.ascontiguousarray(self.FLLs[self.nmax_F])).to(dev)
+        self.w_d = {t: torch.from_numpy(self.r_weights[t]).to(dev) for t in self.templates}
+        self.w_tau_sqrt = {t: torch.from_numpy(self.r_weights[t] / np.sqrt(4. * np.pi)).to(dev)
+                           for t in self.templates}
+        self.mw_d = torch.from_numpy(1. + (self.base.m_arr[self.base.l_arr <= self.lmax] > 0)).to(dev, rdt)
+        self._lmaj = {}
+
+    # ---------------------------------------------------------------- timing
+    def report_timings(self):
+        print("\n## Timings ##")
+        for k, v in self.timers.items():
+            if v > 0:
+                print("%s: %.2fs" % (k, v))
+        print("\nNumber of SHTs: %d forward, %d reverse; SHT time %.2fs" % (self.base.n_SHTs_forward,
+                                                                           self.base.n_SHTs_reverse,
+                                                                           self.base.time_sht))
+
+    def reset_timings(self):
+        for k in self.timers:
+            self.timers[k] = 0.
+        self.base.time_sht = 0.
+        self.base.n_SHTs_forward = 0
+        self.base.n_SHTs_reverse = 0
+
+    # ------------------------------------------------------------ filtering
+    def _h_dev(self, h):
+        """applySinv output (nf, nlm(lmax)) -> device (npol_h, nlm(lmax)) of the backend dtype."""
+        if not isinstance(h, torch.Tensor):
+            h = torch.from_numpy(np.ascontiguousarray(h))
+        h = h.to(self.base.device, self.base.cdtype)
+        npol = 2 if self.pol else 1
+        return h[:npol].contiguous()
+
+    def _lmajor_buffer(self, ncol, lmax):
+        plan = self.base.plan(lmax, 0)
+        ld = _round_up(ncol, plan.lmajor_multiple)
+        key = (ld, lmax)
+        buf = self._lmaj.get(key)
+        if buf is None:
+            buf = torch.zeros((plan.nlm, ld), dtype=self.base.cdtype, device=self.base.device)
+            self._lmaj[key] = buf
+        return buf, plan
+
+    def _filter(self, hs, P_weight=1. / np.sqrt(4. * np.pi)):
+        """P and Q maps of each filtered field (_apply_all_filters 'p00' and 'q',
+        tspec_template.py:1036-1095 -> 799-855): one batched synthesis.
+
+        Returns (len(hs), 2, N_r, Npix): [:, 0] = P (weighted by P_weight), [:, 1] = Q."""
+        t0 = time.time()
+        nr = self.N_r
+        ncol = 2 * nr * len(hs)
+        buf, plan = self._lmajor_buffer(ncol, self.lmax)
+        for i, h in enumerate(hs):
+            K.weight_alm_lmajor(self.fP_d, h, self.lmax, self.lmin, P_weight, buf, col_offset=2 * nr * i)
+            K.weight_alm_lmajor(self.fQ_d, h, self.lmax, self.lmin, 1., buf, col_offset=2 * nr * i + nr)
+        if self.kernel_events is not None:
+            # split stages with device events around the Legendre stage (kernel timing)
+            key = ('phase', ncol)
+            ph = self._lmaj.get(key)
+            if ph is None:
+                ph = plan.phase_buffer(ncol)
+                self._lmaj[key] = ph
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.legendre_synthesis_lmajor(buf, ncol, ph)
+            e1.record()
+            self.kernel_events.append((ncol, e0, e1))
+            maps = plan.ring_synthesis(ph, ncol)
+        else:
+            maps = plan.synthesis_lmajor(buf, ncol)
+        self.base.n_SHTs_reverse += ncol
+        self.timers['map_transforms'] += time.time() - t0
+        return maps.view(len(hs), 2, nr, self.base.Npix)
+
+    # ---------------------------------------------------------- tauNL pieces
+    def _A_maps(self, P1, Q1, P2=None, Q2=None, out=None):
+        """[P Q]_LM of fields 1 = (P1, Q1) and 2 = (P2, Q2) (tspec_template.py:1294-1303, n = 0):
+        Y^dag[Q1 P1] without field 2, else Y^dag[Q2 P1 + Q1 P2], at nLmax, as (1, N_r, nlm(nLmax))."""
+        t0 = time.time()
+        if P2 is None:
+            prod = K.multiplyRR(Q1, P1)
+        else:
+            prod = K.multiplyRR_sym(Q2, P1, Q1, P2)
+        plan = self.base.plan(self.nLmax, 0)
+        A = plan.adjoint(prod.view(self.N_r, 1, -1), np.pi / (3. * self.base.Nside ** 2), out=out)
+        self.base.n_SHTs_forward += self.N_r
+        self.timers['tauNL_products'] += time.time() - t0
+        return A.view(1, self.N_r, -1)
+
+    def _tau_sum(self, A, B, t, out=None, accumulate=False, scale=1.0):
+        """tau_sum_n0 (tauNL_utils.pyx:604-646) on device, result scaled on host by `scale`."""
+        t0 = time.time()
+        r = K.tau_sum_n0(A, B, self.FLL_d, self.w_d[t], self.nLmax, self.Lmin, self.Lmax, 0)
+        self.timers['tauNL_summation'] += time.time() - t0
+        return r * scale
+
+    # ------------------------------------------------------------------ sims
+    def _process_pairs(self, pairs, input_type):
+        """applySinv + filters for a list of (alpha, beta) inputs -> list of (maps_a, maps_b)."""
+        t0 = time.time()
+        hs = []
+        for a, b in pairs:
+            hs.append(self._h_dev(self.applySinv(a, input_type=input_type, lmax=self.lmax)))
+            hs.append(self._h_dev(self.applySinv(b, input_type=input_type, lmax=self.lmax)))
+        self.timers['Sinv'] += time.time() - t0
+        maps = self._filter(hs)
+        return [(maps[2 * i], maps[2 * i + 1]) for i in range(len(pairs))]
+
+    def _sim_pair(self, ii, Cl_input):
+        if self.ones_mask:
+            a = self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, output_type='harmonic', lmax=self.lmax,
+                                        deconvolve_beam=False)
+            b = self.base.generate_data(int(2e5) + ii, Cl_input=Cl_input, output_type='harmonic', lmax=self.lmax,
+                                        deconvolve_beam=False)
+            return (a, b), 'harmonic'
+        a = self._masked(self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False))
+        b = self._masked(self.base.generate_data(int(2e5) + ii, Cl_input=Cl_input, deconvolve_beam=False))
+        return (a, b), 'map'
+
+    def _masked(self, m):
+        if isinstance(m, torch.Tensor):
+            if not hasattr(self, 'mask_d'):
+                mk = np.asarray(self.mask, dtype=np.float64)
+                self.mask_d = torch.from_numpy(mk).to(self.base.device, m.dtype)
+            return (m * self.mask_d.to(m.dtype)).contiguous()
+        return self.mask * m
+
+    def _collapse_init(self):
+        dev, rdt = self.base.device, self.base.rdtype
+        self.t0_num = np.zeros(len(self.templates))
+        if 'gNL-loc' in self.templates:
+            self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=rdt, device=dev)
+            self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=rdt, device=dev)
+        if 'tauNL-loc' in self.templates:
+            self.SA_d = torch.zeros((1, self.N_r, self.base.plan(self.nLmax, 0).nlm), dtype=self.base.cdtype,
+                                    device=dev)
+
+    def _collapse_add(self, maps_a, maps_b):
+        """Add one simulation pair to the collapsed Monte-Carlo sums and to t0."""
+        Pa, Qa = maps_a[0], maps_a[1]
+        Pb, Qb = maps_b[0], maps_b[1]
+        for ii, t in enumerate(self.templates):
+            if t == 'gNL-loc':
+                K.mc_accumulate(Pa, Qa, self.U_d, self.V_d)
+                K.mc_accumulate(Pb, Qb, self.U_d, self.V_d)
+                d = K.gnl_loc_disc_sum(self.w_d[t], Pa, Pb, Qa, Qb)
+                self.t0_num[ii] += 54. / 100. * float(d) / self.N_it * self.base.A_pix * FOURPI15
+            if t == 'tauNL-loc':
+                A_aa = self._A_maps(Pa, Qa)
+                A_bb = self._A_maps(Pb, Qb)
+                self.SA_d += A_aa
+                self.SA_d += A_bb
+                A_ab = self._A_maps(Pa, Qa, Pb, Qb)
+                tau = 4 * float(self._tau_sum(A_aa, A_bb, t)) + 2 * float(self._tau_sum(A_ab, A_ab, t))
+                self.t0_num[ii] += 1. / 24. * tau * 3 / self.N_it * FOURPI15
+
+    def _ingest_sims(self, get_pair, N_pairs, preload, verb):
+        self.N_it = N_pairs
+        print("Using %d pairs of Monte Carlo simulations" % self.N_it)
+        keep_maps = preload and (not self.batch_sims or 'tauNL-loc' in self.templates)
+        self.preload = preload
+        if self.batch_sims:
+            self._collapse_init()
+        if keep_maps:
+            self.proc_maps = []
+        nb = self.sim_batch or max(1, min(4, int(64 // max(1, self.N_r // 4))))
+        process_now = self.batch_sims or keep_maps
+        for i0 in range(0, N_pairs if process_now else 0, nb):
+            idx = list(range(i0, min(N_pairs, i0 + nb)))
+            if verb:
+                print("Processing bias simulation pairs %d-%d of %d" % (idx[0] + 1, idx[-1] + 1, N_pairs))
+            pairs, itype = [], None
+            for ii in idx:
+                pr, itype = get_pair(ii)
+                pairs.append(pr)
+            procs = self._process_pairs(pairs, itype)
+            for ma, mb in procs:
+                if self.batch_sims:
+                    self._collapse_add(ma, mb)
+                if keep_maps:
+                    self.proc_maps.append((ma.clone(), mb.clone()))
+        if not preload:
+            def load_sim_data(ii):
+                pr, itype = get_pair(ii)
+                return self._process_pairs([pr], itype)[0]
+            self.load_sim_data = load_sim_data
+
+    def generate_sims(self, N_pairs, Cl_input=[], preload=True, verb=False):
+        """Gaussian simulation pairs for the two- and zero-field terms (tspec_template.py:1172-1228)."""
+        if len(Cl_input) == 0:
+            Cl_input = self.base.Cl_tot
+        self._ingest_sims(lambda ii: self._sim_pair(ii, Cl_input), N_pairs, preload, verb)
+
+    def load_sims(self, load_sim_pair, N_pairs, verb=False, preload=True, input_type='map'):
+        """User-supplied simulation pairs (tspec_template.py:1128-1170)."""
+        self._ingest_sims(lambda ii: (load_sim_pair(ii), input_type), N_pairs, preload, verb)
+
+    def _sim_maps(self, isim):
+        if self.preload and hasattr(self, 'proc_maps'):
+            return self.proc_maps[isim]
+        return self.load_sim_data(isim)
+
+    # ------------------------------------------------------------ numerator
+    def Tl_numerator(self, data, include_disconnected_term=True, verb=False, input_type='map'):
+        """Numerator of the unwindowed estimator for every template (tspec_template.py:1338-1954)."""
+        t_start = time.time()
+        if not hasattr(self, 'r_arr'):
+            raise Exception("Need to supply radial integration points or run optimize_radial_sampling_1d()!")
+        if not hasattr(self, 'preload') and include_disconnected_term:
+            raise Exception("Need to generate or specify bias simulations!")
+        if self.pol:
+            assert len(data) == 3, "Data must contain T, Q, U components!"
+        else:
+            assert (len(data) == 1 and len(data[0]) == self.base.Npix) or len(data) == self.base.Npix, \
+                "Data must contain T only!"
+        compute_t0 = not (hasattr(self, 't0_num') and include_disconnected_term)
+
+        t0 = time.time()
+        h = self._h_dev(self.applySinv(data, input_type=input_type, lmax=self.lmax))
+        self.timers['Sinv'] += time.time() - t0
+        maps = self._filter([h])[0]
+        P, Q = maps[0], maps[1]
+
+        nT = len(self.templates)
+        t4 = [None] * nT
+        t2 = [0.] * nT
+        A_dd = None
+        collapsed = include_disconnected_term and self.batch_sims
+        for ii, t in enumerate(self.templates):
+            if t == 'gNL-loc':
+                t1 = time.time()
+                c4 = 9. / 100. * 4. * self.base.A_pix * FOURPI15
+                if collapsed:
+                    c2 = -54. / 100. * self.base.A_pix * FOURPI15 / self.N_it
+                    t4[ii] = K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4, self.U_d, self.V_d, c2)
+                else:
+                    t4[ii] = K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4)
+                self.timers['gNL_summation'] += time.time() - t1
+            if t == 'tauNL-loc':
+                A_dd = self._A_maps(P, Q)
+                t4[ii] = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)
+                if collapsed:
+                    t4[ii] = t4[ii] + self._tau_sum(self.SA_d, A_dd, t, scale=-1. / 24. * 6. / self.N_it / 2. *
+                                                    FOURPI15 * 4.)
+        t4 = np.array([float(x) for x in t4])
+
+        if include_disconnected_term:
+            t2 = np.zeros(nT)
+            t0n = np.zeros(nT)
+            if collapsed:
+                # remaining per-simulation work: tauNL-loc A_sd terms (quadratic in the simulation)
+                if 'tauNL-loc' in self.templates:
+                    ii = self.templates.index('tauNL-loc')
+                    t = 'tauNL-loc'
+                    acc = torch.zeros((), dtype=torch.float64, device=self.base.device)
+                    for isim in range(self.N_it):
+                        ma, mb = self._sim_maps(isim)
+                        for ms in (ma, mb):
+                            A_sd = self._A_maps(ms[0], ms[1], P, Q)
+                            acc = acc + 2 * self._tau_sum(A_sd, A_sd, t)
+                    t2[ii] = -1. / 24. * float(acc) * 6 / self.N_it / 2. * FOURPI15
+            else:
+                for isim in range(self.N_it):
+                    ma, mb = self._sim_maps(isim)
+                    for ii, t in enumerate(self.templates):
+                        if t == 'gNL-loc':
+                            s = (float(K.gnl_loc_disc_sum(self.w_d[t], P, ma[0], Q, ma[1])) +
+                                 float(K.gnl_loc_disc_sum(self.w_d[t], P, mb[0], Q, mb[1])))
+                            t2[ii] += -54. / 100. * s / self.N_it * self.base.A_pix * FOURPI15
+                            if compute_t0:
+                                s = float(K.gnl_loc_disc_sum(self.w_d[t], ma[0], mb[0], ma[1], mb[1]))
+                                t0n[ii] += 54. / 100. * s / self.N_it * self.base.A_pix * FOURPI15
+                        if t == 'tauNL-loc':
+                            A_aa = self._A_maps(ma[0], ma[1])
+                            A_ad = self._A_maps(ma[0], ma[1], P, Q)
+                            tau = 4 * float(self._tau_sum(A_aa, A_dd, t)) + 2 * float(self._tau_sum(A_ad, A_ad, t))
+                            A_bb = self._A_maps(mb[0], mb[1])
+                            A_bd = self._A_maps(mb[0], mb[1], P, Q)
+                            tau += 4 * float(self._tau_sum(A_bb, A_dd, t)) + 2 * float(self._tau_sum(A_bd, A_bd, t))
+                            t2[ii] += -1. / 24. * tau * 6 / self.N_it / 2. * FOURPI15
+                            if compute_t0:
+                                A_ab = self._A_maps(ma[0], ma[1], mb[0], mb[1])
+                                tau = 4 * float(self._tau_sum(A_aa, A_bb, t)) + 2 * float(self._tau_sum(A_ab, A_ab, t))
+                                t0n[ii] += 1. / 24. * tau * 3 / self.N_it * FOURPI15
+            if compute_t0 and not collapsed:
+                self.t0_num = t0n
+            out = t4 + t2 + self.t0_num
+        else:
+            print("## No subtraction of (parity-conserving) disconnected terms performed!")
+            out = t4
+        self.timers['numerator'] += time.time() - t_start
+        return out
+
+    # --------------------------------------------------------------- Fisher
+    def _transform_maps(self, maps, f, w, out, scale):
+        """sum_r w_r f_l^X(r) Y^dag[map_r]_lm (tspec_template.py:2575-2581), added into out."""
+        plan = self.base.plan(self.lmax, 0)
+        lm = plan.adjoint(maps.view(self.N_r, 1, -1).contiguous(), np.pi / (3. * self.base.Nside ** 2))
+        self.base.n_SHTs_forward += self.N_r
+        K.radial_sum(lm.view(self.N_r, -1), w, f, self.lmax, self.lmin, 1 + 2 * self.pol, out=out, scale=scale,
+                     accumulate=True)
+
+    def _Q_gnl(self, Pm, Qm, Qs, ii):
+        """gNL-loc Q-derivatives (tspec_template.py:2713-2740) into Qs[:, ii]."""
+        t = 'gNL-loc'
+        w = self.w_d[t]
+        s4 = np.sqrt(4. * np.pi)
+        P0 = [Pm[0] * s4, Pm[1] * s4]
+        fp, fq = self.fP_d, self.fQ_d
+        self._transform_maps(K.multiply(P0[0], P0[0], Qm[0]), fp, w, Qs[0, ii], 162. / 25.)
+        self._transform_maps(K.multiply(P0[0], P0[0], P0[0]), fq, w, Qs[0, ii], 54. / 25.)
+        self._transform_maps(K.multiply(P0[1], P0[1], Qm[1]), fp, w, Qs[1, ii], 162. / 25.)
+        self._transform_maps(K.multiply(P0[1], P0[1], P0[1]), fq, w, Qs[1, ii], 54. / 25.)
+        self._transform_maps(K.multiply_asym(P0[0], P0[1], Qm[0], Qm[1]), fp, w, Qs[2, ii], 54. / 25.)
+        self._transform_maps(K.multiply(P0[0], P0[0], P0[1]), fq, w, Qs[2, ii], 54. / 25.)
+        self._transform_maps(K.multiply_asym(P0[1], P0[0], Qm[1], Qm[0]), fp, w, Qs[3, ii], 54. / 25.)
+        self._transform_maps(K.multiply(P0[0], P0[1], P0[1]), fq, w, Qs[3, ii], 54. / 25.)
+
+    def _Q_tau(self, Pm, Qm, Qs, ii, weight):
+        """tauNL-loc Q-derivatives: _F_PQ (2287-2306) + _get_Q4_perms (2521-2554) with n1 = n3 = n = 0."""
+        t = 'tauNL-loc'
+        w = self.w_d[t]
+        Npix = self.base.Npix
+        # F_vecs[i] = to_map_vec(convolve_F_n000(Y^dag[PQ])) for (i1, i2) = (0,0), (1,1), (0,1) sym
+        F = []
+        for i1, i2 in ((0, 0), (1, 1), (0, 1)):
+            if i1 == i2:
+                A = self._A_maps(Pm[i1], Qm[i1])
+            else:
+                # Q[i2] P[i1] + Q[i1] P[i2] (multiplyRR_sym, tspec_template.py:1975)
+                A = self._A_maps(Pm[i1], Qm[i1], Pm[i2], Qm[i2])
+            buf, plan = self._lmajor_buffer(self.N_r, self.nLmax)
+            K.convolve_F_lmajor(A[0], self.FLL_d, w, self.nLmax, self.Lmin, self.Lmax, buf)
+            F.append(plan.synthesis_lmajor(buf, self.N_r).view(self.N_r, Npix).clone())
+            self.base.n_SHTs_reverse += self.N_r
+        # (pq_inds, F_inds, weights) of the four Q4 permutations
+        perms = [([0], [0], [12.]), ([1], [1], [12.]), ([1, 0], [0, 2], [4., 4.]), ([0, 1], [1, 2], [4., 4.])]
+        for q, (pq, fi, wt) in enumerate(perms):
+            # step 1: q_l^X integral of [sum w P F]_lm (compute_productP_real_all, integrate_pq)
+            if len(pq) == 1:
+                prodP = K.multiplyRR(Pm[pq[0]], F[fi[0]])
+                prodQ = K.multiplyRR(Qm[pq[0]], F[fi[0]])
+            else:
+                prodP = K.multiplyRR_sym(Pm[pq[0]], F[fi[0]], Pm[pq[1]], F[fi[1]])
+                prodQ = K.multiplyRR_sym(Qm[pq[0]], F[fi[0]], Qm[pq[1]], F[fi[1]])
+            self._transform_maps(prodP, self.fQ_d, w, Qs[q, ii], 0.5 * wt[0] * weight)
+            # step 2: p_l^X integral of [sum w Q F]_lm with r weights / sqrt(4 pi)
+            self._transform_maps(prodQ, self.fP_d, self.w_tau_sqrt[t], Qs[q, ii], 0.5 * wt[0] * weight)
+
+    def _weight_Q(self, Qs, weighting):
+        """_weight_Q_maps (tspec_template.py:2556-2572)."""
+        nT = len(self.templates)
+        for q in range(4):
+            for ii in range(nT):
+                if weighting == 'Ainv':
+                    full = Qs[q, ii] * torch.from_numpy(self.base.beam[:, self.base.l_arr[self.base.l_arr <= self.lmax]]).to(
+                        Qs.device, Qs.dtype)
+                    t0 = time.time()
+                    if self.ones_mask:
+                        r = self.applySinv(full, input_type='harmonic', lmax=self.lmax)
+                    else:
+                        r = self.applySinv(self.mask * self.base.to_map(full, lmax=self.lmax), lmax=self.lmax)
+                    self.timers['Sinv'] += time.time() - t0
+                    if not isinstance(r, torch.Tensor):
+                        r = torch.from_numpy(np.asarray(r))
+                    r = r.to(Qs.device, Qs.dtype)
+                    Qs[q, ii] = r * torch.from_numpy(self.lminfilt).to(Qs.device)
+                else:
+                    Qs[q, ii] = Qs[q, ii] * self.mw_d
+
+    def compute_fisher_contribution(self, seed, verb=False):
+        """Fisher contribution from one pair of GRFs (tspec_template.py:2590-3052)."""
+        print("Computing Fisher matrix with seed %d" % seed)
+        t0 = time.time()
+        a_maps = []
+        for ii in range(2):
+            if self.ones_mask:
+                a_maps.append(self.base.generate_data(seed=seed + int((1 + ii) * 1e9), output_type='harmonic',
+                                                      lmax=self.lmax, deconvolve_beam=True))
+            else:
+                a_maps.append(self.base.generate_data(seed=seed + int((1 + ii) * 1e9), output_type='harmonic',
+                                                      deconvolve_beam=True))
+        self.timers['fish_grfs'] += time.time() - t0
+        return self.fisher_contribution_from_grfs(a_maps, verb=verb)
+
+    def fisher_contribution_from_grfs(self, a_maps, verb=False):
+        """Fisher contribution of two given harmonic GRFs a_maps (the body of
+        compute_fisher_contribution after generate_data, tspec_template.py:2616-3052)."""
+        t_start = time.time()
+        nT = len(self.templates)
+        nslot = 1 + 2 * self.pol
+        nlm = (self.lmax + 1) * (self.lmax + 2) // 2
+
+        def compute_Q4(weighting):
+            t0 = time.time()
+            if weighting == 'Sinv':
+                if self.ones_mask:
+                    lf = self.base.l_arr <= self.lmax
+                    U = [self.applySinv(self.base.beam_lm[:, lf] * a, input_type='harmonic', lmax=self.lmax)
+                         for a in a_maps]
+                else:
+                    U = [self.applySinv(self.mask * self.base.to_map(self.base.beam_lm * a), lmax=self.lmax)
+                         for a in a_maps]
+                self.timers['Sinv'] += time.time() - t0
+            else:
+                if self.ones_mask:
+                    U = [self.base.applyAinv(a, input_type='harmonic', lmax=self.lmax) for a in a_maps]
+                else:
+                    lf = self.base.l_arr <= self.lmax
+                    U = [self.base.applyAinv(a, input_type='harmonic')[:, lf] for a in a_maps]
+                self.timers['Ainv'] += time.time() - t0
+            hs = [self._h_dev(u) for u in U]
+            maps = self._filter(hs)  # (2, 2, N_r, Npix)
+            Pm = [maps[0, 0], maps[1, 0]]
+            Qm = [maps[0, 1], maps[1, 1]]
+            Qs = torch.zeros((4, nT, nslot, nlm), dtype=self.base.cdtype, device=self.base.device)
+            t1 = time.time()
+            for ii, t in enumerate(self.templates):
+                if t == 'gNL-loc':
+                    self._Q_gnl(Pm, Qm, Qs, ii)
+                if t == 'tauNL-loc':
+                    self._Q_tau(Pm, Qm, Qs, ii, FOURPI15)
+            self.timers['fish_deriv'] += time.time() - t1
+            t1 = time.time()
+            self._weight_Q(Qs, weighting)
+            self.timers['fish_weighting'] += time.time() - t1
+            return Qs.reshape(4, nT, -1).contiguous()
+
+        Q4_Sinv = compute_Q4('Sinv')
+        Q4_Ainv = compute_Q4('Ainv')
+        t0 = time.time()
+        fish = K.outer_product(Q4_Sinv, Q4_Ainv).cpu().numpy()
+        self.timers['fish_outer'] += time.time() - t0
+        self.timers['fisher'] += time.time() - t_start
+        return fish
+
+    def compute_fisher(self, N_it, verb=False):
+        """Monte-Carlo Fisher matrix and its inverse (tspec_template.py:3054-3072)."""
+        fish = np.zeros((len(self.templates), len(self.templates)))
+        for seed in range(N_it):
+            print("Computing Fisher contribution %d of %d" % (seed + 1, N_it))
+            fish += self.compute_fisher_contribution(seed, verb=verb * (seed == 0)) / N_it
+        self.fish = fish
+        self.inv_fish = np.linalg.inv(fish)
+        return fish
+
+    def Tl_unwindowed(self, data, fish=[], include_disconnected_term=True, verb=False, input_type='map'):
+        """Unwindowed amplitudes F^-1 numerator (tspec_template.py:3074-3108)."""
+        if len(fish) != 0:
+            self.fish = fish
+            self.inv_fish = np.linalg.inv(fish)
+        if not hasattr(self, 'inv_fish'):
+            raise Exception("Need to compute Fisher matrix first!")
+        Tl_num = self.Tl_numerator(data, include_disconnected_term=include_disconnected_term, verb=verb,
+                                   input_type=input_type)
+        Tl_out = np.matmul(self.inv_fish, Tl_num)
+        return {t: Tl_out[i] for i, t in enumerate(self.templates)}
