@@ -47,7 +47,9 @@ class OracleTSpec:
         names = [['TT', 'TE', 'TB'], ['TE', 'EE', 'EB'], ['TB', 'EB', 'BB']]
         for a in range(nf):
             for b in range(nf):
-                C[:, a, b] = np.asarray(Cl_tot[names[a][b]])[:L1]
+                k = names[a][b]
+                # TB, EB default to zero (base.py:45-46)
+                C[:, a, b] = np.asarray(Cl_tot[k])[:L1] if k in Cl_tot else 0.
         inv = np.zeros_like(C)
         good = np.linalg.det(C) > 0
         inv[good] = np.linalg.inv(C[good])

@@ -11,15 +11,6 @@
 
 namespace ps {
 
-__device__ __forceinline__ void lm_to_lm(long lm, int lmax, int* l, int* m) {
-    const double B = 2.0 * lmax + 3.0;
-    int mm = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
-    while (mm > 0 && (long)mm * (2 * lmax + 1 - mm) / 2 > lm) mm--;
-    while (mm < lmax && (long)(mm + 1) * (2 * lmax + 1 - (mm + 1)) / 2 <= lm) mm++;
-    *m = mm;
-    *l = (int)(lm - (long)mm * (2 * lmax + 1 - mm) / 2);
-}
-
 // out[a] = post[a,l] sum_b mat[a,b,l] pre[b,l] in[b]   (accumulate: out += ...)
 template <typename T>
 __global__ void k_lmat(int nf, int lmax, const double* __restrict__ mat, const double* __restrict__ pre,
@@ -29,7 +20,7 @@ __global__ void k_lmat(int nf, int lmax, const double* __restrict__ mat, const d
     const long lm = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (lm >= nlm) return;
     int l, m;
-    lm_to_lm(lm, lmax, &l, &m);
+    lm_decode(lm, lmax, &l, &m);
     const int L1 = lmax + 1;
     double xr[3], xi[3];
     for (int b = 0; b < nf; b++) {

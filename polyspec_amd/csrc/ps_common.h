@@ -88,6 +88,17 @@ __device__ __forceinline__ T scale_factor(int k) {
     return T(0);
 }
 
+// healpy m-major index lm = m(2 lmax + 1 - m)/2 + l  ->  (l, m).  The m block
+// starts at s(m) = m(2 lmax + 3 - m)/2; m is the largest value with s(m) <= lm.
+__device__ __forceinline__ void lm_decode(long lm, int lmax, int* l, int* m) {
+    const double B = 2.0 * lmax + 3.0;
+    int mm = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
+    while (mm > 0 && (long)mm * (2 * lmax + 3 - mm) / 2 > lm) mm--;
+    while (mm < lmax && (long)(mm + 1) * (2 * lmax + 2 - mm) / 2 <= lm) mm++;
+    *m = mm;
+    *l = (int)(lm - (long)mm * (2 * lmax + 1 - mm) / 2);
+}
+
 // Remap a linear block id so that each XCD (8 per MI355X; the dispatcher sends
 // block p to XCD p % 8) receives a contiguous range of logical blocks.
 __device__ __forceinline__ unsigned xcd_remap(unsigned p, unsigned nblocks) {

@@ -43,12 +43,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 // for lmin <= l <= lmax, zero otherwise (and zero in padding columns r >= nr).
 // f: (lmax_f+1, nslot, nr) real; h: (npol, nlm) complex (full m-major set).
 __device__ __forceinline__ int l_of_lm(long lm, int lmax) {
-    // m-major index lm = m(2 lmax + 1 - m)/2 + l  ->  l
-    const double B = 2.0 * lmax + 3.0;
-    int m = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
-    while (m > 0 && (long)m * (2 * lmax + 1 - m) / 2 > lm) m--;
-    while (m < lmax && (long)(m + 1) * (2 * lmax + 1 - (m + 1)) / 2 <= lm) m++;
-    return (int)(lm - (long)m * (2 * lmax + 1 - m) / 2);
+    int l, m;
+    lm_decode(lm, lmax, &l, &m);
+    return l;
 }
 
 template <typename T>

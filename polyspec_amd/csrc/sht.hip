@@ -815,6 +815,11 @@ __global__ __launch_bounds__(256) void k_leg_adj_sp(ShtDev<T> d, int nmap, const
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const double* cf4 = d.coef + 4 * base;
     const int lstart = m > d.spin ? m : d.spin;
+    // coefficients with m <= l < spin vanish identically
+    for (int i = tid; i < (lstart - m) * BC * 2; i += blockDim.x) {
+        const int l = m + i / (BC * 2), b = chunk * BC + (i / 2) % BC, g = i % 2;
+        if (l <= d.lmax && b < nmap) alm[(size_t)(2 * b + g) * alm_stride + base + l] = cplx<T>{T(0), T(0)};
+    }
     for (int l0 = lstart; l0 <= d.lmax; l0 += LC) {
         cplx<T> acc[LC][BC][2];  // G, C
 #pragma unroll

@@ -1,0 +1,121 @@
+"""GPU parity of the radial-node and CG vector kernels against the FP64 oracle
+loop nests / numpy definitions on seeded inputs.  FP64 kernels: 1e-12 relative;
+FP32: 1e-5 relative (FP64 accumulation, FP32 storage)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def T(x, dt):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda().to(dt)
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("lmax", [0, 1, 7, 30, 200])
+def test_lmat_apply(gpu, prec, lmax):
+    from polyspec_amd import kernels as K
+    rng = np.random.default_rng(lmax)
+    ls, ms = oracle.getlm(lmax)
+    nf = 3
+    x = rng.standard_normal((nf, len(ls))) + 1j * rng.standard_normal((nf, len(ls)))
+    M = rng.standard_normal((nf, nf, lmax + 1))
+    pre = rng.random((nf, lmax + 1))
+    post = rng.random((nf, lmax + 1))
+    cdt = torch.complex128 if prec == 64 else torch.complex64
+    y = K.lmat_apply(T(x, cdt), lmax, mat=M, pre=pre, post=post, alpha=0.5)
+    ref = 0.5 * post[:, ls] * np.einsum('abl,bl->al', M[:, :, ls], pre[:, ls] * x)
+    assert rel(y, ref) < (1e-13 if prec == 64 else 1e-5)
+    d = K.mdot(T(x, cdt), T(x * 0.3 + 1j, cdt), lmax)
+    dref = np.sum((x * np.conj(x * 0.3 + 1j)).real * (1. + (ms > 0)))
+    assert abs(float(d) - dref) < (1e-12 if prec == 64 else 1e-5) * np.sum(np.abs(x) * np.abs(x * 0.3 + 1j) * 2)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_weight_radial_tau_outer(gpu, prec):
+    from polyspec_amd import kernels as K
+    rng = np.random.default_rng(5)
+    lmax, lmin, nr = 40, 2, 6
+    ls, ms = oracle.getlm(lmax)
+    rdt = torch.float64 if prec == 64 else torch.float32
+    cdt = torch.complex128 if prec == 64 else torch.complex64
+    tol = 1e-12 if prec == 64 else 2e-5
+    f = rng.standard_normal((lmax + 2, 3, nr))
+    h = rng.standard_normal((2, len(ls))) + 1j * rng.standard_normal((2, len(ls)))
+    ld = 16
+    out = torch.zeros((len(ls), ld), dtype=cdt, device='cuda')
+    K.weight_alm_lmajor(T(f, rdt), T(h, cdt), lmax, lmin, 0.7, out, col_offset=3)
+    sel = ls >= lmin
+    ref = np.zeros((nr, len(ls)), complex)
+    ref[:, sel] = oracle.apply_fl_weights(f, h[:, sel], ls[sel], 0.7)
+    got = out.cpu().numpy()
+    assert rel(got[:, 3:3 + nr].T, ref) < tol
+    assert np.all(got[:, :3] == 0) and np.all(got[:, 3 + nr:] == 0)
+
+    lm = rng.standard_normal((nr, len(ls))) + 1j * rng.standard_normal((nr, len(ls)))
+    w = rng.random(nr)
+    rs = K.radial_sum(T(lm, cdt), w, T(f, rdt), lmax, lmin, 3, scale=2.0)
+    ref = np.zeros((3, len(ls)), complex)
+    ref[:, sel] = 2.0 * oracle.radial_sum(lm[:, sel], w, f, ls[sel])
+    assert rel(rs, ref) < tol
+
+    # tau_sum_n0 over L in [Lmin, Lmax] with the (nmu, nr, nlm) layout
+    Lmaxs, Lmin, Lmax = 30, 1, 25
+    Ls, Ms = oracle.getlm(Lmaxs)
+    A = rng.standard_normal((1, nr, len(Ls))) + 1j * rng.standard_normal((1, nr, len(Ls)))
+    B = rng.standard_normal((1, nr, len(Ls))) + 1j * rng.standard_normal((1, nr, len(Ls)))
+    F = rng.standard_normal((Lmaxs + 1, nr, nr))
+    w2 = w.copy()
+    w2[1] = 0.
+    got = float(K.tau_sum_n0(T(A, cdt), T(B, cdt), F, w2, Lmaxs, Lmin, Lmax))
+    ref = oracle.tau_sum_n0(np.transpose(A, (0, 2, 1)), np.transpose(B, (0, 2, 1)), F, w2, Ls, Ms, Lmin, Lmax)
+    assert abs(got - ref) < (1e-11 if prec == 64 else 1e-5) * np.abs(A).sum() * np.abs(B).max() * np.abs(F).max()
+
+    # convolve_F into l-major columns
+    buf = torch.zeros((len(Ls), 8), dtype=cdt, device='cuda')
+    K.convolve_F_lmajor(T(A[0], cdt), F, w2, Lmaxs, Lmin, Lmax, buf)
+    Li = np.where((Ls >= Lmin) & (Ls <= Lmax), np.arange(len(Ls)), -1).astype(np.int32)
+    ref = oracle.convolve_F_n000(np.ascontiguousarray(A[0].T), F, w2, Ls, Li, len(Ls))
+    assert rel(buf.cpu().numpy()[:, :nr], ref) < tol
+
+    Qa = rng.standard_normal((4, 2, 300)) + 1j * rng.standard_normal((4, 2, 300))
+    Qb = rng.standard_normal((4, 2, 300)) + 1j * rng.standard_normal((4, 2, 300))
+    assert rel(K.outer_product(T(Qa, cdt), T(Qb, cdt)), oracle.outer_product(Qa, Qb)) < tol
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_pixel_kernels(gpu, prec):
+    from polyspec_amd import kernels as K
+    rng = np.random.default_rng(9)
+    nr, npix = 5, 3000
+    rdt = torch.float64 if prec == 64 else torch.float32
+    tol = 1e-12 if prec == 64 else 1e-5
+    a, b, c, d = (rng.standard_normal((nr, npix)) for _ in range(4))
+    w = rng.random(nr)
+    ta, tb, tc, td = (T(x, rdt) for x in (a, b, c, d))
+    assert rel(K.multiplyRR(ta, tb), a * b) < tol
+    assert rel(K.multiplyRR_sym(ta, tb, tc, td), a * b + c * d) < tol
+    assert rel(K.multiply(ta, tb, tc), a * b * c) < tol
+    assert rel(K.multiply_asym(ta, tb, tc, td), a * (2 * b * c + a * d)) < tol
+    s = float(K.gnl_loc_sum(w, ta, tb))
+    assert abs(s - oracle.gnl_loc_sum(w, a, b)) < tol * np.sum(w[:, None] * np.abs(a) ** 3 * np.abs(b))
+    s = float(K.gnl_loc_disc_sum(w, ta, tb, tc, td))
+    ref = oracle.gnl_loc_disc_sum(w, a, b, c, d)
+    assert abs(s - ref) < tol * np.sum(w[:, None] * (a * a * np.abs(b * d) + b * b * np.abs(a * c)))
+    U = torch.zeros_like(ta)
+    V = torch.zeros_like(ta)
+    K.mc_accumulate(torch.stack([tb, tc]).contiguous(), torch.stack([td, ta]).contiguous(), U, V)
+    assert rel(U, b * d + c * a) < tol and rel(V, b * b + c * c) < tol
+    f = float(K.gnl_loc_fused_sum(ta, tb, w, 0.3, U, V, -0.7))
+    ref = 0.3 * oracle.gnl_loc_sum(w, a, b) - 0.7 * (oracle.gnl_loc_disc_sum(w, a, b, b, d) +
+                                                      oracle.gnl_loc_disc_sum(w, a, c, b, a))
+    assert abs(f - ref) < 10 * tol * np.sum(w[:, None] * np.abs(a) ** 2 * (np.abs(a * b) + np.abs(U.cpu().numpy())
+                                                                           + np.abs(V.cpu().numpy())))
