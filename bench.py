@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-simulation FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-nodes", type=int, default=2)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run its own "
@@ -238,6 +240,15 @@ def main():
                      "algorithmic_flops_per_launch": flops},
         "mc_precompute_s": t_mc,
     }
+
+    if args.copy_calib:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
 
     if not args.no_plain:
         log("plain per-simulation FP64 path ...")

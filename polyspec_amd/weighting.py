@@ -133,9 +133,11 @@ class Weightings:
             out = K.lmat_apply(a, lmax, mat=Ltot, post=bl)
         elif input_type == 'map':
             m = input_map if not npy else torch.from_numpy(np.ascontiguousarray(input_map))
-            alm = self.base.to_lm(m.to(self.base.device, self.base.rdtype).reshape(self.nf, -1).contiguous())
-            if lmax != self.base.lmax:
-                alm = truncate_lm(alm, self.base.lmax, lmax)
+            # The reference transforms at base_lmax and keeps l <= lmax (weighting.py:210-214);
+            # the adjoint coefficients with l <= lmax do not depend on the higher l, so the
+            # transform is evaluated at lmax directly.
+            alm = self.base.to_lm(m.to(self.base.device, self.base.rdtype).reshape(self.nf, -1).contiguous(),
+                                  lmax=lmax)
             out = K.lmat_apply(alm, lmax, mat=Ltot, post=bl)
         else:
             raise AssertionError("input_type must be 'map' or 'harmonic'")
