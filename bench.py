@@ -120,6 +120,54 @@ def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None):
                          r_weights={'gNL-loc': w}, batch_sims=batch), W
 
 
+def lognormal_noise(base, mask, sigma_ln=0.5, fwhm_deg=5., seed=11, noise_uK_arcmin=30.):
+    """Pixel noise variance sigma0^2 exp(g), g a GRF smoothed on fwhm_deg with std sigma_ln
+    (T; twice that for Q, U), sigma0 from noise_uK_arcmin."""
+    import torch
+    L = base.lmax
+    l = np.arange(L + 1)
+    sig = np.radians(fwhm_deg) / np.sqrt(8 * np.log(2))
+    cl = np.exp(-l * (l + 1) * sig ** 2)
+    g = base.synalm(seed, {'TT': cl, 'EE': 0 * cl, 'TE': 0 * cl, 'BB': 0 * cl}, L)
+    gm = base.to_map(g)[0]
+    gm = gm.to(torch.float64) if isinstance(gm, torch.Tensor) else torch.from_numpy(gm)
+    gm = (gm - gm.mean()) / gm.std() * sigma_ln
+    s0 = (noise_uK_arcmin * np.pi / 10800.) ** 2 / base.A_pix
+    v = s0 * torch.exp(gm).cpu().numpy()
+    return np.stack([v, 2 * v, 2 * v])
+
+
+def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32):
+    """Inverse-variance CG (applySinv_optimal's solver) at base_lmax to rz/rz0 < thresh."""
+    import torch
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.weighting import CGD, Weightings
+    from polyspec_amd import kernels as K
+    raw, tot, beam = spectra(nside)
+    base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=precision)
+    base.grf_device = True
+    ncov = lognormal_noise(base, mask)
+    masks = np.repeat(mask[None], 3, 0)
+    W = Weightings(base, masks, dict(raw), noise_cov=ncov)
+    sig = base.generate_data(seed=int(8e6))
+    noise = torch.from_numpy(np.sqrt(ncov)).to(base.device, base.rdtype) * torch.randn(
+        (3, base.Npix), dtype=base.rdtype, device=base.device, generator=torch.Generator(device=base.device).manual_seed(5))
+    d = ((sig + noise) * torch.from_numpy(masks).to(base.device, base.rdtype)).contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a = base.to_lm(d)
+    x0 = K.lmat_apply(K.lmat_apply(a, base.lmax, mat=W.invCtot_d, post=W.beam_l_d), base.lmax, mat=W.Cl_l_d)
+    cg = CGD(W, d, x0=x0, preconditioner='harmonic')
+    cg.run(nstep=nstep, thresh=thresh)
+    out = cg.get_ivar_lm()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"iterations": cg.i, "converged": cg.err < thresh, "wall_s": dt, "ms_per_iteration": dt / max(cg.i, 1) * 1e3,
+            "final_rz_over_rz0": cg.err, "final_r_over_b": cg.history[-1][2] if cg.history else None,
+            "threshold": thresh, "preconditioner": "harmonic (Sigma_lm, weighting.py:304-305)",
+            "lmax": base.lmax, "precision": "FP%d SHTs, FP64 dot products" % precision}
+
+
 def cpu_baseline(data_np, base, tm, nr_sample, lmax):
     """FP64 oracle on the host: S^-1 of the data plus `nr_sample` of the 40 radial
     nodes (2 syntheses each + gnl sums), extrapolated to one numerator."""
@@ -166,6 +214,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-simulation FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-nodes", type=int, default=2)
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG S^-1 section")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -276,6 +325,13 @@ def main():
             (t_plain, value * t_plain, (diff / sigma) if sigma else None))
         del tm64, base64
         torch.cuda.empty_cache()
+
+    if not args.no_cg:
+        log("CG S^-1 (harmonic preconditioner) ...")
+        cg = bench_cg(nside, mask)
+        result["cg"] = {"plain": cg}
+        log("cg: %d iterations, %.2f s (%.1f ms/iteration), rz/rz0 %.2e, |r|/|b| %s" %
+            (cg["iterations"], cg["wall_s"], cg["ms_per_iteration"], cg["final_rz_over_rz0"], cg["final_r_over_b"]))
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")

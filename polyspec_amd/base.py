@@ -307,7 +307,10 @@ class PolySpec:
         for a in range(nf):
             for b in range(nf):
                 k = keys[a][b]
-                C[:, a, b] = np.asarray(Cl_input[k] if k in Cl_input else Cl_input[k[::-1]])[:lmax + 1]
+                if k in Cl_input:
+                    C[:, a, b] = np.asarray(Cl_input[k])[:lmax + 1]
+                elif k[::-1] in Cl_input:
+                    C[:, a, b] = np.asarray(Cl_input[k[::-1]])[:lmax + 1]  # TB/EB may be absent: zero
         Lc = np.zeros_like(C)
         for l in range(lmax + 1):
             # eigen-decomposition square root: also valid for singular C_l

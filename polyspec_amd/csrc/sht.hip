@@ -1002,30 +1002,41 @@ size_t sht_phase_bytes(const ShtPlan& p, int nmap) {
     return (size_t)nmap * p.ncomp * (p.mmax + 1) * p.nrings * 2 * p.prec;
 }
 
+// Column-chunk size of the synthesis kernels: the smallest instantiated chunk covering
+// the batch (a single map uses BC = 1 instead of wasting the padding columns).
+template <typename T>
+static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st) {
+    constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
+    if (ncol <= 1) return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
+    if (ncol <= 2) return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
+    if (ncol <= 4) return launch_synth_s0<T, 4>(p, ncol, ld, alm_t, phase, st);
+    if (BCMAX > 8 && ncol <= 8) return launch_synth_s0<T, 8>(p, ncol, ld, alm_t, phase, st);
+    return launch_synth_s0<T, BCMAX>(p, ncol, ld, alm_t, phase, st);
+}
+
+template <typename T>
+static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* alm_t, void* phase, hipStream_t st) {
+    constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
+    if (nmap <= 1) return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
+    if (BCMAX > 2 && nmap <= 2) return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
+    return launch_synth_sp<T, BCMAX>(p, nmap, ld, alm_t, phase, st);
+}
+
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
                          void* phase, hipStream_t st) {
-    const int ncol = nmap * p.ncomp;
-    int rc;
-    if (p.prec == 4) {
-        rc = p.spin == 0 ? launch_synth_s0<float, SYN_BC_S0_F>(p, ncol, ld, alm_t, phase, st)
-                         : launch_synth_sp<float, SYN_BC_SP_F>(p, nmap, ld, alm_t, phase, st);
-        if (rc) return rc;
-        return launch_ring_synth<float>(p, ncol, phase, map, map_stride, st);
-    }
-    rc = p.spin == 0 ? launch_synth_s0<double, SYN_BC_S0_D>(p, ncol, ld, alm_t, phase, st)
-                     : launch_synth_sp<double, SYN_BC_SP_D>(p, nmap, ld, alm_t, phase, st);
+    int rc = sht_legendre_synthesis_lmajor(p, nmap, alm_t, ld, phase, st);
     if (rc) return rc;
-    return launch_ring_synth<double>(p, ncol, phase, map, map_stride, st);
+    return sht_ring_synthesis(p, nmap, phase, map, map_stride, st);
 }
 
 int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
                                   hipStream_t st) {
     const int ncol = nmap * p.ncomp;
     if (p.prec == 4)
-        return p.spin == 0 ? launch_synth_s0<float, SYN_BC_S0_F>(p, ncol, ld, alm_t, phase, st)
-                           : launch_synth_sp<float, SYN_BC_SP_F>(p, nmap, ld, alm_t, phase, st);
-    return p.spin == 0 ? launch_synth_s0<double, SYN_BC_S0_D>(p, ncol, ld, alm_t, phase, st)
-                       : launch_synth_sp<double, SYN_BC_SP_D>(p, nmap, ld, alm_t, phase, st);
+        return p.spin == 0 ? launch_synth_s0_any<float>(p, ncol, ld, alm_t, phase, st)
+                           : launch_synth_sp_any<float>(p, nmap, ld, alm_t, phase, st);
+    return p.spin == 0 ? launch_synth_s0_any<double>(p, ncol, ld, alm_t, phase, st)
+                       : launch_synth_sp_any<double>(p, nmap, ld, alm_t, phase, st);
 }
 
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st) {
