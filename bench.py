@@ -137,35 +137,91 @@ def lognormal_noise(base, mask, sigma_ln=0.5, fwhm_deg=5., seed=11, noise_uK_arc
     return np.stack([v, 2 * v, 2 * v])
 
 
-def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32):
-    """Inverse-variance CG (applySinv_optimal's solver) at base_lmax to rz/rz0 < thresh."""
+def _cg_trace(W, d, precond, nstep, thresh_rz=1e-6, rb_marks=(1e-3, 1e-4)):
+    """One solve from the Wiener start, timed to the reference stopping rule rz/rz0 < thresh_rz
+    (weighting.py:342-351), then continued to record when |r|/|b| first passes rb_marks."""
+    import torch
+    from polyspec_amd.learned import FlexibleCGD
+    from polyspec_amd.weighting import CGD
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x0 = W.wiener_start(d)
+    cg = CGD(W, d, x0=x0, preconditioner='harmonic') if precond is None else FlexibleCGD(W, d, precond, x0=x0)
+    out = {"iterations": None, "wall_s": None}
+    marks = {}
+    for _ in range(nstep):
+        cg.step()
+        if out["iterations"] is None and cg.err < thresh_rz:
+            torch.cuda.synchronize()
+            out.update(iterations=cg.i, wall_s=time.perf_counter() - t0, final_r_over_b=cg.history[-1][2])
+        for th in rb_marks:
+            if cg.history[-1][2] < th and th not in marks:
+                marks[th] = {"iterations": cg.i, "wall_s": time.perf_counter() - t0}
+        if out["iterations"] is not None and len(marks) == len(rb_marks):
+            break
+    out["ms_per_iteration"] = out["wall_s"] / out["iterations"] * 1e3 if out["iterations"] else None
+    out["r_over_b_marks"] = {"%.0e" % k: v for k, v in sorted(marks.items(), reverse=True)}
+    out["fallback_at"] = getattr(cg, "fallback_at", None)
+    return out
+
+
+def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=2000, nside_lo=128):
+    """Inverse-variance CG (applySinv_optimal's solver) at base_lmax: plain harmonic CG against the
+    flexible CG with the learned preconditioner, both judged by rz/rz0 < thresh."""
     import torch
     from polyspec_amd.base import PolySpec
-    from polyspec_amd.weighting import CGD, Weightings
-    from polyspec_amd import kernels as K
+    from polyspec_amd.learned import LearnedPreconditioner
+    from polyspec_amd.weighting import Weightings
     raw, tot, beam = spectra(nside)
     base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=precision)
     base.grf_device = True
     ncov = lognormal_noise(base, mask)
     masks = np.repeat(mask[None], 3, 0)
     W = Weightings(base, masks, dict(raw), noise_cov=ncov)
-    sig = base.generate_data(seed=int(8e6))
-    noise = torch.from_numpy(np.sqrt(ncov)).to(base.device, base.rdtype) * torch.randn(
-        (3, base.Npix), dtype=base.rdtype, device=base.device, generator=torch.Generator(device=base.device).manual_seed(5))
-    d = ((sig + noise) * torch.from_numpy(masks).to(base.device, base.rdtype)).contiguous()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    a = base.to_lm(d)
-    x0 = K.lmat_apply(K.lmat_apply(a, base.lmax, mat=W.invCtot_d, post=W.beam_l_d), base.lmax, mat=W.Cl_l_d)
-    cg = CGD(W, d, x0=x0, preconditioner='harmonic')
-    cg.run(nstep=nstep, thresh=thresh)
-    out = cg.get_ivar_lm()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    return {"iterations": cg.i, "converged": cg.err < thresh, "wall_s": dt, "ms_per_iteration": dt / max(cg.i, 1) * 1e3,
-            "final_rz_over_rz0": cg.err, "final_r_over_b": cg.history[-1][2] if cg.history else None,
-            "threshold": thresh, "preconditioner": "harmonic (Sigma_lm, weighting.py:304-305)",
-            "lmax": base.lmax, "precision": "FP%d SHTs, FP64 dot products" % precision}
+    mk = torch.from_numpy(masks).to(base.device, base.rdtype)
+    sq = torch.from_numpy(np.sqrt(ncov)).to(base.device, base.rdtype)
+    sigcl = {k: np.asarray(v) * beam[0] ** 2 for k, v in raw.items()}
+
+    def data(i):
+        sig = torch.as_tensor(base.generate_data(seed=int(8e6) + i, Cl_input=dict(sigcl))).to(base.device, base.rdtype)
+        g = torch.Generator(device=base.device).manual_seed(5 + i)
+        n = sq * torch.randn((3, base.Npix), dtype=base.rdtype, device=base.device, generator=g)
+        return ((sig + n) * mk).contiguous()
+    d = data(0)
+    info = {"lmax": base.lmax, "threshold": thresh, "precision": "FP%d SHTs, FP64 dot products" % precision,
+            "data": "masked signal + lognormal noise (sigma_ln=0.5, 5 deg), seed-separated from the training data"}
+    plain = _cg_trace(W, d, None, nstep, thresh)
+    plain["preconditioner"] = "harmonic (Sigma_lm, weighting.py:304-305)"
+    log("cg plain: %s" % plain)
+    res = {"plain": plain, "setup": info}
+    if learned_pairs:
+        LP = LearnedPreconditioner(W, nside_lo=nside_lo)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, T = LP.collect((data(1 + i) for i in range(10 ** 4)), max_pairs=learned_pairs, iters=40)
+        torch.cuda.synchronize()
+        t_col = time.perf_counter() - t0
+        nv = max(8, X.shape[0] // 20)
+        before = LP.relative_loss(X[-nv:], T[-nv:])
+        t0 = time.perf_counter()
+        hist = LP.train(X[:-nv], T[:-nv], epochs=4, batch=4, lr=1e-3)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        after = LP.relative_loss(X[-nv:], T[-nv:])
+        del X, T
+        torch.cuda.empty_cache()
+        LP.time, LP.n_calls = 0., 0
+        learned = _cg_trace(W, d, LP, nstep, thresh)
+        learned.update({"preconditioner": "Sigma + coarse stencil network (Nside_lo=%d, l<=%d, 4 layers x 64 ch), "
+                                          "flexible CG, plain CG as judge and fallback" % (nside_lo, LP.Llo),
+                        "preconditioner_ms_per_call": LP.time / max(LP.n_calls, 1) * 1e3,
+                        "training": {"pairs": learned_pairs, "collect_s": t_col, "train_s": t_train, "epochs": 4,
+                                     "loss_per_epoch": hist,
+                                     "heldout_rel_error_sigma_alone": before,
+                                     "heldout_rel_error_learned": after}})
+        log("cg learned: %s" % learned)
+        res["learned"] = learned
+    return res
 
 
 def cpu_baseline(data_np, base, tm, nr_sample, lmax):
@@ -215,6 +271,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-nodes", type=int, default=2)
     ap.add_argument("--no-cg", action="store_true", help="skip the CG S^-1 section")
+    ap.add_argument("--learned-pairs", type=int, default=2000,
+                    help="training pairs for the learned CG preconditioner (0 = plain CG only)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -327,11 +385,8 @@ def main():
         torch.cuda.empty_cache()
 
     if not args.no_cg:
-        log("CG S^-1 (harmonic preconditioner) ...")
-        cg = bench_cg(nside, mask)
-        result["cg"] = {"plain": cg}
-        log("cg: %d iterations, %.2f s (%.1f ms/iteration), rz/rz0 %.2e, |r|/|b| %s" %
-            (cg["iterations"], cg["wall_s"], cg["ms_per_iteration"], cg["final_rz_over_rz0"], cg["final_r_over_b"]))
+        log("CG S^-1: plain harmonic vs learned preconditioner ...")
+        result["cg"] = bench_cg(nside, mask, learned_pairs=args.learned_pairs, nside_lo=min(128, nside))
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")

@@ -175,6 +175,44 @@ int ps_cg_update_xr(int precision, long long n, const double *rz, const double *
 int ps_cg_update_p(int precision, long long n, const double *num, const double *den, const void *z, void *p,
                    void *stream);
 
+/* -------------------------------------------- learned CG preconditioner
+ * Kernels of a 9-point-stencil convolutional network on HEALPix maps (FP32),
+ * used as the nonlinear preconditioner of a flexible CG (no counterpart in the
+ * reference, whose CG uses the 'harmonic' / 'pseudo_inverse' preconditioners of
+ * weighting.py:301-324).  Maps are channels-last (B, npix, C); a layer's weights
+ * are W (9*cin, cout) row-major, its stencil table nbr (npix, 9) (-1 = absent). */
+
+/* Xcol[(b,p), k*cin + c] = X[b, nbr[p,k], c] (0 where absent). */
+int ps_nn_im2col9(const float *X, const int *nbr, int B, int npix, int cin, float *Xcol, void *stream);
+
+/* dX[b,q,c] = sum_j G[(b,p_j), k_j*cin + c] over rev[q, j] = p_j*9 + k_j (j < rw, -1 unused). */
+int ps_nn_col2im9(const float *G, const int *rev, int rw, int B, int npix, int cin, float *dX, void *stream);
+
+/* C (M x N) = op(A) op(B) (+ bias, ReLU when ksplit == 1); op = transpose when trans* != 0.
+ * ksplit > 1 splits K and needs work of ksplit*M*N floats (then C must be dense, ldc = N). */
+int ps_nn_gemm(int transA, int transB, int M, int N, int K, const float *A, long long lda, const float *B,
+               long long ldb, float *C, long long ldc, const float *bias, int relu, int ksplit, float *work,
+               void *stream);
+
+/* dY *= (Y > 0). */
+int ps_nn_relu_backward(long long n, const float *Y, float *dY, void *stream);
+
+/* out[n] = sum_m dY[m, n]; work of ps_nn_colsum_work_floats(M, N) floats. */
+long long ps_nn_colsum_work_floats(int M, int N);
+int ps_nn_colsum(int M, int N, const float *dY, float *work, float *out, void *stream);
+
+/* Adam step t (>= 1) on n parameters. */
+int ps_nn_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+               int t, void *stream);
+
+/* loss = sum_b scale[b] sum_i (Y - T)^2 over B samples of per_sample values (FP64, device scalar);
+ * dY = 2 scale[b] (Y - T) if dY != NULL.  work: 512 doubles. */
+int ps_nn_mse(long long per_sample, int B, const float *Y, const float *T, const double *scale, float *dY,
+              double *work, double *loss, void *stream);
+
+/* Y[b,p,j] = w[p] X[b,p,j] (per-pixel gate; Y == X allowed). */
+int ps_nn_pixel_scale(int B, int npix, int c, const float *w, const float *X, float *Y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,15 @@ SIGNATURES = {
     "ps_mdot": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     "ps_cg_update_xr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "ps_cg_update_p": (_I, [_I, _LL, _P, _P, _P, _P, _P]),
+    "ps_nn_im2col9": (_I, [_P, _P, _I, _I, _I, _P, _P]),
+    "ps_nn_col2im9": (_I, [_P, _P, _I, _I, _I, _I, _P, _P]),
+    "ps_nn_gemm": (_I, [_I, _I, _I, _I, _I, _P, _LL, _P, _LL, _P, _LL, _P, _I, _I, _P, _P]),
+    "ps_nn_relu_backward": (_I, [_LL, _P, _P, _P]),
+    "ps_nn_colsum_work_floats": (_LL, [_I, _I]),
+    "ps_nn_colsum": (_I, [_I, _I, _P, _P, _P, _P]),
+    "ps_nn_adam": (_I, [_LL, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _P]),
+    "ps_nn_mse": (_I, [_LL, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ps_nn_pixel_scale": (_I, [_I, _I, _I, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -143,6 +143,24 @@ class Weightings:
             raise AssertionError("input_type must be 'map' or 'harmonic'")
         return self.base._ret(out, npy)
 
+    def apply_A(self, x):
+        """The CG system matrix A x = C^-1 x + B Y^dag W N^-1 W Y B x at base_lmax
+        (weighting.py:297-299); x (nf, nlm(base_lmax)) on the device."""
+        L = self.base.lmax
+        bx = K.lmat_apply(x, L, post=self.beam_l_d)
+        m = self.base.to_map(bx)
+        m = K.multiplyRR(m, self.wpix_d)
+        a = self.base.to_lm(m)
+        out = K.lmat_apply(x, L, mat=self.inv_Cl_l_d)
+        K.lmat_apply(a, L, post=self.beam_l_d, out=out, accumulate=True)
+        return out
+
+    def wiener_start(self, d):
+        """CG starting point x0 = C_raw B C_tot^-1 Y^dag d (weighting.py:174), d (nf, Npix) on the device."""
+        a = self.base.to_lm(d)
+        t = K.lmat_apply(a, self.base.lmax, mat=self.invCtot_d, post=self.beam_l_d)
+        return K.lmat_apply(t, self.base.lmax, mat=self.Cl_l_d)
+
     def applySinv_optimal(self, input_map, preconditioner='pseudo_inverse', nstep=25, thresh=1e-4, verb=False,
                           input_type='map', lmax=None):
         """Optimal S^-1 by CG from the Wiener start (weighting.py:148-184)."""
@@ -154,12 +172,18 @@ class Weightings:
         npy = not isinstance(input_map, torch.Tensor)
         d = input_map if not npy else torch.from_numpy(np.ascontiguousarray(input_map))
         d = d.to(self.base.device, self.base.rdtype).reshape(self.nf, -1).contiguous()
-        # x0 = C_raw B C_tot^-1 Y^dag d (weighting.py:174)
-        a = self.base.to_lm(d)
-        t = K.lmat_apply(a, self.base.lmax, mat=self.invCtot_d, post=self.beam_l_d)
-        x0 = K.lmat_apply(t, self.base.lmax, mat=self.Cl_l_d)
-        cgd = CGD(self, d, x0=x0, preconditioner=preconditioner, verb=verb)
+        x0 = self.wiener_start(d)
+        if preconditioner == 'learned':
+            # flexible CG with the learned preconditioner (no counterpart in the reference)
+            from .learned import FlexibleCGD
+            if getattr(self, 'learned', None) is None:
+                raise Exception("preconditioner='learned' needs a trained LearnedPreconditioner in self.learned")
+            self.learned.enabled = True
+            cgd = FlexibleCGD(self, d, self.learned, x0=x0, verb=verb)
+        else:
+            cgd = CGD(self, d, x0=x0, preconditioner=preconditioner, verb=verb)
         cgd.run(nstep=nstep, thresh=thresh)
+        self.last_cg = cgd
         out = cgd.get_ivar_lm()
         if lmax != self.base.lmax:
             out = truncate_lm(out, self.base.lmax, lmax)
@@ -184,13 +208,11 @@ def truncate_lm(a, lmax_in, lmax_out):
 class CGD:
     """Conjugate-gradient solver of A x = b (weighting.py:217-367) on the device."""
 
-    def __init__(self, weightings, d, x0=[], test_b=[], preconditioner='pseudo_inverse', verb=False,
-                 learned=None):
+    def __init__(self, weightings, d, x0=[], test_b=[], preconditioner='pseudo_inverse', verb=False):
         self.preconditioner = preconditioner
         self.base = weightings.base
         self.weightings = weightings
         self.verb = verb
-        self.learned = learned
         self.L = self.base.lmax
         W = weightings
         if len(test_b) == 0:
@@ -217,6 +239,8 @@ class CGD:
         self.bnorm = float(torch.sqrt(self.dot(b0, b0)))
         self.err = float(self.rz / self.rz0)
         self.history = []
+        self.callback = None
+        self.keep_Ap = False
 
     def dot(self, a, b):
         """Re sum a conj(b) (1 + [m > 0]) in FP64 (weighting.py:255); device scalar."""
@@ -231,14 +255,7 @@ class CGD:
 
     def _apply_A(self, x):
         """A x = C^-1 x + B Y^dag W N^-1 W Y B x (weighting.py:297-299)."""
-        W = self.weightings
-        bx = K.lmat_apply(x, self.L, post=W.beam_l_d)
-        m = self.base.to_map(bx)
-        m = K.multiplyRR(m, W.wpix_d)
-        a = self.base.to_lm(m)
-        out = K.lmat_apply(x, self.L, mat=W.inv_Cl_l_d)
-        K.lmat_apply(a, self.L, post=W.beam_l_d, out=out, accumulate=True)
-        return out
+        return self.weightings.apply_A(x)
 
     def _apply_M(self, r):
         """Preconditioner (weighting.py:301-324): 'harmonic' Sigma r, or 'pseudo_inverse'."""
@@ -261,6 +278,9 @@ class CGD:
         """One CG iteration (weighting.py:326-340), Fletcher-Reeves beta."""
         Ap = self._apply_A(self.p)
         pAp = self.dot(self.p, Ap)
+        if self.keep_Ap:
+            # exact (A p, p) pair of this iteration, for training a learned preconditioner
+            self.last_p, self.last_Ap = self.p.clone(), Ap
         K.cg_update_xr(self.rz, pAp, self.p, Ap, self.x, self.r)
         del Ap
         z = self._apply_M(self.r)
@@ -272,6 +292,8 @@ class CGD:
         self.err = float(self.rz / self.rz0)
         rn = float(torch.sqrt(self.dot(self.r, self.r)))
         self.history.append((self.i, self.err, rn / self.bnorm))
+        if self.callback is not None:
+            self.callback(self)
 
     def run(self, nstep=10, thresh=1e-6):
         """Iterate until rz/rz0 < thresh (weighting.py:342-351)."""

@@ -1,0 +1,219 @@
+"""Learned preconditioner on the GPU: the stencil network's forward pass and every
+parameter gradient against the FP64 oracle (itself pinned by finite differences on
+the CPU), the GEMM kernel in all four transposition modes and split-K against an
+FP64 product, Adam against its closed form, and the flexible CG's fallback."""
+import numpy as np
+import pytest
+
+from oracle import nn as onn
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(rng, ch):
+    out = []
+    for i in range(len(ch) - 1):
+        out.append((rng.standard_normal((9 * ch[i], ch[i + 1])) * np.sqrt(2. / (9 * ch[i])),
+                    rng.standard_normal(ch[i + 1]) * 0.1))
+    return out
+
+
+@pytest.mark.parametrize("nside,ch,B,gated", [(4, (5, 8, 8, 3), 1, False), (8, (11, 64, 64, 64, 3), 2, True)])
+def test_net_forward_backward_match_oracle(gpu, nside, ch, B, gated):
+    import torch
+    from polyspec_amd.learned import StencilNet, ring_stencil
+    print(f"net nside={nside} ch={ch} B={B}", flush=True)
+    rng = np.random.default_rng(4)
+    nbr, rev = ring_stencil(nside)
+    params = _params(rng, ch)
+    X = rng.standard_normal((B, nbr.shape[0], ch[0]))
+    T = rng.standard_normal((B, nbr.shape[0], ch[-1]))
+    s = rng.uniform(0.5, 1.5, B)
+    gate = rng.uniform(0, 1, nbr.shape[0]).astype(np.float32).astype(np.float64) if gated else None
+    net = StencilNet(nbr, rev, ch, gate=gate)
+    net.set_params(params)
+    # the oracle sees the FP32-rounded parameters and inputs the kernels see
+    p32 = [(W.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)) for W, b in params]
+    X32 = X.astype(np.float32).astype(np.float64)
+    grads, Yo = onn.backward(X32, T, s, p32, nbr, gate)
+    Xd = torch.from_numpy(X32).float().cuda()
+    Y = net.forward(Xd, keep=True)
+    Yg = Y.double().cpu().numpy()
+    assert np.abs(Yg - Yo).max() < 2e-5 * np.abs(Yo).max()
+    loss, dY = net.mse(Y, torch.from_numpy(T).float().cuda(), s)
+    assert abs(float(loss) - onn.loss(Yo, T.astype(np.float32).astype(np.float64), s)) < 1e-4 * float(loss)
+    g = net.backward(dY)
+    for i, ((dW, db), (oW, ob)) in enumerate(zip(g, grads)):
+        dW, db = dW.double().cpu().numpy(), db.double().cpu().numpy()
+        assert np.abs(dW - oW).max() < 1e-4 * np.abs(oW).max(), ("dW", i)
+        assert np.abs(db - ob).max() < 1e-4 * np.abs(ob).max(), ("db", i)
+
+
+@pytest.mark.parametrize("ta,tb,ks", [(0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 5), (0, 0, 3)])
+def test_gemm_matches_fp64(gpu, ta, tb, ks):
+    import torch
+    from polyspec_amd._abi import check, load, ptr, stream_handle
+    M, N, K = 131, 70, 203
+    rng = np.random.default_rng(ta * 4 + tb * 2 + ks)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    Bm = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    ref = (A.T if ta else A).astype(np.float64) @ (Bm.T if tb else Bm).astype(np.float64)
+    use_bias = ks == 1
+    if use_bias:
+        ref = np.maximum(ref + bias, 0.)
+    Ad, Bd = torch.from_numpy(A).cuda(), torch.from_numpy(Bm).cuda()
+    C = torch.full((M, N), np.nan, dtype=torch.float32, device='cuda')
+    work = torch.empty(ks * M * N, dtype=torch.float32, device='cuda')
+    bd = torch.from_numpy(bias).cuda()
+    check(load().ps_nn_gemm(ta, tb, M, N, K, ptr(Ad), A.shape[1], ptr(Bd), Bm.shape[1], ptr(C), N,
+                            ptr(bd) if use_bias else None, int(use_bias), ks, ptr(work), stream_handle()))
+    got = C.double().cpu().numpy()
+    assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max() * np.sqrt(K)
+
+
+def test_adam_closed_form(gpu):
+    import torch
+    from polyspec_amd._abi import check, load, ptr, stream_handle
+    rng = np.random.default_rng(9)
+    n = 1000
+    p = rng.standard_normal(n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    pd, md, vd = (torch.from_numpy(a.copy()).cuda() for a in (p, m, v))
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    pr, mr, vr = p.astype(np.float64), m.astype(np.float64), v.astype(np.float64)
+    for t in range(1, 4):
+        g = rng.standard_normal(n).astype(np.float32)
+        check(load().ps_nn_adam(n, ptr(pd), ptr(torch.from_numpy(g).cuda()), ptr(md), ptr(vd), lr, b1, b2, eps, t,
+                                stream_handle()))
+        mr = b1 * mr + (1 - b1) * g
+        vr = b2 * vr + (1 - b2) * g.astype(np.float64) ** 2
+        pr = pr - lr * (mr / (1 - b1 ** t)) / (np.sqrt(vr / (1 - b2 ** t)) + eps)
+    assert np.abs(pd.double().cpu().numpy() - pr).max() < 1e-5
+
+
+def test_training_reduces_loss(gpu):
+    """A few hundred Adam steps fit a fixed linear stencil map (the target is a 9-point
+    smoothing of the first input channel): the relative loss falls by an order of magnitude."""
+    import torch
+    from polyspec_amd.learned import StencilNet, ring_stencil
+    nbr, rev = ring_stencil(8)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((16, nbr.shape[0], 5))
+    Wt = np.zeros((45, 1))
+    Wt[0::5, 0] = 1. / 9
+    T = onn.forward(X, [(Wt, np.zeros(1))], nbr)[1][0]
+    net = StencilNet(nbr, rev, (5, 16, 16, 1), seed=1)
+    Xd, Td = torch.from_numpy(X).float().cuda(), torch.from_numpy(T).float().cuda()
+    norm = (T ** 2).sum(axis=(1, 2))
+    first = None
+    for it in range(300):
+        idx = rng.choice(16, 4, replace=False)
+        y = net.forward(Xd[idx].contiguous(), keep=True)
+        loss, dY = net.mse(y, Td[idx].contiguous(), 1. / (4 * norm[idx]))
+        net.adam_step(net.backward(dY), lr=3e-3)
+        if first is None:
+            first = float(loss)
+    y = net.forward(Xd)
+    rel = float((((y.double() - Td.double()) ** 2).sum(dim=(1, 2)).cpu().numpy() / norm).mean())
+    assert rel < 0.1 * first, (first, rel)
+
+
+def _cg_problem(nside, seed=0):
+    import torch
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.weighting import Weightings
+    from estimator_setup import grf_alm, spectra
+    import oracle
+    rng = np.random.default_rng(seed)
+    raw, tot, beam = spectra(nside, True)
+    th, ph = oracle.pix_angles(nside)
+    lat = 90 - np.degrees(th)
+    mask = (np.abs(lat) > 20.).astype(float)
+    L = 3 * nside - 1
+    g = oracle.to_map(grf_alm(rng, {'TT': 1. / (1 + np.arange(L + 1)) ** 2}, L, False), nside, L)[0]
+    g = 0.5 * g / g.std()
+    sig0 = (30 * np.pi / 10800.) ** 2 / (4 * np.pi / (12 * nside ** 2))
+    ncov = np.array([sig0 * np.exp(g) * (1 if i == 0 else 2) for i in range(3)])
+    masks = np.repeat(mask[None], 3, 0)
+    base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=64)
+    W = Weightings(base, masks, dict(raw), noise_cov=ncov)
+
+    def data(i):
+        r = np.random.default_rng(100 + i)
+        m = oracle.to_map(grf_alm(r, tot, L, True), nside, L) + np.sqrt(ncov) * r.standard_normal(ncov.shape)
+        return torch.from_numpy(masks * m).cuda()
+    return base, W, data
+
+
+def _solve(W, d, precond=None, nstep=400, thresh=1e-6):
+    from polyspec_amd.learned import FlexibleCGD
+    from polyspec_amd.weighting import CGD
+    x0 = W.wiener_start(d)
+    cg = CGD(W, d, x0=x0, preconditioner='harmonic') if precond is None else FlexibleCGD(W, d, precond, x0=x0)
+    cg.run(nstep=nstep, thresh=thresh)
+    return cg
+
+
+def test_learned_preconditioner_gain(gpu):
+    """Train on exact (A p, p) pairs of plain solves, then measure on unseen data.
+    The gain is measured, with bounds the result must keep:
+    - the network's held-out coarse error relative to Sigma alone (|f - T|^2 / |T|^2,
+      T = g P(A^-1 r - Sigma r) / s) falls well below 1;
+    - the flexible CG, judged by the reference statistic, reaches 1e-6 within a few
+      iterations of plain CG, and its solution passes the same judge."""
+    import torch
+    from polyspec_amd.learned import LearnedPreconditioner
+    print("learned gain nside=64 nside_lo=32", flush=True)
+    base, W, data = _cg_problem(64)
+    LP = LearnedPreconditioner(W, nside_lo=32, seed=0)
+    X, T = LP.collect((data(i) for i in range(40)), max_pairs=400, iters=40)
+    Xv, Tv = X[-40:], T[-40:]
+    before = LP.relative_loss(Xv, Tv)
+    LP.train(X[:-40], T[:-40], epochs=6, batch=4, lr=1e-3)
+    after = LP.relative_loss(Xv, Tv)
+    d = data(999)
+    plain = _solve(W, d)
+    learned = _solve(W, d, LP)
+    it_rb = lambda cg, th: next((h[0] for h in cg.history if h[2] < th), None)
+    print(f"held-out relative error {before:.3f} -> {after:.3f}; iterations to rz<1e-6 plain {plain.i} "
+          f"learned {learned.i} (fallback at {learned.fallback_at}); to |r|/|b|<1e-3 plain {it_rb(plain, 1e-3)} "
+          f"learned {it_rb(learned, 1e-3)}", flush=True)
+    assert 0.9 < before < 1.1
+    assert after < 0.6
+    assert plain.err < 1e-6 and learned.err < 1e-6
+    assert learned.i <= plain.i + 10
+    # the learned solve's answer passes the plain judge (the statistic itself is the judge)
+    assert learned.history[-1][1] < 1e-6
+
+
+def test_flexible_cg_falls_back_on_poisoned_network(gpu):
+    """A network with NaN weights makes r^T M r non-finite at the first application:
+    the solver drops it at once and reproduces plain CG iterate for iterate."""
+    import torch
+    from polyspec_amd.learned import LearnedPreconditioner
+    base, W, data = _cg_problem(32)
+    LP = LearnedPreconditioner(W, nside_lo=16)
+    LP.net.params[-1][0].fill_(float('nan'))
+    d = data(1)
+    plain = _solve(W, d)
+    fl = _solve(W, d, LP)
+    assert fl.fallback_at == 0 and not LP.enabled
+    assert fl.i == plain.i
+    assert np.allclose([h[1] for h in fl.history], [h[1] for h in plain.history], rtol=1e-10, atol=0)
+
+
+def test_flexible_cg_falls_back_when_judge_stalls(gpu):
+    """A finite but harmful network (outputs scaled up 1e4) stalls the judge statistic; the
+    solver falls back to plain CG and still reaches 1e-6."""
+    from polyspec_amd.learned import LearnedPreconditioner
+    base, W, data = _cg_problem(32)
+    LP = LearnedPreconditioner(W, nside_lo=16, seed=3)
+    W_last, b_last = LP.net.params[-1]
+    W_last.mul_(1e4)
+    b_last.fill_(1e3)
+    d = data(2)
+    fl = _solve(W, d, LP, nstep=600)
+    assert fl.err < 1e-6
+    assert fl.fallback_at is not None
