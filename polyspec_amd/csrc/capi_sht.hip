@@ -3,6 +3,8 @@
 #include <new>
 #include <string>
 
+#include <algorithm>
+
 #include "polyspec_hip.h"
 #include "ps_common.h"
 #include "sht.h"
@@ -85,6 +87,8 @@ long long ps_sht_workspace_bytes(const ps_sht_plan* plan, int nmaps, int lmajor_
         long ncol = (long)nmaps * plan->p.ncomp;
         long ld = (ncol + mult - 1) / mult * mult;
         b += (size_t)ld * plan->p.nlm * 2 * plan->p.prec;
+    } else {
+        b = std::max(b, sht_adjoint_work_bytes(plan->p, nmaps));
     }
     return (long long)b;
 }

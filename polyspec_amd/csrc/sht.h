@@ -30,6 +30,8 @@ struct ShtPlan {
 
 int sht_synth_lmajor_ld_multiple(const ShtPlan& p);
 size_t sht_phase_bytes(const ShtPlan& p, int nmap);
+int sht_adjoint_nsplit(const ShtPlan& p, int nmap);
+size_t sht_adjoint_work_bytes(const ShtPlan& p, int nmap);
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
                          void* phase, hipStream_t st);
 int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
@@ -37,7 +39,8 @@ int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t,
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st);
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
                       hipStream_t st);
+// work: sht_adjoint_work_bytes(p, nmap) bytes (ring phases, then ring-slice partial sums)
 int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, void* alm, long alm_stride,
-                double scale, void* phase, hipStream_t st);
+                double scale, void* work, hipStream_t st);
 
 }  // namespace ps

@@ -345,19 +345,36 @@ __device__ __forceinline__ int m_of_q(int q, int mmax) {
     return (q & 1) ? (mmax - (q >> 1)) : (q >> 1);
 }
 
+// Recursion step without the rescale test: valid once k == 0, after which the
+// normalised functions only grow to their O(1) oscillating regime.
+__device__ __forceinline__ void rec_fast(Rec& r, double ax, double g) {
+    const double fn = ax * r.f - g * r.fm1;
+    r.fm1 = r.f;
+    r.f = fn;
+}
+
+// l values per LDS-staged chunk of recursion coefficients and alm.  Even, so that the
+// chunk-local index li has the parity of l - l_start.
+constexpr int LCH = 32;
+
 // -------------------------------------------------- Legendre synthesis, spin 0
-// alm_t: [nlm][ld] complex, ld >= nchunk*BC (padding columns are read, never used)
+// alm_t: [nlm][ld] complex, ld >= nchunk*BC (padding columns are read, never used).
+// One block per (m, column chunk, 256 ring pairs); the block stages LCH rows of alm
+// and the (alpha, gamma) coefficients in LDS, every lane reads them by broadcast.
 template <typename T, int BC>
 __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int ld,
                                                       const cplx<T>* __restrict__ alm_t,
                                                       cplx<T>* __restrict__ phase) {
+    __shared__ cplx<T> sa[LCH * BC];
+    __shared__ double sco[LCH * 2];
     const int nrb = (d.npairs + 255) / 256;
     const int nchunk = (ncol + BC - 1) / BC;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int rb = lb % nrb;
     const int chunk = (lb / nrb) % nchunk;
     const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
-    const int jp = rb * 256 + threadIdx.x;
+    const int tid = threadIdx.x;
+    const int jp = rb * 256 + tid;
     const int jpc = jp < d.npairs ? jp : d.npairs - 1;
 
     const double x = d.cth[jpc];
@@ -371,9 +388,33 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const cplx<T>* a = alm_t + (size_t)chunk * BC;
     const double* cf4 = d.coef + 4 * base;
-    int l = m;
-    for (; l + 1 <= d.lmax; l += 2) {
-        if (__any(r.k >= scale_consts<T>::KMIN)) {
+    if constexpr (BC >= 8) {
+        // wide chunks: the alm row is read straight from memory by wave-uniform (scalar)
+        // loads; staging it in LDS would cost every lane a full row of LDS reads per l
+        int l = m;
+        for (; l + 1 <= d.lmax; l += 2) {
+            if (__any(r.k >= scale_consts<T>::KMIN)) {
+                const T v = (T)(r.f * r.cf);
+                const cplx<T>* al = a + (size_t)(base + l) * ld;
+#pragma unroll
+                for (int j = 0; j < BC; j++) {
+                    ev[j].re += v * al[j].re;
+                    ev[j].im += v * al[j].im;
+                }
+            }
+            rec_step(r, cf4[4 * l] * x, cf4[4 * l + 2]);
+            if (__any(r.k >= scale_consts<T>::KMIN)) {
+                const T v = (T)(r.f * r.cf);
+                const cplx<T>* al = a + (size_t)(base + l + 1) * ld;
+#pragma unroll
+                for (int j = 0; j < BC; j++) {
+                    od[j].re += v * al[j].re;
+                    od[j].im += v * al[j].im;
+                }
+            }
+            rec_step(r, cf4[4 * (l + 1)] * x, cf4[4 * (l + 1) + 2]);
+        }
+        if (l <= d.lmax) {  // trailing even-parity term
             const T v = (T)(r.f * r.cf);
             const cplx<T>* al = a + (size_t)(base + l) * ld;
 #pragma unroll
@@ -382,25 +423,70 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                 ev[j].im += v * al[j].im;
             }
         }
-        rec_step(r, cf4[4 * l] * x, cf4[4 * l + 2]);
-        if (__any(r.k >= scale_consts<T>::KMIN)) {
-            const T v = (T)(r.f * r.cf);
-            const cplx<T>* al = a + (size_t)(base + l + 1) * ld;
-#pragma unroll
-            for (int j = 0; j < BC; j++) {
-                od[j].re += v * al[j].re;
-                od[j].im += v * al[j].im;
+    } else {
+        bool fast = false;
+        for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
+            const int n = min(LCH, d.lmax + 1 - l0);
+            __syncthreads();
+            for (int i = tid; i < n * BC; i += 256) {
+                const int li = i / BC, j = i - li * BC;
+                sa[i] = a[(size_t)(base + l0 + li) * ld + j];
             }
-        }
-        rec_step(r, cf4[4 * (l + 1)] * x, cf4[4 * (l + 1) + 2]);
-    }
-    if (l <= d.lmax) {  // trailing even-parity term
-        const T v = (T)(r.f * r.cf);
-        const cplx<T>* al = a + (size_t)(base + l) * ld;
-#pragma unroll
-        for (int j = 0; j < BC; j++) {
-            ev[j].re += v * al[j].re;
-            ev[j].im += v * al[j].im;
+            for (int i = tid; i < 2 * n; i += 256) sco[i] = cf4[4 * (l0 + (i >> 1)) + 2 * (i & 1)];
+            __syncthreads();
+            if (!fast) fast = __all(r.k == 0);
+            if (fast) {
+                int li = 0;
+                for (; li + 1 < n; li += 2) {
+                    T v = (T)r.f;
+    #pragma unroll
+                    for (int j = 0; j < BC; j++) {
+                        const cplx<T> al = sa[li * BC + j];
+                        ev[j].re += v * al.re;
+                        ev[j].im += v * al.im;
+                    }
+                    rec_fast(r, sco[2 * li] * x, sco[2 * li + 1]);
+                    v = (T)r.f;
+    #pragma unroll
+                    for (int j = 0; j < BC; j++) {
+                        const cplx<T> al = sa[(li + 1) * BC + j];
+                        od[j].re += v * al.re;
+                        od[j].im += v * al.im;
+                    }
+                    rec_fast(r, sco[2 * li + 2] * x, sco[2 * li + 3]);
+                }
+                if (li < n) {  // odd count: the last l of the transform, even parity
+                    const T v = (T)r.f;
+    #pragma unroll
+                    for (int j = 0; j < BC; j++) {
+                        const cplx<T> al = sa[li * BC + j];
+                        ev[j].re += v * al.re;
+                        ev[j].im += v * al.im;
+                    }
+                }
+            } else {
+                for (int li = 0; li < n; li++) {
+                    if (__any(r.k >= scale_consts<T>::KMIN)) {
+                        const T v = (T)(r.f * r.cf);
+                        if (li & 1) {
+    #pragma unroll
+                            for (int j = 0; j < BC; j++) {
+                                const cplx<T> al = sa[li * BC + j];
+                                od[j].re += v * al.re;
+                                od[j].im += v * al.im;
+                            }
+                        } else {
+    #pragma unroll
+                            for (int j = 0; j < BC; j++) {
+                                const cplx<T> al = sa[li * BC + j];
+                                ev[j].re += v * al.re;
+                                ev[j].im += v * al.im;
+                            }
+                        }
+                    }
+                    rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
+                }
+            }
         }
     }
     if (jp >= d.npairs) return;
@@ -420,16 +506,31 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
 // -------------------------------------------------- Legendre synthesis, spin s
 // alm_t: [nlm][ld] complex with columns (map b, comp c) at 2 b + c (G, C).
 template <typename T, int BC>
+__device__ __forceinline__ void synth_sp_acc(cplx<T> (&acc)[BC][4], const cplx<T>* __restrict__ sa, T lp, T ln) {
+#pragma unroll
+    for (int j = 0; j < BC; j++) {
+        const cplx<T> g = sa[2 * j], c = sa[2 * j + 1];
+        acc[j][0].re += lp * g.re; acc[j][0].im += lp * g.im;
+        acc[j][1].re += lp * c.re; acc[j][1].im += lp * c.im;
+        acc[j][2].re += ln * g.re; acc[j][2].im += ln * g.im;
+        acc[j][3].re += ln * c.re; acc[j][3].im += ln * c.im;
+    }
+}
+
+template <typename T, int BC>
 __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int ld,
                                                       const cplx<T>* __restrict__ alm_t,
                                                       cplx<T>* __restrict__ phase) {
+    __shared__ cplx<T> sa[LCH * 2 * BC];
+    __shared__ double sco[LCH * 3];
     const int nrb = (d.npairs + 255) / 256;
     const int nchunk = (nmap + BC - 1) / BC;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int rb = lb % nrb;
     const int chunk = (lb / nrb) % nchunk;
     const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
-    const int jp = rb * 256 + threadIdx.x;
+    const int tid = threadIdx.x;
+    const int jp = rb * 256 + tid;
     const int jpc = jp < d.npairs ? jp : d.npairs - 1;
     const int M1 = d.mmax + 1;
 
@@ -439,52 +540,67 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
     rec_seed(rm, d.seedL[M1 + m] + d.seedA[M1 + m] * d.l2sh[jpc] + d.seedB[M1 + m] * d.l2ch[jpc],
              d.seedS[M1 + m]);
 
-    // acc[par][j][0..3] = sum G lam+, C lam+, G lam-, C lam-
-    cplx<T> acc[2][BC][4];
+    // acc[q][j][0..3] = sum G lam+, C lam+, G lam-, C lam- over l with li parity q
+    cplx<T> acc0[BC][4], acc1[BC][4];
 #pragma unroll
-    for (int p = 0; p < 2; p++)
+    for (int j = 0; j < BC; j++)
 #pragma unroll
-        for (int j = 0; j < BC; j++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[p][j][c] = cplx<T>{T(0), T(0)};
+        for (int c = 0; c < 4; c++) acc0[j][c] = acc1[j][c] = cplx<T>{T(0), T(0)};
 
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const cplx<T>* a = alm_t + (size_t)chunk * BC * 2;
     const double* cf4 = d.coef + 4 * base;
-    const int l0 = m > d.spin ? m : d.spin;
-    for (int l = l0; l <= d.lmax; l++) {
-        const int par = (l + m) & 1;
-        if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
-            const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
-            const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
-            const cplx<T>* al = a + (size_t)(base + l) * ld;
-            if (par == 0) {
-#pragma unroll
-                for (int j = 0; j < BC; j++) {
-                    cplx<T> g = al[2 * j], c = al[2 * j + 1];
-                    acc[0][j][0].re += lp * g.re; acc[0][j][0].im += lp * g.im;
-                    acc[0][j][1].re += lp * c.re; acc[0][j][1].im += lp * c.im;
-                    acc[0][j][2].re += ln * g.re; acc[0][j][2].im += ln * g.im;
-                    acc[0][j][3].re += ln * c.re; acc[0][j][3].im += ln * c.im;
+    const int lstart = m > d.spin ? m : d.spin;
+    bool fast = false;
+    for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
+        const int n = min(LCH, d.lmax + 1 - l0);
+        __syncthreads();
+        for (int i = tid; i < n * 2 * BC; i += 256) {
+            const int li = i / (2 * BC), j = i - li * 2 * BC;
+            sa[i] = a[(size_t)(base + l0 + li) * ld + j];
+        }
+        for (int i = tid; i < 3 * n; i += 256) {
+            const int li = i / 3, c = i - 3 * li;
+            sco[i] = cf4[4 * (l0 + li) + c];
+        }
+        __syncthreads();
+        if (!fast) fast = __all(rp.k == 0 && rm.k == 0);
+        if (fast) {
+            int li = 0;
+            for (; li + 1 < n; li += 2) {
+                synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
+                double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                rec_fast(rp, al * (x + be), ga);
+                rec_fast(rm, al * (x - be), ga);
+                synth_sp_acc<T, BC>(acc1, sa + (li + 1) * 2 * BC, (T)(0.5 * (rp.f + rm.f)),
+                                    (T)(0.5 * (rp.f - rm.f)));
+                al = sco[3 * li + 3], be = sco[3 * li + 4], ga = sco[3 * li + 5];
+                rec_fast(rp, al * (x + be), ga);
+                rec_fast(rm, al * (x - be), ga);
+            }
+            if (li < n)
+                synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
+        } else {
+            for (int li = 0; li < n; li++) {
+                if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
+                    const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                    const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+                    if (li & 1)
+                        synth_sp_acc<T, BC>(acc1, sa + li * 2 * BC, lp, ln);
+                    else
+                        synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, lp, ln);
                 }
-            } else {
-#pragma unroll
-                for (int j = 0; j < BC; j++) {
-                    cplx<T> g = al[2 * j], c = al[2 * j + 1];
-                    acc[1][j][0].re += lp * g.re; acc[1][j][0].im += lp * g.im;
-                    acc[1][j][1].re += lp * c.re; acc[1][j][1].im += lp * c.im;
-                    acc[1][j][2].re += ln * g.re; acc[1][j][2].im += ln * g.im;
-                    acc[1][j][3].re += ln * c.re; acc[1][j][3].im += ln * c.im;
-                }
+                const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                rec_step(rp, al * (x + be), ga);
+                rec_step(rm, al * (x - be), ga);
             }
         }
-        const double al_ = cf4[4 * l], be = cf4[4 * l + 1], ga = cf4[4 * l + 2];
-        rec_step(rp, al_ * (x + be), ga);
-        rec_step(rm, al_ * (x - be), ga);
     }
     if (jp >= d.npairs) return;
     const int rn = jp, rs = d.nrings - 1 - jp;
     const size_t pstride = (size_t)M1 * d.nrings;
+    // li parity 0 holds l + m even unless l_start + m is odd (m < spin)
+    const bool swp = ((lstart + m) & 1) != 0;
 #pragma unroll
     for (int j = 0; j < BC; j++) {
         const int b = chunk * BC + j;
@@ -493,10 +609,11 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
         cplx<T>* phI = phase + (size_t)(2 * b + 1) * pstride + (size_t)m * d.nrings;
         // R = -(G lam+ + i C lam-), I = -(C lam+ - i G lam-); south: lam+ -> (-1)^(l+m),
         // lam- -> -(-1)^(l+m)
-        const cplx<T> GPe = acc[0][j][0], GPo = acc[1][j][0], CPe = acc[0][j][1], CPo = acc[1][j][1];
-        const cplx<T> GMe = acc[0][j][2], GMo = acc[1][j][2], CMe = acc[0][j][3], CMo = acc[1][j][3];
+        const cplx<T> GPe = swp ? acc1[j][0] : acc0[j][0], GPo = swp ? acc0[j][0] : acc1[j][0];
+        const cplx<T> CPe = swp ? acc1[j][1] : acc0[j][1], CPo = swp ? acc0[j][1] : acc1[j][1];
+        const cplx<T> GMe = swp ? acc1[j][2] : acc0[j][2], GMo = swp ? acc0[j][2] : acc1[j][2];
+        const cplx<T> CMe = swp ? acc1[j][3] : acc0[j][3], CMo = swp ? acc0[j][3] : acc1[j][3];
         cplx<T> t;
-        // north
         t = cadd(CMe, CMo);
         phR[rn] = cplx<T>{-(GPe.re + GPo.re) + t.im, -(GPe.im + GPo.im) - t.re};
         t = cadd(GMe, GMo);
@@ -661,31 +778,91 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
     }
 }
 
+// ------------------------------------------------ Legendre adjoint (reduction)
+// Block-wide sum of V partial values per thread: red[v][t] holds thread t's value v;
+// TPV = NT / V consecutive threads sum one row and combine with shuffles.  The row
+// pitch NT + TPV keeps the strided reads free of bank conflicts.  The caller writes red
+// and calls this between two barriers; the sum of row v is returned to thread
+// t = v * TPV.
+template <typename T, int NT, int V>
+__device__ __forceinline__ T red_rows(T (*red)[NT + NT / V], int tid) {
+    constexpr int TPV = NT / V;
+    static_assert(TPV >= 1 && TPV <= 64 && (TPV & (TPV - 1)) == 0, "bad reduction shape");
+    const int row = tid / TPV, part = tid % TPV;
+    T s = T(0);
+#pragma unroll 4
+    for (int i = part; i < NT; i += TPV) s += red[row][i];
+#pragma unroll
+    for (int o = TPV / 2; o >= 1; o >>= 1) s += __shfl_down(s, o, TPV);
+    return s;
+}
+
+// One chunk of LC values of l for NRT ring pairs (spin 0): acc[c] += sum_q lam_l(q) E_q,
+// E = even / odd phase combination by the parity of c.  FAST: every recursion has k == 0
+// (no rescale test, no scale factor); FULL: all LC values of l exist.
+template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
+__device__ __forceinline__ void adj_s0_chunk(cplx<T> (&acc)[LC][BC], Rec (&r)[NRT], const double (&x)[NRT],
+                                             const cplx<T> (&ev)[NRT][BC], const cplx<T> (&od)[NRT][BC],
+                                             const double* __restrict__ sco, int nvalid) {
+#pragma unroll
+    for (int c = 0; c < LC; c++) {
+        if (FULL || c < nvalid) {
+            const double al = sco[2 * c], ga = sco[2 * c + 1];
+#pragma unroll
+            for (int q = 0; q < NRT; q++) {
+                const T v = FAST ? (T)r[q].f : (T)(r[q].f * r[q].cf);
+#pragma unroll
+                for (int j = 0; j < BC; j++) {
+                    const cplx<T> e = (c & 1) ? od[q][j] : ev[q][j];
+                    acc[c][j].re += v * e.re;
+                    acc[c][j].im += v * e.im;
+                }
+                if (FAST)
+                    rec_fast(r[q], al * x[q], ga);
+                else
+                    rec_step(r[q], al * x[q], ga);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------ Legendre adjoint, spin 0
-// phases [col][mmax+1][nrings] -> alm [col][nlm] (scaled). One block per (m, chunk);
-// thread t owns ring pairs t + 256 r, r < NRT.
-template <typename T, int NRT, int BC, int LC>
-__global__ __launch_bounds__(256) void k_leg_adj_s0(ShtDev<T> d, int ncol, const cplx<T>* __restrict__ phase,
-                                                    cplx<T>* __restrict__ alm, long alm_stride, T scale) {
-    __shared__ T red[LC * BC * 2][257];
+// phases [col][mmax+1][nrings] -> alm [col][nlm] (scaled), or, with nsplit > 1, the
+// unscaled partial sums over ring-pair slice s into part[s][col][nlm].  One block of
+// NW waves per (m, column chunk, ring slice); thread t owns ring pairs
+// s * pps + t + 64 NW q (q < NRT) whose recursions run as independent chains; the
+// partial sums over a thread's rings are reduced across the block every LC values of l.
+template <typename T, int NRT, int NW, int BC, int LC>
+__global__ __launch_bounds__(64 * NW) void k_leg_adj_s0(ShtDev<T> d, int ncol, const cplx<T>* __restrict__ phase,
+                                                        cplx<T>* __restrict__ alm, long alm_stride, T scale,
+                                                        int nsplit, int pps) {
+    constexpr int NT = 64 * NW;
+    constexpr int V = LC * BC * 2;
+    constexpr int TPV = NT / V;
+    static_assert(LCH % LC == 0 && (LC & 1) == 0, "LC must be even and divide LCH");
+    __shared__ T red[V][NT + TPV];
+    __shared__ double sco[LCH * 2];
     const int nchunk = (ncol + BC - 1) / BC;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = lb % nchunk;
-    const int m = m_of_q(lb / nchunk, d.mmax);
+    const int sl = (lb / nchunk) % nsplit;
+    const int m = m_of_q(lb / (nchunk * nsplit), d.mmax);
     const int tid = threadIdx.x;
     const size_t pstride = (size_t)(d.mmax + 1) * d.nrings;
+    const int jp0 = sl * pps, jp1 = min(d.npairs, jp0 + pps);
+    cplx<T>* out = nsplit > 1 ? alm + (size_t)sl * ncol * alm_stride : alm;
+    const T osc = nsplit > 1 ? T(1) : scale;
 
     double x[NRT];
     Rec r[NRT];
     cplx<T> ev[NRT][BC], od[NRT][BC];
 #pragma unroll
     for (int q = 0; q < NRT; q++) {
-        const int jp = tid + 256 * q;
-        const bool ok = jp < d.npairs;
-        const int jpc = ok ? jp : 0;
+        const int jp = jp0 + tid + NT * q;
+        const bool ok = jp < jp1;
+        const int jpc = ok ? jp : jp0;
         x[q] = d.cth[jpc];
         rec_seed(r[q], d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
-        if (!ok) r[q].k = -1000000, r[q].cf = 0.0, r[q].f = 0.0;
         const int rn = jpc, rs = d.nrings - 1 - jpc;
 #pragma unroll
         for (int j = 0; j < BC; j++) {
@@ -696,82 +873,120 @@ __global__ __launch_bounds__(256) void k_leg_adj_s0(ShtDev<T> d, int ncol, const
                 pn = P[rn];
                 if (rs != rn) ps = P[rs];
             }
+            // idle slots carry zero phases: their (finite) recursion values never matter
             ev[q][j] = cadd(pn, ps);
             od[q][j] = csub(pn, ps);
         }
     }
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const double* cf4 = d.coef + 4 * base;
-    for (int l0 = m; l0 <= d.lmax; l0 += LC) {
-        cplx<T> acc[LC][BC];
+    for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
+        const int n = min(LCH, d.lmax + 1 - l0);
+        __syncthreads();
+        for (int i = tid; i < 2 * n; i += NT) sco[i] = cf4[4 * (l0 + (i >> 1)) + 2 * (i & 1)];
+        __syncthreads();
+        bool allk0 = true;
 #pragma unroll
-        for (int c = 0; c < LC; c++)
+        for (int q = 0; q < NRT; q++) allk0 = allk0 && (r[q].k == 0);
+        const bool fast = __all(allk0);
+        for (int lc = 0; lc < n; lc += LC) {
+            cplx<T> acc[LC][BC];
 #pragma unroll
-            for (int j = 0; j < BC; j++) acc[c][j] = cplx<T>{T(0), T(0)};
+            for (int c = 0; c < LC; c++)
 #pragma unroll
-        for (int c = 0; c < LC; c++) {
-            const int l = l0 + c;
-            if (l <= d.lmax) {
-                const double al = cf4[4 * l], ga = cf4[4 * l + 2];
+                for (int j = 0; j < BC; j++) acc[c][j] = cplx<T>{T(0), T(0)};
+            const int nv = n - lc;
+            if (nv >= LC) {
+                if (fast) adj_s0_chunk<T, NRT, BC, LC, true, true>(acc, r, x, ev, od, sco + 2 * lc, nv);
+                else adj_s0_chunk<T, NRT, BC, LC, false, true>(acc, r, x, ev, od, sco + 2 * lc, nv);
+            } else {
+                if (fast) adj_s0_chunk<T, NRT, BC, LC, true, false>(acc, r, x, ev, od, sco + 2 * lc, nv);
+                else adj_s0_chunk<T, NRT, BC, LC, false, false>(acc, r, x, ev, od, sco + 2 * lc, nv);
+            }
 #pragma unroll
-                for (int q = 0; q < NRT; q++) {
-                    const T v = (T)(r[q].f * r[q].cf);
+            for (int c = 0; c < LC; c++)
 #pragma unroll
-                    for (int j = 0; j < BC; j++) {
-                        const cplx<T> e = (c & 1) ? od[q][j] : ev[q][j];
-                        acc[c][j].re += v * e.re;
-                        acc[c][j].im += v * e.im;
-                    }
-                    rec_step(r[q], al * x[q], ga);
+                for (int j = 0; j < BC; j++) {
+                    red[(c * BC + j) * 2][tid] = acc[c][j].re;
+                    red[(c * BC + j) * 2 + 1][tid] = acc[c][j].im;
+                }
+            __syncthreads();
+            const T sum = red_rows<T, NT, V>(red, tid);
+            if (tid % TPV == 0) {
+                const int row = tid / TPV;
+                const int c = row / (2 * BC), j = (row >> 1) % BC, part = row & 1;
+                const int l = l0 + lc + c, col = chunk * BC + j;
+                if (c < nv && col < ncol)
+                    reinterpret_cast<T*>(out + (size_t)col * alm_stride + base + l)[part] = sum * osc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m.
+template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
+__device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp)[NRT], Rec (&rm)[NRT],
+                                             const double (&x)[NRT], const cplx<T> (&in4)[NRT][BC][4],
+                                             const double* __restrict__ sco, int nvalid, int p0) {
+#pragma unroll
+    for (int c = 0; c < LC; c++) {
+        if (FULL || c < nvalid) {
+            const bool even = ((c + p0) & 1) == 0;  // l + m even
+            const double al = sco[3 * c], be = sco[3 * c + 1], ga = sco[3 * c + 2];
+#pragma unroll
+            for (int q = 0; q < NRT; q++) {
+                const double fp = FAST ? rp[q].f : rp[q].f * rp[q].cf;
+                const double fm = FAST ? rm[q].f : rm[q].f * rm[q].cf;
+                const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+#pragma unroll
+                for (int j = 0; j < BC; j++) {
+                    const cplx<T> Rp = even ? in4[q][j][0] : in4[q][j][1];
+                    const cplx<T> Ip = even ? in4[q][j][2] : in4[q][j][3];
+                    const cplx<T> Rm = even ? in4[q][j][1] : in4[q][j][0];
+                    const cplx<T> Im = even ? in4[q][j][3] : in4[q][j][2];
+                    // G += -(lp Rp + i ln Im), C += -(lp Ip - i ln Rm)
+                    acc[c][j][0].re += -lp * Rp.re + ln * Im.im;
+                    acc[c][j][0].im += -lp * Rp.im - ln * Im.re;
+                    acc[c][j][1].re += -lp * Ip.re - ln * Rm.im;
+                    acc[c][j][1].im += -lp * Ip.im + ln * Rm.re;
+                }
+                if (FAST) {
+                    rec_fast(rp[q], al * (x[q] + be), ga);
+                    rec_fast(rm[q], al * (x[q] - be), ga);
+                } else {
+                    rec_step(rp[q], al * (x[q] + be), ga);
+                    rec_step(rm[q], al * (x[q] - be), ga);
                 }
             }
         }
-#pragma unroll
-        for (int c = 0; c < LC; c++)
-#pragma unroll
-            for (int j = 0; j < BC; j++) {
-                red[(c * BC + j) * 2][tid] = acc[c][j].re;
-                red[(c * BC + j) * 2 + 1][tid] = acc[c][j].im;
-            }
-        __syncthreads();
-        // 256 threads reduce LC*BC*2 rows of 256 partial sums
-        constexpr int NROW = LC * BC * 2;
-        constexpr int TPR = 256 / NROW;  // threads per row
-        {
-            const int row = tid / TPR, part = tid % TPR;
-            T s = T(0);
-            for (int i = part; i < 256; i += TPR) s += red[row][i];
-            // combine the TPR partials of a row (consecutive lanes) with shuffles
-            for (int o = TPR / 2; o >= 1; o >>= 1) s += __shfl_down(s, o, TPR);
-            __syncthreads();
-            if (part == 0) red[row][256] = s;
-        }
-        __syncthreads();
-        if (tid < LC * BC) {
-            const int c = tid / BC, j = tid % BC;
-            const int l = l0 + c, col = chunk * BC + j;
-            if (l <= d.lmax && col < ncol)
-                alm[(size_t)col * alm_stride + base + l] =
-                    cplx<T>{red[(c * BC + j) * 2][256] * scale, red[(c * BC + j) * 2 + 1][256] * scale};
-        }
-        __syncthreads();
     }
 }
 
 // ------------------------------------------------ Legendre adjoint, spin s
-// phases [(b, R|I)][mmax+1][nrings] -> alm [(b, G|C)][nlm] (scaled)
-template <typename T, int NRT, int BC, int LC>
-__global__ __launch_bounds__(256) void k_leg_adj_sp(ShtDev<T> d, int nmap, const cplx<T>* __restrict__ phase,
-                                                    cplx<T>* __restrict__ alm, long alm_stride, T scale) {
-    constexpr int NROW = LC * BC * 4;
-    __shared__ T red[NROW][257];
+// phases [(b, R|I)][mmax+1][nrings] -> alm [(b, G|C)][nlm] (scaled); ring slices and
+// partial sums as for spin 0.
+template <typename T, int NRT, int NW, int BC, int LC>
+__global__ __launch_bounds__(64 * NW) void k_leg_adj_sp(ShtDev<T> d, int nmap, const cplx<T>* __restrict__ phase,
+                                                        cplx<T>* __restrict__ alm, long alm_stride, T scale,
+                                                        int nsplit, int pps) {
+    constexpr int NT = 64 * NW;
+    constexpr int V = LC * BC * 4;
+    constexpr int TPV = NT / V;
+    static_assert(LCH % LC == 0 && (LC & 1) == 0, "LC must be even and divide LCH");
+    __shared__ T red[V][NT + TPV];
+    __shared__ double sco[LCH * 3];
     const int nchunk = (nmap + BC - 1) / BC;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = lb % nchunk;
-    const int m = m_of_q(lb / nchunk, d.mmax);
+    const int sl = (lb / nchunk) % nsplit;
+    const int m = m_of_q(lb / (nchunk * nsplit), d.mmax);
     const int tid = threadIdx.x;
     const int M1 = d.mmax + 1;
     const size_t pstride = (size_t)M1 * d.nrings;
+    const int jp0 = sl * pps, jp1 = min(d.npairs, jp0 + pps);
+    cplx<T>* out = nsplit > 1 ? alm + (size_t)sl * 2 * nmap * alm_stride : alm;
+    const T osc = nsplit > 1 ? T(1) : scale;
 
     double x[NRT];
     Rec rp[NRT], rm[NRT];
@@ -779,18 +994,13 @@ __global__ __launch_bounds__(256) void k_leg_adj_sp(ShtDev<T> d, int nmap, const
     cplx<T> in4[NRT][BC][4];
 #pragma unroll
     for (int q = 0; q < NRT; q++) {
-        const int jp = tid + 256 * q;
-        const bool ok = jp < d.npairs;
-        const int jpc = ok ? jp : 0;
+        const int jp = jp0 + tid + NT * q;
+        const bool ok = jp < jp1;
+        const int jpc = ok ? jp : jp0;
         x[q] = d.cth[jpc];
         rec_seed(rp[q], d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
         rec_seed(rm[q], d.seedL[M1 + m] + d.seedA[M1 + m] * d.l2sh[jpc] + d.seedB[M1 + m] * d.l2ch[jpc],
                  d.seedS[M1 + m]);
-        if (!ok) {
-            rp[q].k = rm[q].k = -1000000;
-            rp[q].cf = rm[q].cf = 0.0;
-            rp[q].f = rm[q].f = 0.0;
-        }
         const int rn = jpc, rs = d.nrings - 1 - jpc;
 #pragma unroll
         for (int j = 0; j < BC; j++) {
@@ -816,73 +1026,71 @@ __global__ __launch_bounds__(256) void k_leg_adj_sp(ShtDev<T> d, int nmap, const
     const double* cf4 = d.coef + 4 * base;
     const int lstart = m > d.spin ? m : d.spin;
     // coefficients with m <= l < spin vanish identically
-    for (int i = tid; i < (lstart - m) * BC * 2; i += blockDim.x) {
+    for (int i = tid; i < (lstart - m) * BC * 2; i += NT) {
         const int l = m + i / (BC * 2), b = chunk * BC + (i / 2) % BC, g = i % 2;
-        if (l <= d.lmax && b < nmap) alm[(size_t)(2 * b + g) * alm_stride + base + l] = cplx<T>{T(0), T(0)};
+        if (l <= d.lmax && b < nmap) out[(size_t)(2 * b + g) * alm_stride + base + l] = cplx<T>{T(0), T(0)};
     }
-    for (int l0 = lstart; l0 <= d.lmax; l0 += LC) {
-        cplx<T> acc[LC][BC][2];  // G, C
+    const int p0 = (lstart + m) & 1;
+    for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
+        const int n = min(LCH, d.lmax + 1 - l0);
+        __syncthreads();
+        for (int i = tid; i < 3 * n; i += NT) {
+            const int li = i / 3, c = i - 3 * li;
+            sco[i] = cf4[4 * (l0 + li) + c];
+        }
+        __syncthreads();
+        bool allk0 = true;
 #pragma unroll
-        for (int c = 0; c < LC; c++)
+        for (int q = 0; q < NRT; q++) allk0 = allk0 && (rp[q].k == 0) && (rm[q].k == 0);
+        const bool fast = __all(allk0);
+        for (int lc = 0; lc < n; lc += LC) {
+            cplx<T> acc[LC][BC][2];  // G, C
 #pragma unroll
-            for (int j = 0; j < BC; j++) acc[c][j][0] = acc[c][j][1] = cplx<T>{T(0), T(0)};
+            for (int c = 0; c < LC; c++)
 #pragma unroll
-        for (int c = 0; c < LC; c++) {
-            const int l = l0 + c;
-            if (l <= d.lmax) {
-                const bool even = ((l + m) & 1) == 0;
-                const double al = cf4[4 * l], be = cf4[4 * l + 1], ga = cf4[4 * l + 2];
+                for (int j = 0; j < BC; j++) acc[c][j][0] = acc[c][j][1] = cplx<T>{T(0), T(0)};
+            const int nv = n - lc;
+            if (nv >= LC) {
+                if (fast) adj_sp_chunk<T, NRT, BC, LC, true, true>(acc, rp, rm, x, in4, sco + 3 * lc, nv, p0);
+                else adj_sp_chunk<T, NRT, BC, LC, false, true>(acc, rp, rm, x, in4, sco + 3 * lc, nv, p0);
+            } else {
+                if (fast) adj_sp_chunk<T, NRT, BC, LC, true, false>(acc, rp, rm, x, in4, sco + 3 * lc, nv, p0);
+                else adj_sp_chunk<T, NRT, BC, LC, false, false>(acc, rp, rm, x, in4, sco + 3 * lc, nv, p0);
+            }
 #pragma unroll
-                for (int q = 0; q < NRT; q++) {
-                    const double fp = rp[q].f * rp[q].cf, fm = rm[q].f * rm[q].cf;
-                    const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+            for (int c = 0; c < LC; c++)
 #pragma unroll
-                    for (int j = 0; j < BC; j++) {
-                        const cplx<T> Rp = even ? in4[q][j][0] : in4[q][j][1];
-                        const cplx<T> Ip = even ? in4[q][j][2] : in4[q][j][3];
-                        const cplx<T> Rm = even ? in4[q][j][1] : in4[q][j][0];
-                        const cplx<T> Im = even ? in4[q][j][3] : in4[q][j][2];
-                        // G += -(lp Rp + i ln Im), C += -(lp Ip - i ln Rm)
-                        acc[c][j][0].re += -lp * Rp.re + ln * Im.im;
-                        acc[c][j][0].im += -lp * Rp.im - ln * Im.re;
-                        acc[c][j][1].re += -lp * Ip.re - ln * Rm.im;
-                        acc[c][j][1].im += -lp * Ip.im + ln * Rm.re;
+                for (int j = 0; j < BC; j++)
+#pragma unroll
+                    for (int g = 0; g < 2; g++) {
+                        red[((c * BC + j) * 2 + g) * 2][tid] = acc[c][j][g].re;
+                        red[((c * BC + j) * 2 + g) * 2 + 1][tid] = acc[c][j][g].im;
                     }
-                    rec_step(rp[q], al * (x[q] + be), ga);
-                    rec_step(rm[q], al * (x[q] - be), ga);
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < LC; c++)
-#pragma unroll
-            for (int j = 0; j < BC; j++)
-#pragma unroll
-                for (int g = 0; g < 2; g++) {
-                    red[((c * BC + j) * 2 + g) * 2][tid] = acc[c][j][g].re;
-                    red[((c * BC + j) * 2 + g) * 2 + 1][tid] = acc[c][j][g].im;
-                }
-        __syncthreads();
-        constexpr int TPR = 256 / NROW;
-        {
-            const int row = tid / TPR, part = tid % TPR;
-            T s = T(0);
-            for (int i = part; i < 256; i += TPR) s += red[row][i];
-            for (int o = TPR / 2; o >= 1; o >>= 1) s += __shfl_down(s, o, TPR);
             __syncthreads();
-            if (part == 0) red[row][256] = s;
-        }
-        __syncthreads();
-        if (tid < LC * BC * 2) {
-            const int c = tid / (BC * 2), j = (tid / 2) % BC, g = tid % 2;
-            const int l = l0 + c, b = chunk * BC + j;
-            if (l <= d.lmax && b < nmap) {
-                const int row = ((c * BC + j) * 2 + g) * 2;
-                alm[(size_t)(2 * b + g) * alm_stride + base + l] =
-                    cplx<T>{red[row][256] * scale, red[row + 1][256] * scale};
+            const T sum = red_rows<T, NT, V>(red, tid);
+            if (tid % TPV == 0) {
+                const int row = tid / TPV;
+                const int c = row / (4 * BC), j = (row >> 2) % BC, g = (row >> 1) & 1, part = row & 1;
+                const int l = l0 + lc + c, b = chunk * BC + j;
+                if (c < nv && b < nmap)
+                    reinterpret_cast<T*>(out + (size_t)(2 * b + g) * alm_stride + base + l)[part] = sum * osc;
             }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+}
+
+// alm[col][lm] = scale * sum_s part[s][col][lm] (fixed order)
+template <typename T>
+__global__ void k_adj_sum_splits(const cplx<T>* __restrict__ part, int nsplit, int ncol, long nlm,
+                                 cplx<T>* __restrict__ alm, long alm_stride, T scale) {
+    const long n = (long)ncol * nlm;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        cplx<T> s{T(0), T(0)};
+        for (int k = 0; k < nsplit; k++) s = cadd(s, part[(size_t)k * n + i]);
+        const long col = i / nlm, lm = i - col * nlm;
+        alm[(size_t)col * alm_stride + lm] = cplx<T>{s.re * scale, s.im * scale};
     }
 }
 
@@ -955,39 +1163,101 @@ static int launch_ring_adjoint(const ShtPlan& p, int ncol, const void* map, long
     return 0;
 }
 
-template <typename T, int NRT>
-static int launch_adj_nrt(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
-                          double scale, hipStream_t st) {
+// Ring slices of the adjoint: with few batch columns the blocks of small m (the longest
+// recursions) set the kernel time, so their rings are split over nsplit blocks whose
+// partial sums a second pass adds in fixed order.
+int sht_adjoint_nsplit(const ShtPlan& p, int nmap) {
+    const bool many = p.spin == 0 ? nmap * p.ncomp > 2 : nmap > 1;
+    if (many) return 1;
+    int ns = 1;
+    while (ns < 8 && (p.npairs + 2 * ns - 1) / (2 * ns) >= 64) ns *= 2;
+    return ns;
+}
+
+size_t sht_adjoint_work_bytes(const ShtPlan& p, int nmap) {
+    const size_t ph = (sht_phase_bytes(p, nmap) + 255) & ~(size_t)255;
+    const int ns = sht_adjoint_nsplit(p, nmap);
+    return ns > 1 ? ph + (size_t)ns * nmap * p.ncomp * p.nlm * 2 * p.prec : ph;
+}
+
+template <typename T, int NRT, int NW, int BC>
+static int launch_adj_cfg(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
+                          double scale, int ns, int pps, hipStream_t st) {
     ShtDev<T> d = dev_view<T>(p);
-    // LDS reduction buffer: LC * BC * (2 | 4) rows of 257 reals, kept under 33 KB
     if (p.spin == 0) {
-        constexpr int BC = 2;
-        constexpr int LC = sizeof(T) == 8 ? 4 : 8;
+        constexpr int LC = 8;
         const int nchunk = (ncol + BC - 1) / BC;
-        hipLaunchKernelGGL((k_leg_adj_s0<T, NRT, BC, LC>), dim3((unsigned)(nchunk * (p.mmax + 1))), dim3(256), 0,
-                           st, d, ncol, (const cplx<T>*)phase, (cplx<T>*)alm, alm_stride, (T)scale);
+        hipLaunchKernelGGL((k_leg_adj_s0<T, NRT, NW, BC, LC>), dim3((unsigned)(nchunk * ns * (p.mmax + 1))),
+                           dim3(64 * NW), 0, st, d, ncol, (const cplx<T>*)phase, (cplx<T>*)alm, alm_stride,
+                           (T)scale, ns, pps);
     } else {
-        constexpr int BC = (NRT >= 8 || sizeof(T) == 8) ? 1 : 2;
         constexpr int LC = 4;
         const int nmap = ncol / 2;
         const int nchunk = (nmap + BC - 1) / BC;
-        hipLaunchKernelGGL((k_leg_adj_sp<T, NRT, BC, LC>), dim3((unsigned)(nchunk * (p.mmax + 1))), dim3(256), 0,
-                           st, d, nmap, (const cplx<T>*)phase, (cplx<T>*)alm, alm_stride, (T)scale);
+        hipLaunchKernelGGL((k_leg_adj_sp<T, NRT, NW, BC, LC>), dim3((unsigned)(nchunk * ns * (p.mmax + 1))),
+                           dim3(64 * NW), 0, st, d, nmap, (const cplx<T>*)phase, (cplx<T>*)alm, alm_stride,
+                           (T)scale, ns, pps);
     }
     PS_LAUNCH_CHECK();
     return 0;
 }
 
+// Latency shape (few columns, ring slices): up to 8 ring pairs per thread, one column
+// chunk of 2 (spin 0) or 1 map (spin s) per block.
+template <typename T>
+static int launch_adj_latency(const ShtPlan& p, int ncol, const void* phase, void* dst, long dstride,
+                              double scale, int ns, int pps, hipStream_t st) {
+    if (p.spin == 0) {
+        constexpr int BC = 2;
+        if (pps <= 64) return launch_adj_cfg<T, 1, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+        if (pps <= 128) return launch_adj_cfg<T, 2, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+        if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+        if (pps <= 512) return launch_adj_cfg<T, 8, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+        if (pps <= 1024) return launch_adj_cfg<T, 8, 2, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+        return launch_adj_cfg<T, 8, 4, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    }
+    constexpr int BC = 1;
+    if (pps <= 64) return launch_adj_cfg<T, 1, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (pps <= 128) return launch_adj_cfg<T, 2, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (pps <= 512) return launch_adj_cfg<T, 8, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (pps <= 1024) return launch_adj_cfg<T, 8, 2, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    return launch_adj_cfg<T, 8, 4, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+}
+
+// Throughput shape (many columns): 4 ring pairs per thread and more waves per block,
+// wider column chunks.
+template <typename T>
+static int launch_adj_throughput(const ShtPlan& p, int ncol, const void* phase, void* dst, long dstride,
+                                 double scale, int pps, hipStream_t st) {
+    constexpr int BC = 2;
+    if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
+    if (pps <= 512) return launch_adj_cfg<T, 4, 2, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
+    if (pps <= 1024) return launch_adj_cfg<T, 4, 4, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
+    return launch_adj_cfg<T, 4, 8, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
+}
+
 template <typename T>
 static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride, double scale,
-                      hipStream_t st) {
-    const int nrt = (p.npairs + 255) / 256;
-    if (nrt <= 1) return launch_adj_nrt<T, 1>(p, ncol, phase, alm, alm_stride, scale, st);
-    if (nrt <= 2) return launch_adj_nrt<T, 2>(p, ncol, phase, alm, alm_stride, scale, st);
-    if (nrt <= 4) return launch_adj_nrt<T, 4>(p, ncol, phase, alm, alm_stride, scale, st);
-    if (nrt <= 8) return launch_adj_nrt<T, 8>(p, ncol, phase, alm, alm_stride, scale, st);
-    set_error("adjoint: nside above 1024 is not supported");
-    return -1;
+                      void* part, hipStream_t st) {
+    if (p.npairs > 2048) {
+        set_error("adjoint: nside above 1024 is not supported");
+        return -1;
+    }
+    const int nmap = ncol / p.ncomp;
+    const int ns = sht_adjoint_nsplit(p, nmap);
+    const int pps = (p.npairs + ns - 1) / ns;
+    void* dst = ns > 1 ? part : alm;
+    const long dstride = ns > 1 ? p.nlm : alm_stride;
+    const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
+    int rc = many ? launch_adj_throughput<T>(p, ncol, phase, dst, dstride, scale, pps, st)
+                  : launch_adj_latency<T>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (rc || ns == 1) return rc;
+    const long n = (long)ncol * p.nlm;
+    hipLaunchKernelGGL(k_adj_sum_splits<T>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, st,
+                       (const cplx<T>*)part, ns, ncol, p.nlm, (cplx<T>*)alm, alm_stride, (T)scale);
+    PS_LAUNCH_CHECK();
+    return 0;
 }
 
 // synthesis BC (batch columns per block)
@@ -1060,17 +1330,19 @@ int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stri
 }
 
 int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, void* alm, long alm_stride,
-                double scale, void* phase, hipStream_t st) {
+                double scale, void* work, hipStream_t st) {
     const int ncol = nmap * p.ncomp;
+    void* phase = work;
+    void* part = (char*)work + ((sht_phase_bytes(p, nmap) + 255) & ~(size_t)255);
     int rc;
     if (p.prec == 4) {
         rc = launch_ring_adjoint<float>(p, ncol, map, map_stride, phase, st);
         if (rc) return rc;
-        return launch_adj<float>(p, ncol, phase, alm, alm_stride, scale, st);
+        return launch_adj<float>(p, ncol, phase, alm, alm_stride, scale, part, st);
     }
     rc = launch_ring_adjoint<double>(p, ncol, map, map_stride, phase, st);
     if (rc) return rc;
-    return launch_adj<double>(p, ncol, phase, alm, alm_stride, scale, st);
+    return launch_adj<double>(p, ncol, phase, alm, alm_stride, scale, part, st);
 }
 
 }  // namespace ps
