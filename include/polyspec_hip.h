@@ -68,9 +68,9 @@ int ps_sht_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_
                             void *work, long long work_bytes, void *stream);
 
 /* The two stages of ps_sht_synthesis_lmajor, for callers that schedule or time them
- * separately: the Legendre stage writes ring phases F_m(ring) laid out
- * [nmaps*ncomp][mmax+1][4 nside - 1] complex (ps_sht_phase_bytes bytes), the ring
- * stage turns them into maps by per-ring FFTs. */
+ * separately: the Legendre stage writes ring phases F_m(ring) exp(i m phi0(ring)) in
+ * tiles of 16 rings, [ring / 16][mmax+1][ring % 16][nmaps*ncomp] complex
+ * (ps_sht_phase_bytes bytes); the ring stage turns them into maps by per-ring FFTs. */
 int ps_sht_phase_bytes(const ps_sht_plan *plan, int nmaps, long long *bytes);
 int ps_sht_legendre_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_t, int ld, void *phase,
                                      void *stream);

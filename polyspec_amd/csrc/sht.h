@@ -19,6 +19,9 @@ struct ShtPlan {
     int nside = 0, lmax = 0, mmax = 0, spin = 0, prec = 4;  // prec: bytes per real (4 or 8)
     int nrings = 0, npairs = 0, ncomp = 1;
     long npix = 0, nlm = 0, maxM = 1;
+    long maxM_pow2 = 1, maxM_blue = 1;  // largest FFT length of the power-of-two / Bluestein rings
+    int n_pow2 = 0;                     // power-of-two rings, listed first in d_rlist
+    int* d_rlist = nullptr;             // ring indices: power-of-two rings, then Bluestein rings
     int device = 0;
     RingInfo* d_rings = nullptr;
     double *d_l2s = nullptr, *d_l2sh = nullptr, *d_l2ch = nullptr, *d_seedL = nullptr, *d_seedS = nullptr;

@@ -10,13 +10,16 @@
 //     alm of the chunk are wave-uniform (scalar loads) and the batch is
 //     accumulated in registers.  Seeds are built in FP64 from log2 tables and
 //     carried as scaled value * 2^(64 k) so that sin^m(theta) never underflows.
-//   Ring stage      ring phases <-> pixels: one block per (ring, map); m is
-//     folded modulo nphi in LDS, then an in-LDS radix-2 FFT (nphi a power of
-//     two) or a Bluestein chirp-z transform over a power-of-two length.
+//   Ring stage      ring phases <-> pixels: m is folded modulo nphi in LDS, then
+//     an in-LDS radix-2 FFT (nphi a power of two) or a Bluestein chirp-z
+//     transform over a power-of-two length.  Synthesis: one block per (ring,
+//     group of column pairs), two real columns per complex transform; adjoint:
+//     one block per (ring, map).
 //
-// Layouts: phases [map*ncomp][mmax+1][nrings] complex; maps [map][ncomp][npix];
-// alm (user) [map][ncomp][nlm]; alm (Legendre input) [nlm][ld] with the
-// (map, comp) columns contiguous for each (l, m).
+// Layouts: synthesis phases in tiles of 16 rings, [ring/16][mmax+1][ring%16][map*ncomp]
+// complex, already rotated by exp(i m phi0(ring)); adjoint phases [map*ncomp][mmax+1][nrings]; maps
+// [map][ncomp][npix]; alm (user) [map][ncomp][nlm]; alm (Legendre input) [nlm][ld]
+// with the (map, comp) columns contiguous for each (l, m).
 #include <math.h>
 #include <string.h>
 
@@ -118,6 +121,20 @@ int ShtPlan::build() {
         }
         maxM = std::max<long>(maxM, M);
     }
+    // ring lists for the synthesis FFTs: power-of-two rings first, then Bluestein rings
+    std::vector<int> rlist;
+    maxM_pow2 = maxM_blue = 1;
+    for (int i = 0; i < nrings; i++)
+        if (rings[i].voff < 0) {
+            rlist.push_back(i);
+            maxM_pow2 = std::max<long>(maxM_pow2, rings[i].M);
+        }
+    n_pow2 = (int)rlist.size();
+    for (int i = 0; i < nrings; i++)
+        if (rings[i].voff >= 0) {
+            rlist.push_back(i);
+            maxM_blue = std::max<long>(maxM_blue, rings[i].M);
+        }
     // twiddles e^{-2 pi i k / maxM}, k < maxM/2
     std::vector<double> tw((size_t)maxM);  // interleaved re, im
     for (long k = 0; k < maxM / 2; k++) {
@@ -234,6 +251,7 @@ int ShtPlan::build() {
 
     // ---- upload
     if (upload(&d_rings, rings)) return -2;
+    if (upload(&d_rlist, rlist)) return -2;
     if (upload(&d_l2s, l2s)) return -2;
     if (upload(&d_l2sh, l2sh)) return -2;
     if (upload(&d_l2ch, l2ch)) return -2;
@@ -257,7 +275,7 @@ int ShtPlan::build() {
 
 ShtPlan::~ShtPlan() {
     void* ptrs[] = {d_rings, d_l2s, d_l2sh, d_l2ch, d_seedL, d_seedS, d_seedA,
-                    d_seedB, d_cth, d_tw, d_blue, d_coef};
+                    d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -345,6 +363,33 @@ __device__ __forceinline__ int m_of_q(int q, int mmax) {
     return (q & 1) ? (mmax - (q >> 1)) : (q >> 1);
 }
 
+template <typename T>
+__device__ __forceinline__ cplx<T> expi_pi(long num, long den) {
+    // exp(i pi num/den) with exact argument reduction
+    long r = num % (2 * den);
+    if (r < 0) r += 2 * den;
+    T s, c;
+    sincospi_t((T)r / (T)den, &s, &c);
+    return cplx<T>{c, s};
+}
+
+// exp(i pi num/den) for 0 <= num < 2^31
+template <typename T>
+__device__ __forceinline__ cplx<T> expi_pi32(int num, int den) {
+    const int r = num % (2 * den);
+    T s, c;
+    sincospi_t((T)r / (T)den, &s, &c);
+    return cplx<T>{c, s};
+}
+
+// Synthesis phase layout: tiles of PRT consecutive rings, [ring / PRT][m][ring % PRT][col].
+// A wave of the Legendre stage (64 consecutive rings, one m) writes a few compact
+// regions; the ring stage reads one ring's m rows PRT * ncol elements apart.
+constexpr int PRT = 16;
+__host__ __device__ __forceinline__ size_t syn_phase_off(int ring, int m, int mmax, int ncol) {
+    return (((size_t)(ring / PRT) * (mmax + 1) + m) * PRT + (ring % PRT)) * (size_t)ncol;
+}
+
 // Recursion step without the rescale test: valid once k == 0, after which the
 // normalised functions only grow to their O(1) oscillating regime.
 __device__ __forceinline__ void rec_fast(Rec& r, double ax, double g) {
@@ -424,16 +469,31 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
             }
         }
     } else {
+        // chunk staging: thread t holds alm element t (row t / BC) and the (alpha, gamma) of
+        // row t; the next chunk is fetched while the current one is processed
+        static_assert(LCH * BC <= 256, "one alm element per thread per chunk");
+        cplx<T> pa{T(0), T(0)};
+        double pal = 0.0, pga = 0.0;
+        auto fetch = [&](int l0n) {
+            const int nn = min(LCH, d.lmax + 1 - l0n);
+            if (tid < nn * BC) pa = a[(size_t)(base + l0n + tid / BC) * ld + tid % BC];
+            if (tid < nn) {
+                pal = cf4[4 * (l0n + tid)];
+                pga = cf4[4 * (l0n + tid) + 2];
+            }
+        };
+        fetch(m);
         bool fast = false;
         for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
             const int n = min(LCH, d.lmax + 1 - l0);
             __syncthreads();
-            for (int i = tid; i < n * BC; i += 256) {
-                const int li = i / BC, j = i - li * BC;
-                sa[i] = a[(size_t)(base + l0 + li) * ld + j];
+            if (tid < n * BC) sa[tid] = pa;
+            if (tid < n) {
+                sco[2 * tid] = pal;
+                sco[2 * tid + 1] = pga;
             }
-            for (int i = tid; i < 2 * n; i += 256) sco[i] = cf4[4 * (l0 + (i >> 1)) + 2 * (i & 1)];
             __syncthreads();
+            if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
             if (!fast) fast = __all(r.k == 0);
             if (fast) {
                 int li = 0;
@@ -491,14 +551,156 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
     }
     if (jp >= d.npairs) return;
     const int rn = jp, rs = d.nrings - 1 - jp;
-    const size_t pstride = (size_t)(d.mmax + 1) * d.nrings;
+    const RingInfo ri = d.rings[rn];
+    const cplx<T> rot = expi_pi<T>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0), same for both rings
+    cplx<T>* phn = phase + syn_phase_off(rn, m, d.mmax, ncol) + chunk * BC;
+    cplx<T>* phs = phase + syn_phase_off(rs, m, d.mmax, ncol) + chunk * BC;
 #pragma unroll
     for (int j = 0; j < BC; j++) {
-        const int col = chunk * BC + j;
-        if (col < ncol) {
-            cplx<T>* ph = phase + (size_t)col * pstride + (size_t)m * d.nrings;
-            ph[rn] = cadd(ev[j], od[j]);
-            if (rs != rn) ph[rs] = csub(ev[j], od[j]);
+        if (chunk * BC + j < ncol) {
+            phn[j] = cmul(cadd(ev[j], od[j]), rot);
+            if (rs != rn) phs[j] = cmul(csub(ev[j], od[j]), rot);
+        }
+    }
+}
+
+// ------------------------------------- Legendre synthesis, spin 0, FP32 matrix cores
+// The contraction F_m(ring, col) = sum_l lam_lm(ring) a_lm(col) of one m for 32 ring pairs
+// and 16 complex (32 real) columns runs on v_mfma_f32_32x32x2f32: A = lam (32 rings x 2
+// values of l), B = alm (2 values of l x 32 real columns), D += A B.  Even and odd l - m
+// accumulate separately (north = even + odd, south = even - odd), so one MFMA takes
+// l and l + 2.  Lane layout (A, B: one value per lane, D: 16 per lane):
+//   A[i][k] at lane i + 32 k;  B[k][j] at lane j + 32 k;
+//   D[i][j] at lane j + 32 ((i / 4) % 2), register 4 (i / 8) + i % 4.
+// Lanes i and i + 32 run the same FP64 recursion (ring i) and feed l, l + 1 (k = 0) and
+// l + 2, l + 3 (k = 1).  One block = 4 waves = 128 ring pairs, one m, one 16-column chunk;
+// the chunk's alm rows are staged in LDS.
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int ncol, int ld,
+                                                           const cplx<float>* __restrict__ alm_t,
+                                                           cplx<float>* __restrict__ phase) {
+    constexpr int BCM = 16;
+    __shared__ __align__(16) float sa[LCH * 2 * BCM];
+    __shared__ double sco[LCH * 2];
+    __shared__ cplx<float> rot[128];
+    const int nrb = (d.npairs + 127) / 128;
+    const int nchunk = (ncol + BCM - 1) / BCM;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = lb % nrb;
+    const int chunk = (lb / nrb) % nchunk;
+    const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int jp = rb * 128 + wave * 32 + li32;
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    if (tid < 128) {
+        const int jr = min(rb * 128 + tid, d.npairs - 1);
+        const RingInfo ri = d.rings[jr];
+        rot[tid] = expi_pi<float>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    }
+
+    const double x = d.cth[jpc];
+    Rec r;
+    rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
+    v16f Dev = {0}, Dod = {0};
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const float* a = reinterpret_cast<const float*>(alm_t + (size_t)chunk * BCM);
+    const double* cf4 = d.coef + 4 * base;
+    // chunk staging: thread t holds 4 floats of alm row t / 8 and the (alpha, gamma) of row t;
+    // the next chunk is fetched into registers while the current one is processed
+    static_assert(LCH * 2 * BCM == 4 * 256, "one float4 per thread per chunk");
+    float4 pa;
+    double pal = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        const int nn = min(LCH, d.lmax + 1 - l0n);
+        const int li = tid >> 3;
+        pa = (li < nn) ? *reinterpret_cast<const float4*>(a + (size_t)(base + l0n + li) * 2 * ld + 4 * (tid & 7))
+                       : float4{0.f, 0.f, 0.f, 0.f};
+        if (tid < nn) {
+            pal = cf4[4 * (l0n + tid)];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    fetch(m);
+    bool fast = false;
+    for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
+        const int n = min(LCH, d.lmax + 1 - l0);
+        __syncthreads();
+        *reinterpret_cast<float4*>(sa + 4 * tid) = pa;
+        if (tid < n) {
+            sco[2 * tid] = pal;
+            sco[2 * tid + 1] = pga;
+        }
+        __syncthreads();
+        if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
+        if (!fast) fast = __all(r.k == 0);
+        if (fast && n == LCH) {
+            // every scale exponent is zero and the chunk is full: straight-line groups
+#pragma unroll 2
+            for (int g = 0; g < LCH; g += 4) {
+                const double v0 = r.f;
+                rec_fast(r, sco[2 * g] * x, sco[2 * g + 1]);
+                const double v1 = r.f;
+                rec_fast(r, sco[2 * g + 2] * x, sco[2 * g + 3]);
+                const double v2 = r.f;
+                rec_fast(r, sco[2 * g + 4] * x, sco[2 * g + 5]);
+                const double v3 = r.f;
+                rec_fast(r, sco[2 * g + 6] * x, sco[2 * g + 7]);
+                const float aev = (float)(h ? v2 : v0);
+                const float aod = (float)(h ? v3 : v1);
+                const float bev = sa[(g + 2 * h) * 2 * BCM + li32];
+                const float bod = sa[(g + 1 + 2 * h) * 2 * BCM + li32];
+                Dev = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev, 0, 0, 0);
+                Dod = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod, 0, 0, 0);
+            }
+            continue;
+        }
+        for (int g = 0; g < n; g += 4) {
+            // recursion values at l0+g .. l0+g+3 (zero past the end of the chunk)
+            double v[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int li = g + c;
+                if (li < n) {
+                    v[c] = fast ? r.f : r.f * r.cf;
+                    if (fast)
+                        rec_fast(r, sco[2 * li] * x, sco[2 * li + 1]);
+                    else
+                        rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
+                } else {
+                    v[c] = 0.0;
+                }
+            }
+            if (!fast && !__any(r.k >= scale_consts<float>::KMIN)) continue;  // all values below FP32 range
+            const float aev = (float)(h ? v[2] : v[0]);
+            const float aod = (float)(h ? v[3] : v[1]);
+            const int le = g + 2 * h, lo = g + 1 + 2 * h;
+            const float bev = le < n ? sa[le * 2 * BCM + li32] : 0.f;
+            const float bod = lo < n ? sa[lo * 2 * BCM + li32] : 0.f;
+            Dev = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev, 0, 0, 0);
+            Dod = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod, 0, 0, 0);
+        }
+    }
+    // D[i][j]: ring i of the wave, real column j = lane % 32 (complex column j / 2, part j & 1)
+    const int col = chunk * BCM + (li32 >> 1);
+    const bool re = (li32 & 1) == 0;
+#pragma unroll
+    for (int vv = 0; vv < 16; vv++) {
+        const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+        const int jr = rb * 128 + wave * 32 + i;
+        const float e = Dev[vv], o = Dod[vv];
+        float nv = e + o, sv = e - o;
+        const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
+        const cplx<float> w = rot[wave * 32 + i];
+        // (re + i im)(w.re + i w.im)
+        const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+        const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+        if (jr < d.npairs && col < ncol) {
+            const int rn = jr, rs = d.nrings - 1 - jr;
+            reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+            if (rs != rn) reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
         }
     }
 }
@@ -551,62 +753,98 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
     const cplx<T>* a = alm_t + (size_t)chunk * BC * 2;
     const double* cf4 = d.coef + 4 * base;
     const int lstart = m > d.spin ? m : d.spin;
-    bool fast = false;
-    for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
-        const int n = min(LCH, d.lmax + 1 - l0);
-        __syncthreads();
-        for (int i = tid; i < n * 2 * BC; i += 256) {
-            const int li = i / (2 * BC), j = i - li * 2 * BC;
-            sa[i] = a[(size_t)(base + l0 + li) * ld + j];
-        }
-        for (int i = tid; i < 3 * n; i += 256) {
-            const int li = i / 3, c = i - 3 * li;
-            sco[i] = cf4[4 * (l0 + li) + c];
-        }
-        __syncthreads();
-        if (!fast) fast = __all(rp.k == 0 && rm.k == 0);
-        if (fast) {
-            int li = 0;
-            for (; li + 1 < n; li += 2) {
-                synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
-                double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
-                rec_fast(rp, al * (x + be), ga);
-                rec_fast(rm, al * (x - be), ga);
-                synth_sp_acc<T, BC>(acc1, sa + (li + 1) * 2 * BC, (T)(0.5 * (rp.f + rm.f)),
-                                    (T)(0.5 * (rp.f - rm.f)));
-                al = sco[3 * li + 3], be = sco[3 * li + 4], ga = sco[3 * li + 5];
-                rec_fast(rp, al * (x + be), ga);
-                rec_fast(rm, al * (x - be), ga);
+    if constexpr (BC >= 4) {
+        // wide chunks: alm rows straight from memory by wave-uniform (scalar) loads
+        for (int l = lstart; l <= d.lmax; l++) {
+            if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
+                const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+                const cplx<T>* al = a + (size_t)(base + l) * ld;
+                if ((l - lstart) & 1)
+                    synth_sp_acc<T, BC>(acc1, al, lp, ln);
+                else
+                    synth_sp_acc<T, BC>(acc0, al, lp, ln);
             }
-            if (li < n)
-                synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
-        } else {
-            for (int li = 0; li < n; li++) {
-                if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
-                    const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
-                    const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
-                    if (li & 1)
-                        synth_sp_acc<T, BC>(acc1, sa + li * 2 * BC, lp, ln);
-                    else
-                        synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, lp, ln);
+            const double* c = cf4 + 4 * l;
+            rec_step(rp, c[0] * (x + c[1]), c[2]);
+            rec_step(rm, c[0] * (x - c[1]), c[2]);
+        }
+    } else {
+        // chunk staging: thread t holds alm elements t and t + 256 (rows / (2 BC)) and the
+        // (alpha, beta, gamma) of row t; the next chunk is fetched while the current one runs
+        static_assert(LCH * 2 * BC <= 512, "two alm elements per thread per chunk");
+        cplx<T> pa0{T(0), T(0)}, pa1{T(0), T(0)};
+        double pal = 0.0, pbe = 0.0, pga = 0.0;
+        auto fetch = [&](int l0n) {
+            const int nn = min(LCH, d.lmax + 1 - l0n);
+            if (tid < nn * 2 * BC) pa0 = a[(size_t)(base + l0n + tid / (2 * BC)) * ld + tid % (2 * BC)];
+            if (tid + 256 < nn * 2 * BC)
+                pa1 = a[(size_t)(base + l0n + (tid + 256) / (2 * BC)) * ld + (tid + 256) % (2 * BC)];
+            if (tid < nn) {
+                pal = cf4[4 * (l0n + tid)];
+                pbe = cf4[4 * (l0n + tid) + 1];
+                pga = cf4[4 * (l0n + tid) + 2];
+            }
+        };
+        fetch(lstart);
+        bool fast = false;
+        for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
+            const int n = min(LCH, d.lmax + 1 - l0);
+            __syncthreads();
+            if (tid < n * 2 * BC) sa[tid] = pa0;
+            if (tid + 256 < n * 2 * BC) sa[tid + 256] = pa1;
+            if (tid < n) {
+                sco[3 * tid] = pal;
+                sco[3 * tid + 1] = pbe;
+                sco[3 * tid + 2] = pga;
+            }
+            __syncthreads();
+            if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
+            if (!fast) fast = __all(rp.k == 0 && rm.k == 0);
+            if (fast) {
+                int li = 0;
+                for (; li + 1 < n; li += 2) {
+                    synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
+                    double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                    rec_fast(rp, al * (x + be), ga);
+                    rec_fast(rm, al * (x - be), ga);
+                    synth_sp_acc<T, BC>(acc1, sa + (li + 1) * 2 * BC, (T)(0.5 * (rp.f + rm.f)),
+                                        (T)(0.5 * (rp.f - rm.f)));
+                    al = sco[3 * li + 3], be = sco[3 * li + 4], ga = sco[3 * li + 5];
+                    rec_fast(rp, al * (x + be), ga);
+                    rec_fast(rm, al * (x - be), ga);
                 }
-                const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
-                rec_step(rp, al * (x + be), ga);
-                rec_step(rm, al * (x - be), ga);
+                if (li < n)
+                    synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
+            } else {
+                for (int li = 0; li < n; li++) {
+                    if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
+                        const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                        const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+                        if (li & 1)
+                            synth_sp_acc<T, BC>(acc1, sa + li * 2 * BC, lp, ln);
+                        else
+                            synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, lp, ln);
+                    }
+                    const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                    rec_step(rp, al * (x + be), ga);
+                    rec_step(rm, al * (x - be), ga);
+                }
             }
         }
     }
     if (jp >= d.npairs) return;
     const int rn = jp, rs = d.nrings - 1 - jp;
-    const size_t pstride = (size_t)M1 * d.nrings;
+    const RingInfo ri = d.rings[rn];
+    const cplx<T> rot = expi_pi<T>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    const int ncol = 2 * nmap;
     // li parity 0 holds l + m even unless l_start + m is odd (m < spin)
     const bool swp = ((lstart + m) & 1) != 0;
 #pragma unroll
     for (int j = 0; j < BC; j++) {
         const int b = chunk * BC + j;
         if (b >= nmap) continue;
-        cplx<T>* phR = phase + (size_t)(2 * b) * pstride + (size_t)m * d.nrings;
-        cplx<T>* phI = phase + (size_t)(2 * b + 1) * pstride + (size_t)m * d.nrings;
+        cplx<T> R[2], I[2];  // north, south
         // R = -(G lam+ + i C lam-), I = -(C lam+ - i G lam-); south: lam+ -> (-1)^(l+m),
         // lam- -> -(-1)^(l+m)
         const cplx<T> GPe = swp ? acc1[j][0] : acc0[j][0], GPo = swp ? acc0[j][0] : acc1[j][0];
@@ -615,14 +853,20 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
         const cplx<T> CMe = swp ? acc1[j][3] : acc0[j][3], CMo = swp ? acc0[j][3] : acc1[j][3];
         cplx<T> t;
         t = cadd(CMe, CMo);
-        phR[rn] = cplx<T>{-(GPe.re + GPo.re) + t.im, -(GPe.im + GPo.im) - t.re};
+        R[0] = cplx<T>{-(GPe.re + GPo.re) + t.im, -(GPe.im + GPo.im) - t.re};
         t = cadd(GMe, GMo);
-        phI[rn] = cplx<T>{-(CPe.re + CPo.re) - t.im, -(CPe.im + CPo.im) + t.re};
+        I[0] = cplx<T>{-(CPe.re + CPo.re) - t.im, -(CPe.im + CPo.im) + t.re};
+        t = csub(CMo, CMe);
+        R[1] = cplx<T>{-(GPe.re - GPo.re) + t.im, -(GPe.im - GPo.im) - t.re};
+        t = csub(GMo, GMe);
+        I[1] = cplx<T>{-(CPe.re - CPo.re) - t.im, -(CPe.im - CPo.im) + t.re};
+        cplx<T>* phn = phase + syn_phase_off(rn, m, d.mmax, ncol) + 2 * b;
+        phn[0] = cmul(R[0], rot);
+        phn[1] = cmul(I[0], rot);
         if (rs != rn) {
-            t = csub(CMo, CMe);
-            phR[rs] = cplx<T>{-(GPe.re - GPo.re) + t.im, -(GPe.im - GPo.im) - t.re};
-            t = csub(GMo, GMe);
-            phI[rs] = cplx<T>{-(CPe.re - CPo.re) - t.im, -(CPe.im - CPo.im) + t.re};
+            cplx<T>* phs = phase + syn_phase_off(rs, m, d.mmax, ncol) + 2 * b;
+            phs[0] = cmul(R[1], rot);
+            phs[1] = cmul(I[1], rot);
         }
     }
 }
@@ -669,62 +913,259 @@ __device__ void fft_dit(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long 
     }
 }
 
+// LDS element index of entry i of a transform buffer: one pad slot per 32 entries keeps
+// bit-reversed reads and short butterfly strides off a single bank; buffers are
+// spaced fft_pitch(M) apart (odd pitch: the NB buffers of a fold start on distinct banks).
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
+__host__ __device__ __forceinline__ int fft_pitch(int M) { return M + (M >> 5) + 1; }
+
+// Batched transforms over NB padded buffers of length M <= 256 * 4 * FFT_MAXQ.  Two
+// radix-2 stages are fused into one radix-2^2 pass in registers (half the LDS traffic,
+// same bit-reversed ordering); with log2 M odd one plain radix-2 stage remains.  The
+// twiddles of a pass are loaded for all of a thread's butterflies before any of them
+// runs.  tw[k] = exp(-2 pi i k / maxM); the inverse transform uses conjugates.
+constexpr int FFT_MAXQ = 8;
+
+__device__ __forceinline__ int ilog2_dev(int v) { return 31 - __clz(v); }
+
 template <typename T>
-__device__ __forceinline__ cplx<T> expi_pi(long num, long den) {
-    // exp(i pi num/den) with exact argument reduction
-    long r = num % (2 * den);
-    if (r < 0) r += 2 * den;
-    T s, c;
-    sincospi_t((T)r / (T)den, &s, &c);
-    return cplx<T>{c, s};
+__device__ __forceinline__ cplx<T> mul_mi(cplx<T> a, bool inv) {  // a * (-i), or a * i for inv
+    return inv ? cplx<T>{-a.im, a.re} : cplx<T>{a.im, -a.re};
 }
 
-// phases [col][mmax+1][nrings] -> maps [col][npix]
-template <typename T>
-__global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const cplx<T>* __restrict__ phase,
-                                                    T* __restrict__ map, long map_stride) {
+// one radix-2 stage of half-size h over all buffers (DIF when !dit)
+template <typename T, int NB>
+__device__ __forceinline__ void fft_r2_stage(cplx<T>* a, int M, int h, const cplx<T>* __restrict__ tw, long maxM,
+                                             bool inv, bool dit) {
+    const int half = M >> 1, pitch = fft_pitch(M);
+    const long ts = maxM / (2L * h);
+    cplx<T> w[2 * FFT_MAXQ];
+#pragma unroll
+    for (int k = 0; k < 2 * FFT_MAXQ; k++) {
+        const int u = threadIdx.x + 256 * k;
+        if (u < half) {
+            w[k] = tw[(u & (h - 1)) * ts];
+            if (inv) w[k].im = -w[k].im;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * FFT_MAXQ; k++) {
+        const int u = threadIdx.x + 256 * k;
+        if (u < half) {
+            const int pos = u & (h - 1);
+            const int i = pidx(((u - pos) << 1) + pos), j = pidx(((u - pos) << 1) + pos + h);
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                cplx<T>* b = a + q * pitch;
+                if (dit) {
+                    const cplx<T> x = b[i], y = cmul(b[j], w[k]);
+                    b[i] = cadd(x, y);
+                    b[j] = csub(x, y);
+                } else {
+                    const cplx<T> x = b[i], y = b[j];
+                    b[i] = cadd(x, y);
+                    b[j] = cmul(csub(x, y), w[k]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Decimation in frequency: natural order in, bit-reversed out.  Fused stages (h, h/2).
+template <typename T, int NB>
+__device__ void fft_dif_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
+    const int quarter = M >> 2, pitch = fft_pitch(M);
+    int h = M >> 1;
+    for (; h >= 2; h >>= 2) {
+        const int hh = h >> 1;
+        const long ts = maxM / (2L * h);  // stage h; stage h/2 uses 2 ts
+        cplx<T> wA[FFT_MAXQ];
+#pragma unroll
+        for (int k = 0; k < FFT_MAXQ; k++) {
+            const int v = threadIdx.x + 256 * k;
+            if (v < quarter) {
+                wA[k] = tw[(v & (hh - 1)) * ts];
+                if (inv) wA[k].im = -wA[k].im;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < FFT_MAXQ; k++) {
+            const int v = threadIdx.x + 256 * k;
+            if (v < quarter) {
+                const int pos = v & (hh - 1);
+                const int base = (v - pos) * 4 + pos;  // block of 2h entries, h/2 quads each
+                const int i0 = pidx(base), i1 = pidx(base + hh), i2 = pidx(base + h), i3 = pidx(base + h + hh);
+                const cplx<T> wa = wA[k], wb = mul_mi(wa, inv), wc = cmul(wa, wa);
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    cplx<T>* b = a + q * pitch;
+                    const cplx<T> e0 = b[i0], e1 = b[i1], e2 = b[i2], e3 = b[i3];
+                    const cplx<T> a0 = cadd(e0, e2), a2 = cmul(csub(e0, e2), wa);
+                    const cplx<T> a1 = cadd(e1, e3), a3 = cmul(csub(e1, e3), wb);
+                    b[i0] = cadd(a0, a1);
+                    b[i1] = cmul(csub(a0, a1), wc);
+                    b[i2] = cadd(a2, a3);
+                    b[i3] = cmul(csub(a2, a3), wc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (h == 1) fft_r2_stage<T, NB>(a, M, 1, tw, maxM, inv, false);
+}
+
+// Decimation in time: bit-reversed order in, natural out.  Fused stages (h, 2h).
+template <typename T, int NB>
+__device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
+    const int quarter = M >> 2, pitch = fft_pitch(M);
+    int h = 1;
+    if (ilog2_dev(M) & 1) {
+        fft_r2_stage<T, NB>(a, M, 1, tw, maxM, inv, true);
+        h = 2;
+    }
+    for (; 2 * h < M; h <<= 2) {
+        const long ts = maxM / (2L * h);  // stage h; stage 2h uses ts / 2
+        cplx<T> wA[FFT_MAXQ], wB[FFT_MAXQ];
+#pragma unroll
+        for (int k = 0; k < FFT_MAXQ; k++) {
+            const int v = threadIdx.x + 256 * k;
+            if (v < quarter) {
+                const int pos = v & (h - 1);
+                wA[k] = tw[pos * ts];
+                wB[k] = tw[pos * (ts >> 1)];
+                if (inv) {
+                    wA[k].im = -wA[k].im;
+                    wB[k].im = -wB[k].im;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < FFT_MAXQ; k++) {
+            const int v = threadIdx.x + 256 * k;
+            if (v < quarter) {
+                const int pos = v & (h - 1);
+                const int base = (v - pos) * 4 + pos;  // block of 4h entries, h quads each
+                const int i0 = pidx(base), i1 = pidx(base + h), i2 = pidx(base + 2 * h), i3 = pidx(base + 3 * h);
+                const cplx<T> wa = wA[k], wb = wB[k], wc = mul_mi(wb, inv);
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    cplx<T>* b = a + q * pitch;
+                    const cplx<T> e0 = b[i0], e1 = cmul(b[i1], wa), e2 = b[i2], e3 = cmul(b[i3], wa);
+                    const cplx<T> a0 = cadd(e0, e1), a1 = csub(e0, e1), a2 = cadd(e2, e3), a3 = csub(e2, e3);
+                    const cplx<T> t2 = cmul(a2, wb), t3 = cmul(a3, wc);
+                    b[i0] = cadd(a0, t2);
+                    b[i2] = csub(a0, t2);
+                    b[i1] = cadd(a1, t3);
+                    b[i3] = csub(a1, t3);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// phases [ring][mmax+1][ncol], already rotated by exp(i m phi0) -> maps [col][npix].
+// One block per (ring of ring_list, group of 2 NB columns).  Columns 2q and 2q+1 of the
+// group share one complex transform: with A_j, B_j their Hermitian folds
+// (A_j = sum_{m = j mod n} F_m + sum_{m = -j mod n, m > 0} conj(F_m), Im F_0 dropped),
+// the inverse DFT of A + i B is f_2q + i f_2q+1, both real.  m is folded modulo nphi in
+// rounds of nphi consecutive values (each round hits every j once, so no two threads
+// add to one element), direct and mirrored terms in separate passes.  Then an LDS
+// radix-2 FFT (nphi a power of two) or a Bluestein chirp-z transform.
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __restrict__ ring_list, int ncol,
+                                                    const cplx<T>* __restrict__ phase, T* __restrict__ map,
+                                                    long map_stride) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
+    constexpr int G = 2 * NB;
+    const int ngrp = (ncol + G - 1) / G;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int ring = lb % d.nrings;
-    const int col = lb / d.nrings;
+    const int grp = lb % ngrp;
+    const int ring = ring_list[lb / ngrp];
     const RingInfo ri = d.rings[ring];
     const int n = ri.nphi, M = ri.M, logM = ri.logM;
     const bool pow2 = ri.voff < 0;
-    const cplx<T>* F = phase + ((size_t)col * (d.mmax + 1)) * d.nrings + ring;
-    const long phq = ri.phi_q, php = ri.phi_p;
+    const int tid = threadIdx.x;
+    const int c0 = grp * G;
+    const cplx<T>* P = phase + syn_phase_off(ring, 0, d.mmax, ncol) + c0;
+    const size_t mstride = (size_t)PRT * ncol;
+    __shared__ cplx<T> part[256];
 
-    if (!pow2)
-        for (int j = threadIdx.x; j < M; j += blockDim.x) buf[j] = cplx<T>{T(0), T(0)};
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        cplx<T> c{T(0), T(0)};
-        for (int m = j; m <= d.mmax; m += n) {
-            cplx<T> g = cmul(F[(size_t)m * d.nrings], expi_pi<T>(m * php, phq));
-            c = cadd(c, g);
+    const int pitch = fft_pitch(M);
+    if (!pow2)  // Bluestein: the input is zero-padded from n to M
+        for (int i = tid; i < NB * pitch; i += 256) buf[i] = cplx<T>{T(0), T(0)};
+    // Fold, owner computes: item (j, q) sums m = j + k n (direct) and m = (n - j) + k n or,
+    // for j = 0, n + k n (mirrored, conjugated), split over S slices of k when the ring
+    // has fewer than 256 items; slices are added in a fixed order.
+    const int nitem = n * NB;
+    const int S = nitem >= 256 ? 1 : 256 / nitem;
+    const int step = S * n;
+    for (int i = tid; i < nitem * S; i += 256) {
+        const int sl = i / nitem, r = i - sl * nitem;
+        const int j = r / NB, q = r - j * NB;
+        const bool hasA = c0 + 2 * q < ncol, hasB = c0 + 2 * q + 1 < ncol;
+        const cplx<T>* src = P + 2 * q;
+        cplx<T> acc{T(0), T(0)};
+        for (int m = j + sl * n; m <= d.mmax; m += step) {
+            cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
+            if (hasA) A = src[(size_t)m * mstride];
+            if (hasB) B = src[(size_t)m * mstride + 1];
+            if (m == 0) A.im = B.im = T(0);
+            acc.re += A.re - B.im;  // A + i B
+            acc.im += A.im + B.re;
         }
-        for (int m = (j == 0) ? n : n - j; m <= d.mmax; m += n) {
-            cplx<T> g = cmul(F[(size_t)m * d.nrings], expi_pi<T>(m * php, phq));
-            c = cadd(c, cconj(g));
+        for (int m = (j ? n - j : n) + sl * n; m <= d.mmax; m += step) {
+            cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
+            if (hasA) A = src[(size_t)m * mstride];
+            if (hasB) B = src[(size_t)m * mstride + 1];
+            acc.re += A.re + B.im;  // conj(A) + i conj(B)
+            acc.im += B.re - A.im;
         }
-        if (!pow2) c = cmul(c, expi_pi<T>((long)j * j, n));  // Bluestein pre-chirp
-        buf[j] = c;
+        if (S == 1) {
+            if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));  // Bluestein pre-chirp
+            buf[q * pitch + pidx(j)] = acc;
+        } else {
+            part[i] = acc;
+        }
+    }
+    if (S > 1) {
+        __syncthreads();
+        for (int r = tid; r < nitem; r += 256) {
+            cplx<T> acc = part[r];
+            for (int sl = 1; sl < S; sl++) acc = cadd(acc, part[sl * nitem + r]);
+            const int j = r / NB, q = r - j * NB;
+            if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));
+            buf[q * pitch + pidx(j)] = acc;
+        }
     }
     __syncthreads();
-    T* out = map + (size_t)col * map_stride + ri.start;
+    const int nc = min(G, ncol - c0);
     if (pow2) {
-        fft_dif(buf, M, d.tw, d.maxM, true);
-        for (int k = threadIdx.x; k < n; k += blockDim.x) out[k] = buf[bitrev(k, logM)].re;
+        fft_dif_nb<T, NB>(buf, M, d.tw, d.maxM, true);
+        for (int i = tid; i < NB * n; i += 256) {
+            const int q = i / n, k = i - q * n;
+            const cplx<T> z = buf[q * pitch + pidx(bitrev(k, logM))];
+            if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re;
+            if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im;
+        }
     } else {
-        fft_dif(buf, M, d.tw, d.maxM, false);
+        fft_dif_nb<T, NB>(buf, M, d.tw, d.maxM, false);
         const cplx<T>* V = d.blue + ri.voff;
-        for (int k = threadIdx.x; k < M; k += blockDim.x) buf[k] = cmul(buf[k], V[k]);
+        for (int i = tid; i < NB * M; i += 256) {
+            const int q = i >> logM, k = i & (M - 1);
+            cplx<T>& e = buf[q * pitch + pidx(k)];
+            e = cmul(e, V[k]);
+        }
         __syncthreads();
-        fft_dit(buf, M, d.tw, d.maxM, true);
+        fft_dit_nb<T, NB>(buf, M, d.tw, d.maxM, true);
         const T invM = T(1) / (T)M;
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            cplx<T> v = cmul(buf[k], expi_pi<T>((long)k * k, n));
-            out[k] = v.re * invM;
+        for (int i = tid; i < NB * n; i += 256) {
+            const int q = i / n, k = i - q * n;
+            const cplx<T> z = cmul(buf[q * pitch + pidx(k)], expi_pi32<T>(k * k, n));
+            if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
+            if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
         }
     }
 }
@@ -880,11 +1321,24 @@ __global__ __launch_bounds__(64 * NW) void k_leg_adj_s0(ShtDev<T> d, int ncol, c
     }
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const double* cf4 = d.coef + 4 * base;
+    // coefficients of the next chunk are fetched into registers while the current one runs
+    double pal = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        if (tid < min(LCH, d.lmax + 1 - l0n)) {
+            pal = cf4[4 * (l0n + tid)];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    fetch(m);
     for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
         const int n = min(LCH, d.lmax + 1 - l0);
         __syncthreads();
-        for (int i = tid; i < 2 * n; i += NT) sco[i] = cf4[4 * (l0 + (i >> 1)) + 2 * (i & 1)];
+        if (tid < n) {
+            sco[2 * tid] = pal;
+            sco[2 * tid + 1] = pga;
+        }
         __syncthreads();
+        if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
         bool allk0 = true;
 #pragma unroll
         for (int q = 0; q < NRT; q++) allk0 = allk0 && (r[q].k == 0);
@@ -1031,14 +1485,25 @@ __global__ __launch_bounds__(64 * NW) void k_leg_adj_sp(ShtDev<T> d, int nmap, c
         if (l <= d.lmax && b < nmap) out[(size_t)(2 * b + g) * alm_stride + base + l] = cplx<T>{T(0), T(0)};
     }
     const int p0 = (lstart + m) & 1;
+    double pal = 0.0, pbe = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        if (tid < min(LCH, d.lmax + 1 - l0n)) {
+            pal = cf4[4 * (l0n + tid)];
+            pbe = cf4[4 * (l0n + tid) + 1];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    fetch(lstart);
     for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
         const int n = min(LCH, d.lmax + 1 - l0);
         __syncthreads();
-        for (int i = tid; i < 3 * n; i += NT) {
-            const int li = i / 3, c = i - 3 * li;
-            sco[i] = cf4[4 * (l0 + li) + c];
+        if (tid < n) {
+            sco[3 * tid] = pal;
+            sco[3 * tid + 1] = pbe;
+            sco[3 * tid + 2] = pga;
         }
         __syncthreads();
+        if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
         bool allk0 = true;
 #pragma unroll
         for (int q = 0; q < NRT; q++) allk0 = allk0 && (rp[q].k == 0) && (rm[q].k == 0);
@@ -1135,18 +1600,45 @@ static int launch_synth_sp(const ShtPlan& p, int nmap, int ld, const void* alm_t
     return 0;
 }
 
-template <typename T>
-static int launch_ring_synth(const ShtPlan& p, int ncol, const void* phase, void* map, long map_stride,
-                             hipStream_t st) {
+// Column pairs per block: as many FFT buffers as fit 64 KB of LDS for the ring class.
+template <typename T, int NB>
+static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
+                                const void* phase, void* map, long map_stride, hipStream_t st) {
     ShtDev<T> d = dev_view<T>(p);
-    size_t sh = lds_bytes<T>(p);
-    if (sh > 65536)
-        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T>,
+    const size_t sh = (size_t)NB * fft_pitch((int)maxM) * sizeof(cplx<T>);
+    if (sh + 256 * sizeof(cplx<T>) > 65536)  // dynamic buffers plus the static slice partials
+        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T, NB>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(k_ring_synth<T>, dim3((unsigned)(p.nrings * ncol)), dim3(256), sh, st, d,
+    const int ngrp = (ncol + 2 * NB - 1) / (2 * NB);
+    hipLaunchKernelGGL((k_ring_synth<T, NB>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d, list, ncol,
                        (const cplx<T>*)phase, (T*)map, map_stride);
     PS_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename T>
+static int launch_ring_synth_list(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
+                                  const void* phase, void* map, long map_stride, hipStream_t st) {
+    if (nlist == 0) return 0;
+    if (maxM > 4L * 256 * FFT_MAXQ) {
+        set_error("ring synthesis: FFT length above 8192 is not supported");
+        return -1;
+    }
+    const long per = maxM * (long)sizeof(cplx<T>);
+    if (ncol >= 8 && 4 * per <= 65536)
+        return launch_ring_synth_nb<T, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    if (ncol >= 4 && 2 * per <= 65536)
+        return launch_ring_synth_nb<T, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    return launch_ring_synth_nb<T, 1>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+}
+
+template <typename T>
+static int launch_ring_synth(const ShtPlan& p, int ncol, const void* phase, void* map, long map_stride,
+                             hipStream_t st) {
+    int rc = launch_ring_synth_list<T>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st);
+    if (rc) return rc;
+    return launch_ring_synth_list<T>(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol, phase, map,
+                                     map_stride, st);
 }
 
 template <typename T>
@@ -1269,14 +1761,28 @@ int sht_synth_lmajor_ld_multiple(const ShtPlan& p) {
 }
 
 size_t sht_phase_bytes(const ShtPlan& p, int nmap) {
-    return (size_t)nmap * p.ncomp * (p.mmax + 1) * p.nrings * 2 * p.prec;
+    // synthesis tiles cover the ring count rounded up to PRT (the adjoint layout is smaller)
+    const size_t rings = (size_t)(p.nrings + PRT - 1) / PRT * PRT;
+    return (size_t)nmap * p.ncomp * (p.mmax + 1) * rings * 2 * p.prec;
 }
 
 // Column-chunk size of the synthesis kernels: the smallest instantiated chunk covering
 // the batch (a single map uses BC = 1 instead of wasting the padding columns).
+static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
+                                hipStream_t st) {
+    ShtDev<float> d = dev_view<float>(p);
+    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 15) / 16;
+    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    hipLaunchKernelGGL(k_leg_synth_s0_mfma, dim3(nb), dim3(256), 0, st, d, ncol, ld, (const cplx<float>*)alm_t,
+                       (cplx<float>*)phase);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 template <typename T>
 static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st) {
     constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
+    if (sizeof(T) == 4 && ncol > 8) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
     if (ncol <= 1) return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
     if (ncol <= 2) return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
     if (ncol <= 4) return launch_synth_s0<T, 4>(p, ncol, ld, alm_t, phase, st);
