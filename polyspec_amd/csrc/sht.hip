@@ -346,16 +346,25 @@ __device__ __forceinline__ void rec_seed(Rec& r, double l2v, double sign) {
 }
 
 __device__ __forceinline__ void rec_step(Rec& r, double ax, double g) {
-    // ax = alpha * (x + beta), g = gamma
+    // ax = alpha * (x + beta), g = gamma; the scale exponent is updated by rec_rescale
     const double fn = ax * r.f - g * r.fm1;
     r.fm1 = r.f;
     r.f = fn;
-    if (r.k < 0 && fabs(fn) > scale_consts<double>::THR) {
-        r.f *= scale_consts<double>::DN;
-        r.fm1 *= scale_consts<double>::DN;
-        r.k += 1;
-        r.cf = scale_factor<double>(r.k);
-    }
+}
+
+// Scale-exponent update, called every few steps (at most 8) instead of after each one.
+// Scaling by 2^-64 is exact, so the values f 2^(64 k) do not depend on when it happens;
+// between calls f can grow past THR by the growth of those steps (below 2^6 per step),
+// far from overflow since the true values stay O(1).  A lagging k is at most one below
+// its per-step value, so accumulation tests use KMIN - 1.
+__device__ __forceinline__ void rec_rescale(Rec& r) {
+    const bool up = (r.k < 0) && (fmax(fabs(r.f), fabs(r.fm1)) > scale_consts<double>::THR);
+    const int e = up ? -64 : 0;
+    r.f = ldexp(r.f, e);
+    r.fm1 = ldexp(r.fm1, e);
+    r.k += up ? 1 : 0;
+    const double DN = scale_consts<double>::DN;
+    r.cf = (r.k == 0) ? 1.0 : (r.k == -1) ? DN : (r.k == -2) ? DN * DN : 0.0;
 }
 
 __device__ __forceinline__ int m_of_q(int q, int mmax) {
@@ -438,7 +447,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
         // loads; staging it in LDS would cost every lane a full row of LDS reads per l
         int l = m;
         for (; l + 1 <= d.lmax; l += 2) {
-            if (__any(r.k >= scale_consts<T>::KMIN)) {
+            if (__any(r.k >= scale_consts<T>::KMIN - 1)) {
                 const T v = (T)(r.f * r.cf);
                 const cplx<T>* al = a + (size_t)(base + l) * ld;
 #pragma unroll
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                 }
             }
             rec_step(r, cf4[4 * l] * x, cf4[4 * l + 2]);
-            if (__any(r.k >= scale_consts<T>::KMIN)) {
+            if (__any(r.k >= scale_consts<T>::KMIN - 1)) {
                 const T v = (T)(r.f * r.cf);
                 const cplx<T>* al = a + (size_t)(base + l + 1) * ld;
 #pragma unroll
@@ -458,6 +467,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                 }
             }
             rec_step(r, cf4[4 * (l + 1)] * x, cf4[4 * (l + 1) + 2]);
+            if (r.k < 0) rec_rescale(r);
         }
         if (l <= d.lmax) {  // trailing even-parity term
             const T v = (T)(r.f * r.cf);
@@ -526,7 +536,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                 }
             } else {
                 for (int li = 0; li < n; li++) {
-                    if (__any(r.k >= scale_consts<T>::KMIN)) {
+                    if (__any(r.k >= scale_consts<T>::KMIN - 1)) {
                         const T v = (T)(r.f * r.cf);
                         if (li & 1) {
     #pragma unroll
@@ -545,6 +555,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                         }
                     }
                     rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
+                    if ((li & 3) == 3) rec_rescale(r);
                 }
             }
         }
@@ -673,7 +684,8 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
                     v[c] = 0.0;
                 }
             }
-            if (!fast && !__any(r.k >= scale_consts<float>::KMIN)) continue;  // all values below FP32 range
+            if (!fast) rec_rescale(r);
+            if (!fast && !__any(r.k >= scale_consts<float>::KMIN - 1)) continue;  // all values below FP32 range
             const float aev = (float)(h ? v[2] : v[0]);
             const float aod = (float)(h ? v[3] : v[1]);
             const int le = g + 2 * h, lo = g + 1 + 2 * h;
@@ -756,7 +768,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
     if constexpr (BC >= 4) {
         // wide chunks: alm rows straight from memory by wave-uniform (scalar) loads
         for (int l = lstart; l <= d.lmax; l++) {
-            if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
+            if (__any((rp.k >= scale_consts<T>::KMIN - 1) || (rm.k >= scale_consts<T>::KMIN - 1))) {
                 const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
                 const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
                 const cplx<T>* al = a + (size_t)(base + l) * ld;
@@ -768,6 +780,10 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
             const double* c = cf4 + 4 * l;
             rec_step(rp, c[0] * (x + c[1]), c[2]);
             rec_step(rm, c[0] * (x - c[1]), c[2]);
+            if (((l - lstart) & 3) == 3) {
+                rec_rescale(rp);
+                rec_rescale(rm);
+            }
         }
     } else {
         // chunk staging: thread t holds alm elements t and t + 256 (rows / (2 BC)) and the
@@ -818,7 +834,7 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
                     synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
             } else {
                 for (int li = 0; li < n; li++) {
-                    if (__any((rp.k >= scale_consts<T>::KMIN) || (rm.k >= scale_consts<T>::KMIN))) {
+                    if (__any((rp.k >= scale_consts<T>::KMIN - 1) || (rm.k >= scale_consts<T>::KMIN - 1))) {
                         const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
                         const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
                         if (li & 1)
@@ -829,6 +845,10 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
                     const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
                     rec_step(rp, al * (x + be), ga);
                     rec_step(rm, al * (x - be), ga);
+                    if ((li & 3) == 3) {
+                        rec_rescale(rp);
+                        rec_rescale(rm);
+                    }
                 }
             }
         }
@@ -1258,13 +1278,13 @@ __device__ __forceinline__ void adj_s0_chunk(cplx<T> (&acc)[LC][BC], Rec (&r)[NR
                     acc[c][j].re += v * e.re;
                     acc[c][j].im += v * e.im;
                 }
-                if (FAST)
-                    rec_fast(r[q], al * x[q], ga);
-                else
-                    rec_step(r[q], al * x[q], ga);
+                rec_fast(r[q], al * x[q], ga);
             }
         }
     }
+    if (!FAST)
+#pragma unroll
+        for (int q = 0; q < NRT; q++) rec_rescale(r[q]);
 }
 
 // ------------------------------------------------ Legendre adjoint, spin 0
@@ -1405,16 +1425,17 @@ __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp
                     acc[c][j][1].re += -lp * Ip.re - ln * Rm.im;
                     acc[c][j][1].im += -lp * Ip.im + ln * Rm.re;
                 }
-                if (FAST) {
-                    rec_fast(rp[q], al * (x[q] + be), ga);
-                    rec_fast(rm[q], al * (x[q] - be), ga);
-                } else {
-                    rec_step(rp[q], al * (x[q] + be), ga);
-                    rec_step(rm[q], al * (x[q] - be), ga);
-                }
+                rec_fast(rp[q], al * (x[q] + be), ga);
+                rec_fast(rm[q], al * (x[q] - be), ga);
             }
         }
     }
+    if (!FAST)
+#pragma unroll
+        for (int q = 0; q < NRT; q++) {
+            rec_rescale(rp[q]);
+            rec_rescale(rm[q]);
+        }
 }
 
 // ------------------------------------------------ Legendre adjoint, spin s
