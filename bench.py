@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gNL-loc numerators/s at Nside 512 (batched vs plain GPU); CG iters to 1e-6"
 PEAK_FP32 = 157.3e12
+# HBM bytes of one k_leg_synth_s0_mfma launch (80 maps, Nside 512, lmax 1024): rocprofv3 FETCH_SIZE x 2
+# + WRITE_SIZE x 1 (factors from a 1 GiB copy), profiles/leg_synth_s0_mfma_nside512_lmax1024_80maps_pmc.md
+LEG_TRAFFIC = {(512, 1024, 80): (186470.8 * 2 + 1311359.4) * 1024}
 PEAK_HBM = 8.0e12
 
 
@@ -341,9 +344,10 @@ def main():
                                "batched FP32 path (Legendre recursion FP64)" % (nside, lmax, nr, args.nsims),
                    "nside": nside, "lmax": lmax, "n_r": nr, "n_sim_pairs": args.nsims},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "k_leg_synth_s0<float,16> (Legendre synthesis, %d maps per launch, %.3f ms avg; "
-                               "VALU FP32 FMAs, priced against the FP32 peak)" % (ncol, leg_ms),
+                     "frac": achieved / PEAK_FP32, "traffic": LEG_TRAFFIC.get((nside, lmax, ncol)),
+                     "kernel": "k_leg_synth_s0_mfma (Legendre synthesis, %d maps per launch, %.3f ms avg; "
+                               "v_mfma_f32_32x32x2f32 contraction fed by the FP64 recursion, priced against the "
+                               "dense FP32 matrix peak)" % (ncol, leg_ms),
                      "algorithmic_flops_per_launch": flops},
         "mc_precompute_s": t_mc,
     }

@@ -103,7 +103,8 @@ int ps_weight_alm_lmajor(int precision, const void *f, int nslot, int nr, const 
 /* c4 * tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270) plus, when U and V are
  * given, c2 * sum_r w_r sum_p (P^2 U + P Q V): with U = sum_s P_s Q_s and
  * V = sum_s P_s^2 this equals c2 * sum_s gnl_loc_disc_sum(w, P, P_s, Q, Q_s)
- * (tauNL_utils.pyx:275-285, summed at tspec_template.py:1565-1575). */
+ * (tauNL_utils.pyx:275-285, summed at tspec_template.py:1565-1575).
+ * P, Q in the working precision; U, V are FP64 (double*) in both precisions. */
 int ps_gnl_loc_fused_sum(int precision, const void *P, const void *Q, const void *U, const void *V,
                          const double *w, int nr, long long npix, double c4, double c2, double *scratch,
                          double *out, int accumulate, void *stream);
@@ -115,7 +116,8 @@ int ps_gnl_loc_disc_sum(int precision, const void *P1, const void *P2, const voi
 
 /* U += sum_s P_s Q_s, V += sum_s P_s^2 over nsim stacked (P_s, Q_s) of n values
  * (the Monte-Carlo sums that collapse the per-simulation loop of
- * tspec_template.py:1542-1580). */
+ * tspec_template.py:1542-1580).  P, Q in the working precision; the sums U, V are
+ * FP64 (double*) in both precisions. */
 int ps_mc_accumulate(int precision, const void *P, const void *Q, int nsim, long long n, void *U, void *V,
                      void *stream);
 

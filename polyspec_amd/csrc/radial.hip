@@ -83,19 +83,20 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
 // (tauNL_utils.pyx:275-285) over the simulations s.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gnl_fused(const T* __restrict__ P, const T* __restrict__ Q,
-                                                   const T* __restrict__ U, const T* __restrict__ V,
+                                                   const double* __restrict__ U, const double* __restrict__ V,
                                                    const double* __restrict__ w, int nr, long npix, double c4,
                                                    double c2, double* __restrict__ part) {
     __shared__ double sh[256];
     double acc = 0.0;
-    const long n = (long)nr * npix;
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int r = (int)(i / npix);
-        const double p = (double)P[i], q = (double)Q[i];
-        double t = c4 * p * p * p * q;
-        if (U) t += c2 * (p * p * (double)U[i] + p * q * (double)V[i]);
-        acc += w[r] * t;
+    for (long px = (long)blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += stride) {
+        for (int r = 0; r < nr; r++) {
+            const long i = (long)r * npix + px;
+            const double p = (double)P[i], q = (double)Q[i];
+            double t = c4 * p * p * p * q;
+            if (U) t += c2 * (p * p * U[i] + p * q * V[i]);
+            acc += w[r] * t;
+        }
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -110,12 +111,13 @@ __global__ __launch_bounds__(256) void k_gnl_disc(const T* __restrict__ P1, cons
                                                   double* __restrict__ part) {
     __shared__ double sh[256];
     double acc = 0.0;
-    const long n = (long)nr * npix;
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int r = (int)(i / npix);
-        const double p1 = (double)P1[i], p2 = (double)P2[i];
-        acc += w[r] * (p1 * p1 * p2 * (double)Q2[i] + p2 * p2 * p1 * (double)Q1[i]);
+    for (long px = (long)blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += stride) {
+        for (int r = 0; r < nr; r++) {
+            const long i = (long)r * npix + px;
+            const double p1 = (double)P1[i], p2 = (double)P2[i];
+            acc += w[r] * (p1 * p1 * p2 * (double)Q2[i] + p2 * p2 * p1 * (double)Q1[i]);
+        }
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -124,17 +126,17 @@ __global__ __launch_bounds__(256) void k_gnl_disc(const T* __restrict__ P1, cons
 // U += sum over the batch of P_s Q_s, V += sum of P_s^2; P, Q: (nsim, n), U, V: (n)
 template <typename T>
 __global__ void k_mc_accumulate(const T* __restrict__ P, const T* __restrict__ Q, int nsim, long n,
-                                T* __restrict__ U, T* __restrict__ V) {
+                                double* __restrict__ U, double* __restrict__ V) {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        double u = (double)U[i], v = (double)V[i];
+        double u = U[i], v = V[i];
         for (int s = 0; s < nsim; s++) {
             const double p = (double)P[(long)s * n + i], q = (double)Q[(long)s * n + i];
             u += p * q;
             v += p * p;
         }
-        U[i] = (T)u;
-        V[i] = (T)v;
+        U[i] = u;
+        V[i] = v;
     }
 }
 
@@ -318,7 +320,7 @@ static int reduce_launch_fused(const void* P, const void* Q, const void* U, cons
                                long long npix, double c4, double c2, double* part, double* out, int accumulate,
                                hipStream_t st) {
     hipLaunchKernelGGL(k_gnl_fused<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const T*)P, (const T*)Q,
-                       (const T*)U, (const T*)V, w, nr, (long)npix, c4, c2, part);
+                       (const double*)U, (const double*)V, w, nr, (long)npix, c4, c2, part);
     PS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part, RED_BLOCKS, out, accumulate);
     PS_LAUNCH_CHECK();
@@ -379,7 +381,7 @@ int ps_mc_accumulate(int precision, const void* P, const void* Q, int nsim, long
     unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
     if (precision == 32)
         hipLaunchKernelGGL(k_mc_accumulate<float>, dim3(nb), dim3(256), 0, st, (const float*)P, (const float*)Q, nsim,
-                           (long)n, (float*)U, (float*)V);
+                           (long)n, (double*)U, (double*)V);
     else if (precision == 64)
         hipLaunchKernelGGL(k_mc_accumulate<double>, dim3(nb), dim3(256), 0, st, (const double*)P, (const double*)Q,
                            nsim, (long)n, (double*)U, (double*)V);

@@ -57,8 +57,9 @@ def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
 
 
 def gnl_loc_fused_sum(P, Q, w, c4=1.0, U=None, V=None, c2=0.0, out=None, accumulate=False):
-    """c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] as a device scalar."""
+    """c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] as a device scalar (U, V FP64)."""
     _chk(P, Q, U, V)
+    assert U is None or (U.dtype == torch.float64 and V.dtype == torch.float64), "U, V must be float64"
     w = _dev_f64(w, P.device)
     if out is None:
         out = torch.zeros((), dtype=torch.float64, device=P.device)
@@ -87,8 +88,9 @@ def gnl_loc_disc_sum(w, P1, P2, Q1, Q2, out=None, accumulate=False):
 
 
 def mc_accumulate(P, Q, U, V):
-    """U += sum_s P_s Q_s, V += sum_s P_s^2 over the leading (simulation) axis of P, Q."""
+    """U += sum_s P_s Q_s, V += sum_s P_s^2 over the leading (simulation) axis of P, Q (U, V FP64)."""
     _chk(P, Q, U, V)
+    assert U.dtype == torch.float64 and V.dtype == torch.float64, "Monte-Carlo sums U, V must be float64"
     n = U.numel()
     nsim = P.numel() // n
     check(load().ps_mc_accumulate(_prec(P), ptr(P), ptr(Q), nsim, n, ptr(U), ptr(V), stream_handle()))

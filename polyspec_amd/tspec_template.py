@@ -267,8 +267,9 @@ class TSpecTemplate:
         dev, rdt = self.base.device, self.base.rdtype
         self.t0_num = np.zeros(len(self.templates))
         if 'gNL-loc' in self.templates:
-            self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=rdt, device=dev)
-            self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=rdt, device=dev)
+            # Monte-Carlo sums in FP64 whatever the working precision
+            self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
+            self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
         if 'tauNL-loc' in self.templates:
             self.SA_d = torch.zeros((1, self.N_r, self.base.plan(self.nLmax, 0).nlm), dtype=self.base.cdtype,
                                     device=dev)

@@ -110,8 +110,8 @@ def test_pixel_kernels(gpu, prec):
     s = float(K.gnl_loc_disc_sum(w, ta, tb, tc, td))
     ref = oracle.gnl_loc_disc_sum(w, a, b, c, d)
     assert abs(s - ref) < tol * np.sum(w[:, None] * (a * a * np.abs(b * d) + b * b * np.abs(a * c)))
-    U = torch.zeros_like(ta)
-    V = torch.zeros_like(ta)
+    U = torch.zeros_like(ta, dtype=torch.float64)  # Monte-Carlo sums are FP64 in both precisions
+    V = torch.zeros_like(ta, dtype=torch.float64)
     K.mc_accumulate(torch.stack([tb, tc]).contiguous(), torch.stack([td, ta]).contiguous(), U, V)
     assert rel(U, b * d + c * a) < tol and rel(V, b * b + c * c) < tol
     f = float(K.gnl_loc_fused_sum(ta, tb, w, 0.3, U, V, -0.7))
