@@ -177,6 +177,18 @@ int ps_cg_update_xr(int precision, long long n, const double *rz, const double *
 int ps_cg_update_p(int precision, long long n, const double *num, const double *den, const void *z, void *p,
                    void *stream);
 
+/* -------------------------------------------- binned bispectrum (BSpecBin)
+ * Pixel reductions of the binned-bispectrum numerator (bspec_bin.py:339-403, the
+ * _analyze3 / _analyze1 map products summed with mp.Pool):
+ *   out[t] (+)= scale * sum_p [A_i1 B_i2 C_i3 + A_i2 B_i3 C_i1 + A_i3 B_i1 C_i2](p)
+ * for triplets t = (i1, i2, i3) (int32, ntrip x 3) of map indices into the complex
+ * stacks A, B, C (nmap x npix, nmap <= 128, FP32 or FP64 complex); out is complex FP64.
+ * scratch: ps_bspec_scratch_bytes(ntrip) bytes. */
+long long ps_bspec_scratch_bytes(int ntrip);
+int ps_bspec_triple_sums(int precision, const void *A, const void *B, const void *C, int nmap, long long npix,
+                         const int *triplets, int ntrip, double scale, void *scratch, long long scratch_bytes,
+                         void *out, int accumulate, void *stream);
+
 /* -------------------------------------------- learned CG preconditioner
  * Kernels of a 9-point-stencil convolutional network on HEALPix maps (FP32),
  * used as the nonlinear preconditioner of a flexible CG (no counterpart in the

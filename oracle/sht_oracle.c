@@ -334,10 +334,14 @@ int orc_sht_synthesis(int nside, int lmax, int mmax, int spin, int nmaps,
                     rec_t rp, rm;
                     rec_init(&rp, m, spin, th);
                     rec_init(&rm, m, -spin, th);
+                    /* sY_{l,-m} = (-1)^(s+m) conj(-sY_lm): the m < 0 half of a real
+                       field pairs +s with (-1)^s (-s); and sY(pi - theta) = (-1)^(l+m)
+                       (-s)Y(theta), so the south parity of lam+- carries (-1)^s too */
+                    const double nsg = (spin & 1) ? -1.0 : 1.0;
                     for (int l = l0; l <= lmax; l++) {
-                        double fp = rp.f * scale_factor(rp.k), fm = rm.f * scale_factor(rm.k);
+                        double fp = rp.f * scale_factor(rp.k), fm = nsg * rm.f * scale_factor(rm.k);
                         double lp = 0.5 * (fp + fm), lmn = 0.5 * (fp - fm);
-                        int par = (l + m) & 1;
+                        int par = (l + m + spin) & 1;
                         cplx gv = G[idx_lm(lmax, l, m)], cv = C[idx_lm(lmax, l, m)];
                         acc[par][0] += lp * gv;
                         acc[par][1] += lp * cv;
@@ -349,8 +353,8 @@ int orc_sht_synthesis(int nside, int lmax, int mmax, int spin, int nmaps,
                         rec_coef(l, m, -spin, &al, &be, &ga);
                         rec_step(&rm, x, al, be, ga);
                     }
-                    /* north: lam(+-) as is; south: lam+ -> (-1)^(l+m) lam+,
-                       lam- -> -(-1)^(l+m) lam- */
+                    /* north: lam(+-) as is; south: lam+ -> (-1)^(l+m+s) lam+,
+                       lam- -> -(-1)^(l+m+s) lam- */
                     cplx Rn = -(acc[0][0] + acc[1][0] + I * (acc[0][3] + acc[1][3]));
                     cplx In = -(acc[0][1] + acc[1][1] - I * (acc[0][2] + acc[1][2]));
                     cplx Rs = -(acc[0][0] - acc[1][0] + I * (acc[1][3] - acc[0][3]));
@@ -452,16 +456,20 @@ int orc_sht_adjoint(int nside, int lmax, int mmax, int spin, int nmaps, const do
                         Rs = ph[((long)b * 2) * nph + (long)is_ * (mmax + 1) + m];
                         Is = ph[((long)b * 2 + 1) * nph + (long)is_ * (mmax + 1) + m];
                     }
-                    /* sums for lam+ (sign (-1)^(l+m) south) and lam- (sign -(-1)^(l+m)) */
+                    /* sums for lam+ (sign (-1)^(l+m+s) south) and lam- (sign -(-1)^(l+m+s)) */
                     cplx Rsum = Rn + Rs, Rdif = Rn - Rs, Isum = In + Is, Idif = In - Is;
                     cplx *G = alm + ((long)b * 2) * nlm, *C = alm + ((long)b * 2 + 1) * nlm;
                     rec_t rp, rm;
                     rec_init(&rp, m, spin, th);
                     rec_init(&rm, m, -spin, th);
+                    /* sY_{l,-m} = (-1)^(s+m) conj(-sY_lm): the m < 0 half of a real
+                       field pairs +s with (-1)^s (-s); and sY(pi - theta) = (-1)^(l+m)
+                       (-s)Y(theta), so the south parity of lam+- carries (-1)^s too */
+                    const double nsg = (spin & 1) ? -1.0 : 1.0;
                     for (int l = l0; l <= lmax; l++) {
-                        double fp = rp.f * scale_factor(rp.k), fm = rm.f * scale_factor(rm.k);
+                        double fp = rp.f * scale_factor(rp.k), fm = nsg * rm.f * scale_factor(rm.k);
                         double lp = 0.5 * (fp + fm), lmn = 0.5 * (fp - fm);
-                        int even = (((l + m) & 1) == 0);
+                        int even = (((l + m + spin) & 1) == 0);
                         cplx Rp = even ? Rsum : Rdif, Ip = even ? Isum : Idif; /* with lam+ */
                         cplx Rm = even ? Rdif : Rsum, Im = even ? Idif : Isum; /* with lam- */
                         long id = idx_lm(lmax, l, m);

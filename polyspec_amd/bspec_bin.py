@@ -1,0 +1,298 @@
+"""Binned bispectrum numerator on the HIP backend (polyspec/bspec_bin.py:18-420).
+
+Mirrors BSpecBin's interface: the bin and triangle bookkeeping (bspec_bin.py:34-176,
+186-214) is the reference's logic restated; the H maps (bspec_bin.py:177-184, 216-228,
+335-337) are batched spin-1 and spin-2 syntheses of all bins and fields at once on the
+device; the map products summed over pixels by mp.Pool (_analyze3 / _analyze1,
+bspec_bin.py:340-403) are one ps_bspec_triple_sums launch per (A, B, C) assignment.
+
+H maps: for a harmonic field h (nf, nlm) and bin b with mask 1_b(l),
+  (+1)H_b = sum_lm 1Y_lm 1_b conj(h)      (compute_spin_transform_map(., +1))
+  (-2)H_b = sum_lm -2Y_lm 1_b conj(h)     (compute_spin_transform_map(., -2))
+which with the HEALPix spin conventions of base.py:260-288 are the spin-1 / spin-2
+syntheses of (G, C) = (-1_b conj(h), 0), taking Q + iU and Q - iU respectively.  The
+transforms run at the largest binned l (the coefficients above it are zero).
+"""
+import numpy as np
+import torch
+
+from . import kernels as K
+from ._abi import check, load, ptr, stream_handle
+
+
+class BSpecBin:
+    """Binned bispectrum estimation (bspec_bin.py:18-33): base, mask, applySinv, l_bins, ..."""
+
+    def __init__(self, base, mask, applySinv, l_bins, l_bins_squeeze=[], include_partial_triangles=False,
+                 fields=['TTT', 'TTE', 'TEE', 'EEE', 'TBB', 'EBB', 'TTB', 'TEB', 'EEB', 'BBB'], parity='both'):
+        self.base = base
+        self.mask = mask
+        self.applySinv = applySinv
+        self.l_bins = np.asarray(l_bins)
+        self.pol = self.base.pol
+        self.parity = parity
+        self.fields = list(fields)
+        self.min_l = np.min(l_bins)
+        self.Nl = len(l_bins) - 1
+        if len(l_bins_squeeze) > 0:
+            self.l_bins_squeeze = np.asarray(l_bins_squeeze)
+            self.Nl_squeeze = len(l_bins_squeeze) - 1
+            for l_edge in self.l_bins:
+                assert l_edge in self.l_bins_squeeze, "Squeezed bins must contain all the unsqueezed bins!"
+        else:
+            self.l_bins_squeeze = self.l_bins.copy()
+            self.Nl_squeeze = self.Nl
+        self.include_partial_triangles = include_partial_triangles
+        self.beam = self.base.beam
+        self.beam_lm = self.base.beam_lm
+        for f in self.fields:
+            assert f in ['TTT', 'TTE', 'TEE', 'EEE', 'TBB', 'EBB', 'TTB', 'TEB', 'EEB', 'BBB'], \
+                "Unknown field '%s' supplied!" % f
+        assert len(self.fields) == len(np.unique(self.fields)), "Duplicate fields supplied!"
+        if not self.pol and self.fields != ['TTT']:
+            print("## Polarization mode not turned on; setting fields to TT only!")
+            self.fields = ['TTT']
+        assert parity in ['even', 'odd', 'both'], "Parity must be one of 'even', 'odd' or 'both'!"
+        if not self.pol and parity != 'even':
+            print("Caution: scalar 3-point functions can't probe parity-odd primordial physics!")
+        self.chi_arr = [1] if parity == 'even' else [-1] if parity == 'odd' else [1, -1]
+        if np.max(self.l_bins_squeeze) > base.lmax:
+            raise Exception("Maximum l is larger than HEALPix resolution!")
+        if np.max(self.l_bins) > base.lmax // 2:
+            print("## Caution: Maximum l is greater than HEALPix-lmax/2; this might cause boundary effects.")
+        print("Binning: %d bins in [%d, %d]" % (self.Nl, self.min_l, np.max(self.l_bins)))
+        if self.Nl_squeeze != self.Nl:
+            print("Squeezed binning: %d bins in [%d, %d]" % (self.Nl_squeeze, self.min_l, np.max(self.l_bins_squeeze)))
+        self.ones_mask = bool(np.std(self.mask) < 1e-12 and np.abs(np.mean(self.mask) - 1) < 1e-12)
+        print("Mask: ones" if self.ones_mask else "Mask: spatially varying")
+
+        # list of bins (bspec_bin.py:92-112)
+        self.all_bins, self.good_bins = [], []
+        for u in self.fields:
+            u1, u2, u3 = [self.base.indices[u[i]] for i in range(3)]
+            p_u = int(np.prod([self.base.parities[u[i]] for i in range(3)]))
+            for bin1 in range(self.Nl_squeeze):
+                for bin2 in range(self.Nl_squeeze):
+                    if u1 == u2 and bin2 < bin1:
+                        continue
+                    for bin3 in range(self.Nl_squeeze):
+                        if u2 == u3 and bin3 < bin2:
+                            continue
+                        if not self._check_bin(bin1, bin2, bin3):
+                            continue
+                        if min([bin1, bin2, bin3]) >= self.Nl:
+                            continue
+                        self.all_bins.append([u1, u2, u3, bin1, bin2, bin3, p_u])
+                        self.good_bins.append(np.asarray([bin1, bin2, bin3]))
+        self.all_bins = np.asarray(self.all_bins, dtype=np.int64).reshape(-1, 7)
+        gb = np.asarray(self.good_bins).reshape(-1, 3)
+        self.is_good = lambda bins: bool(np.any((gb[:, 0] == bins[0]) & (gb[:, 1] == bins[1]) & (gb[:, 2] == bins[2])))
+
+        # device set-up: H-map transforms at the largest binned l
+        self.nf = 1 + 2 * self.pol
+        self.lmax_b = int(np.max(self.l_bins_squeeze)) - 1
+        L = self.base.lmax
+        ls = self.base.l_arr
+        dev = self.base.device
+        self.ell_bins = [(ls >= self.l_bins_squeeze[b]) & (ls < self.l_bins_squeeze[b + 1])
+                         for b in range(self.Nl_squeeze)]
+        sel = np.concatenate([np.arange(m * (2 * L + 1 - m) // 2 + m, m * (2 * L + 1 - m) // 2 + self.lmax_b + 1)
+                              for m in range(self.lmax_b + 1)])
+        self._sel = torch.from_numpy(sel).to(dev)
+        lb = ls[sel]
+        binmask = np.stack([((lb >= self.l_bins_squeeze[b]) & (lb < self.l_bins_squeeze[b + 1])).astype(float)
+                            for b in range(self.Nl_squeeze)])
+        self._binmask = torch.from_numpy(binmask).to(dev, self.base.rdtype)  # (Nb, nlm_b)
+        # map-stack index of (bin, field) = bin * nf + field
+        ab = self.all_bins
+        trip = np.stack([ab[:, 3] * self.nf + ab[:, 0], ab[:, 4] * self.nf + ab[:, 1], ab[:, 5] * self.nf + ab[:, 2]], 1)
+        self._trip = torch.from_numpy(np.ascontiguousarray(trip.astype(np.int32))).to(dev)
+        self.nmap = self.Nl_squeeze * self.nf
+        assert self.nmap <= 128, "at most 128 (bin, field) maps per stack"
+        self._scratch = None
+
+    # ------------------------------------------------------ bookkeeping
+    def _check_bin(self, bin1, bin2, bin3):
+        """Triangle condition on all modes of the bin or on its centre (bspec_bin.py:126-147)."""
+        if self.include_partial_triangles:
+            l1s = np.arange(self.l_bins_squeeze[bin1], self.l_bins_squeeze[bin1 + 1])[:, None, None]
+            l2s = np.arange(self.l_bins_squeeze[bin2], self.l_bins_squeeze[bin2 + 1])[None, :, None]
+            l3s = np.arange(self.l_bins_squeeze[bin3], self.l_bins_squeeze[bin3 + 1])[None, None, :]
+            filt = (l3s >= abs(l1s - l2s)) * (l3s <= l1s + l2s)
+            return 1 if np.sum(filt) > 0 else 0
+        l1 = 0.5 * (self.l_bins_squeeze[bin1] + self.l_bins_squeeze[bin1 + 1])
+        l2 = 0.5 * (self.l_bins_squeeze[bin2] + self.l_bins_squeeze[bin2 + 1])
+        l3 = 0.5 * (self.l_bins_squeeze[bin3] + self.l_bins_squeeze[bin3 + 1])
+        return 0 if (l3 < abs(l1 - l2) or l3 > l1 + l2) else 1
+
+    def _compute_symmetry_factor(self):
+        """Degeneracy of each bin, stacked over parities (bspec_bin.py:149-175)."""
+        self.sym_factor_all = []
+        for chi in self.chi_arr:
+            sym = []
+            for u1, u2, u3, bin1, bin2, bin3, p_u in self.all_bins:
+                if bin1 == bin2 and u1 == u2 and bin2 == bin3 and u2 == u3:
+                    sym.append(6)
+                elif bin1 == bin2 and u1 == u2:
+                    sym.append(2)
+                elif bin2 == bin3 and u2 == u3:
+                    sym.append(2)
+                else:
+                    sym.append(1)
+            self.sym_factor_all.append(np.asarray(sym))
+        self.sym_factor = np.concatenate(self.sym_factor_all)
+        self.N_b = len(self.sym_factor)
+        print("Using a total of %d bins" % self.N_b)
+
+    def get_ells(self, field='TTT'):
+        """Central (l1, l2, l3) of each bin of one field (bspec_bin.py:186-214)."""
+        assert field in self.fields, "Specified field '%s' not in inputs!" % field
+        u1, u2, u3 = [self.base.indices[field[i]] for i in range(3)]
+        l1s, l2s, l3s = [], [], []
+        for bin1 in range(self.Nl_squeeze):
+            l1 = 0.5 * (self.l_bins_squeeze[bin1] + self.l_bins_squeeze[bin1 + 1])
+            for bin2 in range(self.Nl_squeeze):
+                if u1 == u2 and bin2 < bin1:
+                    continue
+                l2 = 0.5 * (self.l_bins_squeeze[bin2] + self.l_bins_squeeze[bin2 + 1])
+                for bin3 in range(self.Nl_squeeze):
+                    if u2 == u3 and bin3 < bin2:
+                        continue
+                    l3 = 0.5 * (self.l_bins_squeeze[bin3] + self.l_bins_squeeze[bin3 + 1])
+                    if not self.is_good([bin1, bin2, bin3]):
+                        continue
+                    if min([bin1, bin2, bin3]) >= self.Nl:
+                        continue
+                    l1s.append(l1)
+                    l2s.append(l2)
+                    l3s.append(l3)
+        return l1s, l2s, l3s
+
+    # ------------------------------------------------------------ H maps
+    def H_maps(self, h_lm):
+        """(+1)H and (-2)H maps of a harmonic field (nf, nlm(base lmax)) for every bin:
+        two (Nb * nf, Npix) complex stacks, row bin * nf + field."""
+        h = h_lm if isinstance(h_lm, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(h_lm))
+        h = h.to(self.base.device, self.base.cdtype).reshape(self.nf, -1)
+        hb = torch.conj(h.index_select(-1, self._sel))  # (nf, nlm_b)
+        Nb, nf = self.Nl_squeeze, self.nf
+        G = -(self._binmask[:, None, :] * hb[None, :, :]).reshape(Nb * nf, 1, -1)
+        GC = torch.cat([G, torch.zeros_like(G)], dim=1).contiguous()  # (Nb nf, 2, nlm_b)
+        QU1 = self.base.plan(self.lmax_b, 1).synthesis(GC)
+        QU2 = self.base.plan(self.lmax_b, 2).synthesis(GC)
+        self.base.n_SHTs_reverse += 2 * Nb * nf
+        p1 = torch.complex(QU1[:, 0], QU1[:, 1]).contiguous()
+        m2 = torch.complex(QU2[:, 0], -QU2[:, 1]).contiguous()
+        return p1, m2
+
+    def _process_sim(self, sim, input_type='map'):
+        """S^-1 then H maps of one simulation (bspec_bin.py:216-228)."""
+        return self.H_maps(self.applySinv(sim, input_type=input_type))
+
+    def _triple(self, A, B, C, scale, out, accumulate):
+        ntrip = self._trip.shape[0]
+        need = int(load().ps_bspec_scratch_bytes(ntrip))
+        if self._scratch is None or self._scratch.numel() < need:
+            self._scratch = torch.empty(need, dtype=torch.uint8, device=self.base.device)
+        prec = 32 if A.dtype == torch.complex64 else 64
+        check(load().ps_bspec_triple_sums(prec, ptr(A), ptr(B), ptr(C), self.nmap, A.shape[-1], ptr(self._trip),
+                                          ntrip, float(scale), ptr(self._scratch), need, ptr(out), int(accumulate),
+                                          stream_handle()))
+
+    def _assemble(self, S):
+        """Per parity: Re S, or Re(i S) = -Im S when p_u chi = -1 (bspec_bin.py:358-364)."""
+        S = S.cpu().numpy()
+        out = []
+        for chi in self.chi_arr:
+            odd = self.all_bins[:, 6] * chi == -1
+            out.append(np.where(odd, -S[:, 1], S[:, 0]))
+        return np.concatenate(out) / self.sym_factor
+
+    # ----------------------------------------------------- simulations
+    def load_sims(self, load_sim, N_sims, verb=False, input_type='map', preload=True):
+        """Monte-Carlo simulations for the linear term (bspec_bin.py:230-265)."""
+        self.N_it = N_sims
+        print("Using %d Monte Carlo simulations" % self.N_it)
+        self.preload = preload
+        if preload:
+            self.p1_H_maps, self.m2_H_maps = [], []
+            for ii in range(self.N_it):
+                if verb:
+                    print("Loading bias simulation %d of %d" % (ii + 1, self.N_it))
+                p1, m2 = self._process_sim(load_sim(ii), input_type=input_type)
+                self.p1_H_maps.append(p1)
+                self.m2_H_maps.append(m2)
+        else:
+            if verb:
+                print("No preloading; simulations will be loaded and accessed at runtime.")
+            self.load_sim_data = lambda ii: self._process_sim(load_sim(ii), input_type=input_type)
+
+    def generate_sims(self, N_sim, Cl_input=[], b_input=None, add_B=False, remove_mean=True, verb=False,
+                      preload=True):
+        """Gaussian simulations with the survey mask (bspec_bin.py:267-315; add_B is not provided)."""
+        if add_B or b_input is not None:
+            raise NotImplementedError("generate_sims(add_B=True) is not provided by the hip backend")
+        if len(Cl_input) == 0:
+            Cl_input = self.base.Cl_tot
+        mask_d = torch.as_tensor(np.asarray(self.mask)).to(self.base.device, self.base.rdtype)
+
+        def sim(ii):
+            if self.ones_mask:
+                return self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
+                                               remove_mean=remove_mean, output_type='harmonic'), 'harmonic'
+            m = self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
+                                        remove_mean=remove_mean)
+            m = torch.as_tensor(m).to(self.base.device, self.base.rdtype)
+            return (mask_d * m).contiguous(), 'map'
+
+        self.N_it = N_sim
+        print("Using %d Monte Carlo simulations" % self.N_it)
+        self.preload = preload
+        if preload:
+            self.p1_H_maps, self.m2_H_maps = [], []
+            for ii in range(self.N_it):
+                if verb:
+                    print("Generating bias simulation %d of %d" % (ii + 1, self.N_it))
+                s, kind = sim(ii)
+                p1, m2 = self._process_sim(s, input_type=kind)
+                self.p1_H_maps.append(p1)
+                self.m2_H_maps.append(m2)
+        else:
+            def load_sim_data(ii):
+                s, kind = sim(ii)
+                return self._process_sim(s, input_type=kind)
+            self.load_sim_data = load_sim_data
+
+    # -------------------------------------------------------- estimator
+    def Bl_numerator(self, data, include_linear_term=True, verb=False):
+        """Unwindowed binned-bispectrum numerator (bspec_bin.py:317-420): cubic term and, with
+        simulations, the linear term, ordered [parity][bin] as the reference's."""
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        if (not hasattr(self, 'preload')) and include_linear_term:
+            raise Exception("Need to generate or specify bias simulations!")
+        h = self.applySinv(data, input_type='map')
+        if verb:
+            print("Computing H maps")
+        p1, m2 = self.H_maps(h)
+        ntrip = self._trip.shape[0]
+        S3 = torch.zeros((ntrip, 2), dtype=torch.float64, device=self.base.device)
+        self._triple(p1, p1, m2, 1.0, S3, 0)
+        b3 = self.base.A_pix * self._assemble(S3) / 3.
+        b1 = np.zeros(self.N_b)
+        if not include_linear_term:
+            print("No linear correction applied!")
+        else:
+            S1 = torch.zeros_like(S3)
+            for ii in range(self.N_it):
+                if (ii + 1) % 5 == 0 and verb:
+                    print("Computing b_1 piece from simulation %d" % (ii + 1))
+                if self.preload:
+                    sp1, sm2 = self.p1_H_maps[ii], self.m2_H_maps[ii]
+                else:
+                    sp1, sm2 = self.load_sim_data(ii)
+                self._triple(p1, sp1, sm2, 1.0, S1, 1)
+                self._triple(sp1, p1, sm2, 1.0, S1, 1)
+                self._triple(sp1, sp1, m2, 1.0, S1, 1)
+            b1 = -self.base.A_pix * self._assemble(S1) / 3. / self.N_it
+        return b3 + b1

@@ -221,6 +221,9 @@ int ShtPlan::build() {
                 B = (ss > 0) ? s - m : s + m;
                 sg = (ss > 0) ? ((m & 1) ? -1.0 : 1.0) : ((s & 1) ? -1.0 : 1.0);
             }
+            // the -s recursion is carried as (-1)^s (-s)lambda: sY_{l,-m} = (-1)^(s+m) conj(-sY_lm),
+            // so the m < 0 half of a real spin field pairs +s with (-1)^s (-s)
+            if (g == 1 && (s & 1)) sg = -sg;
             seedL[(size_t)g * (mmax + 1) + m] = lv;
             seedA[(size_t)g * (mmax + 1) + m] = A;
             seedB[(size_t)g * (mmax + 1) + m] = B;
@@ -858,15 +861,16 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
     const RingInfo ri = d.rings[rn];
     const cplx<T> rot = expi_pi<T>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
     const int ncol = 2 * nmap;
-    // li parity 0 holds l + m even unless l_start + m is odd (m < spin)
-    const bool swp = ((lstart + m) & 1) != 0;
+    // li parity 0 holds l + m + s even unless l_start + m + s is odd; with the (-1)^s in the
+    // -s recursion, sY(pi - theta) = (-1)^(l+m) (-s)Y(theta) gives lam+- the south sign (-1)^(l+m+s)
+    const bool swp = ((lstart + m + d.spin) & 1) != 0;
 #pragma unroll
     for (int j = 0; j < BC; j++) {
         const int b = chunk * BC + j;
         if (b >= nmap) continue;
         cplx<T> R[2], I[2];  // north, south
-        // R = -(G lam+ + i C lam-), I = -(C lam+ - i G lam-); south: lam+ -> (-1)^(l+m),
-        // lam- -> -(-1)^(l+m)
+        // R = -(G lam+ + i C lam-), I = -(C lam+ - i G lam-); south: lam+ -> (-1)^(l+m+s),
+        // lam- -> -(-1)^(l+m+s)
         const cplx<T> GPe = swp ? acc1[j][0] : acc0[j][0], GPo = swp ? acc0[j][0] : acc1[j][0];
         const cplx<T> CPe = swp ? acc1[j][1] : acc0[j][1], CPo = swp ? acc0[j][1] : acc1[j][1];
         const cplx<T> GMe = swp ? acc1[j][2] : acc0[j][2], GMo = swp ? acc0[j][2] : acc1[j][2];
@@ -1398,7 +1402,7 @@ __global__ __launch_bounds__(64 * NW) void k_leg_adj_s0(ShtDev<T> d, int ncol, c
     }
 }
 
-// One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m.
+// One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m + s.
 template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
 __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp)[NRT], Rec (&rm)[NRT],
                                              const double (&x)[NRT], const cplx<T> (&in4)[NRT][BC][4],
@@ -1406,7 +1410,7 @@ __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp
 #pragma unroll
     for (int c = 0; c < LC; c++) {
         if (FULL || c < nvalid) {
-            const bool even = ((c + p0) & 1) == 0;  // l + m even
+            const bool even = ((c + p0) & 1) == 0;  // l + m + s even
             const double al = sco[3 * c], be = sco[3 * c + 1], ga = sco[3 * c + 2];
 #pragma unroll
             for (int q = 0; q < NRT; q++) {
@@ -1505,7 +1509,7 @@ __global__ __launch_bounds__(64 * NW) void k_leg_adj_sp(ShtDev<T> d, int nmap, c
         const int l = m + i / (BC * 2), b = chunk * BC + (i / 2) % BC, g = i % 2;
         if (l <= d.lmax && b < nmap) out[(size_t)(2 * b + g) * alm_stride + base + l] = cplx<T>{T(0), T(0)};
     }
-    const int p0 = (lstart + m) & 1;
+    const int p0 = (lstart + m + d.spin) & 1;
     double pal = 0.0, pbe = 0.0, pga = 0.0;
     auto fetch = [&](int l0n) {
         if (tid < min(LCH, d.lmax + 1 - l0n)) {

@@ -2,6 +2,8 @@
 
   * spin 0 and spin 2 synthesis / adjoint vs brute-force sums at every pixel
     (scipy sph_harm_y; Goldberg closed form) -- tests/golden/sht_bruteforce.npz;
+  * odd spins (1, 3) vs the Goldberg brute force, and spin 1 vs the gradient of
+    a spin-0 map (the HEALPix spin-1 convention);
   * the Goldberg spin-weighted harmonics used there vs spin-raising (eth)
     derivatives of scipy's Y_lm (pins the spin sign convention independently);
   * the Legendre recursion with exponent rescaling vs 60-digit mpmath values at
@@ -48,6 +50,57 @@ def test_spin2_vs_bruteforce(nside):
     adj = oracle.adjoint(np.stack([G[f"s2_n{nside}_Rin"], G[f"s2_n{nside}_Iin"]])[None], nside, lmax, 2)[0]
     assert rel(adj[0], G[f"s2_n{nside}_Gadj"]) < 1e-12
     assert rel(adj[1], G[f"s2_n{nside}_Cadj"]) < 1e-12
+
+
+def _brute_spin(G_, C_, s, lmax, th, ph):
+    """R + iI = -sum_{l, all m} (G + iC)_lm sY_lm with X_{l,-m} = (-1)^m conj(X_lm)."""
+    ls, ms = oracle.getlm(lmax)
+    out = np.zeros(len(th), complex)
+    for i, (l, m) in enumerate(zip(ls, ms)):
+        w = G_[i] + 1j * C_[i]
+        out -= w * mg.goldberg_sylm(s, l, m, th, ph)
+        if m > 0:
+            wn = (-1) ** m * (np.conj(G_[i]) + 1j * np.conj(C_[i]))
+            out -= wn * mg.goldberg_sylm(s, l, -m, th, ph)
+    return out
+
+
+@pytest.mark.parametrize("spin", [1, 3])
+def test_odd_spin_vs_bruteforce(spin):
+    """Odd spins pair +s with (-1)^s (-s) in the m < 0 half (sY_{l,-m} = (-1)^(s+m) conj(-sY_lm)),
+    which even-spin tests cannot see; the reference's to_lm_spin carries the same (-1)^s
+    (polyspec/base.py:255-256)."""
+    nside, lmax = 4, 11
+    th, ph = mg.healpix_centres(nside)
+    a = mg.random_alm(np.random.default_rng(10 + spin), lmax, 2)
+    ls, _ = oracle.getlm(lmax)
+    a[:, ls < spin] = 0
+    RI = oracle.synthesis(a[None], nside, lmax, spin)[0]
+    want = _brute_spin(a[0], a[1], spin, lmax, th, ph)
+    assert rel(RI[0] + 1j * RI[1], want) < 1e-12
+
+
+def test_spin1_synthesis_is_the_gradient():
+    """HEALPix spin-1 convention: G = sqrt(l(l+1)) a_lm, C = 0 synthesises
+    (d_theta T, d_phi T / sin theta) of T = sum a_lm Y_lm (finite differences of scipy's Y_lm)."""
+    nside, lmax = 4, 9
+    th, ph = mg.healpix_centres(nside)
+    a = mg.random_alm(np.random.default_rng(3), lmax, 1)[0]
+    ls, ms = oracle.getlm(lmax)
+
+    def T(t, p):
+        out = np.zeros(len(t))
+        for i, (l, m) in enumerate(zip(ls, ms)):
+            y = sph_harm_y(l, m, t, p)
+            out += (a[i] * y).real * (2 if m else 1)
+        return out
+    h = 1e-5
+    dth = (T(th + h, ph) - T(th - h, ph)) / (2 * h)
+    dph = (T(th, ph + h) - T(th, ph - h)) / (2 * h) / np.sin(th)
+    GC = np.stack([a * np.sqrt(ls * (ls + 1.)), np.zeros_like(a)])[None]
+    RI = oracle.synthesis(GC, nside, lmax, 1)[0]
+    assert rel(RI[0], dth) < 1e-8
+    assert rel(RI[1], dph) < 1e-8
 
 
 def _eth(f, s, th, ph, h=1e-5):

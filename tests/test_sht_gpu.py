@@ -54,25 +54,27 @@ def test_spin0_adjoint_matches_oracle(gpu, nside, lmax, prec, nmaps):
     assert relerr(out, ref) < tol
 
 
-@pytest.mark.parametrize("nside,lmax,prec,nmaps", [
-    (2, 5, 64, 1), (8, 23, 64, 3), (64, 191, 64, 1), (8, 23, 32, 3), (64, 128, 32, 2), (128, 383, 32, 1)])
-def test_spin2_synthesis_and_adjoint_match_oracle(gpu, nside, lmax, prec, nmaps):
+@pytest.mark.parametrize("nside,lmax,prec,nmaps,spin", [
+    (2, 5, 64, 1, 2), (8, 23, 64, 3, 2), (64, 191, 64, 1, 2), (8, 23, 32, 3, 2), (64, 128, 32, 2, 2),
+    (128, 383, 32, 1, 2), (2, 5, 64, 1, 1), (8, 23, 64, 3, 1), (64, 191, 64, 2, 1), (64, 128, 32, 9, 1),
+    (128, 255, 32, 1, 1), (8, 23, 64, 2, 3)])
+def test_spin_synthesis_and_adjoint_match_oracle(gpu, nside, lmax, prec, nmaps, spin):
     from polyspec_amd._abi import ShtPlan
-    print(f"spin2 nside={nside} lmax={lmax} prec={prec} nmaps={nmaps}", flush=True)
-    rng = np.random.default_rng(3000 + nside + lmax + nmaps)
-    a = rand_alm(rng, lmax, (nmaps, 2), lmin=2)
-    ref = oracle.synthesis(a, nside, lmax, 2)
-    plan = ShtPlan(nside, lmax, 2, prec)
+    print(f"spin{spin} nside={nside} lmax={lmax} prec={prec} nmaps={nmaps}", flush=True)
+    rng = np.random.default_rng(3000 + nside + lmax + nmaps + 100 * spin)
+    a = rand_alm(rng, lmax, (nmaps, 2), lmin=spin)
+    ref = oracle.synthesis(a, nside, lmax, spin)
+    plan = ShtPlan(nside, lmax, spin, prec)
     out = plan.synthesis(torch.from_numpy(a).to(gpu, plan.cdtype)).cpu().numpy()
     tol = 1e-11 if prec == 64 else 2e-6
     assert relerr(out, ref) < tol
     m = rng.standard_normal((nmaps, 2, 12 * nside * nside))
-    ref = oracle.adjoint(m, nside, lmax, 2)
+    ref = oracle.adjoint(m, nside, lmax, spin)
     out = plan.adjoint(torch.from_numpy(m).to(gpu, plan.rdtype), 1.0).cpu().numpy()
     assert relerr(out, ref) < tol
 
 
-@pytest.mark.parametrize("nside,lmax,spin", [(512, 1024, 0), (512, 1535, 2), (1024, 2048, 0)])
+@pytest.mark.parametrize("nside,lmax,spin", [(512, 1024, 0), (512, 1535, 2), (256, 767, 1), (1024, 2048, 0)])
 def test_adjointness_full_size(gpu, nside, lmax, spin):
     """<Y a, m> = <a, Y^T m>_w with w_m = 1 + [m > 0]: holds at any size (FP64 kernels)."""
     from polyspec_amd._abi import ShtPlan
