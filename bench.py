@@ -227,6 +227,93 @@ def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=20
     return res
 
 
+def log_bins(lmax, nbins):
+    """nbins log-spaced l-bin edges on [2, lmax + 1], strictly increasing integers."""
+    e = np.geomspace(2, lmax + 1, nbins + 1)
+    out = [2]
+    for x in e[1:]:
+        out.append(max(int(round(x)), out[-1] + 1))
+    out[-1] = lmax + 1
+    return np.asarray(out)
+
+
+def bench_bspec(nside, nsims, nbins=20, plain=True):
+    """Binned bispectrum numerator (TTT/TTE/TEE/EEE, both parities, nbins log-spaced l-bins up to
+    lmax = min(2 nside, 2000)) with nsims Monte-Carlo simulations for the linear term.
+
+    batched FP32: every (bin, field) H map of a field in one spin-1 + one spin-2 launch, H maps of
+    all simulations preloaded in HBM, then Bl_numerator (cubic + linear term).  plain FP64: one
+    transform per (bin, field) and simulations streamed (preload=False), the reference's order.
+    Reported: simulations/s of both (H maps + triple sums per simulation), the triple-sum kernel's
+    rate, and the largest FP32-vs-FP64 difference relative to the largest numerator entry."""
+    import torch
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.bspec_bin import BSpecBin
+    from polyspec_amd.weighting import Weightings
+    lmax = min(2 * nside, 2000)
+    edges = log_bins(lmax, nbins)
+    raw, tot, beam = spectra(nside)
+    mask = synthetic_mask(nside)
+    fields = ['TTT', 'TTE', 'TEE', 'EEE']
+    out = {"nside": nside, "lmax": lmax, "n_bins": len(edges) - 1, "fields": fields, "parity": "both",
+           "n_sims": nsims}
+
+    def setup(prec, batch, gen=True):
+        base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=prec)
+        base.grf_device = True
+        W = Weightings(base, mask, dict(raw))
+        bs = BSpecBin(base, mask, W.applySinv_ideal, edges, fields=fields, parity='both', batch_maps=batch)
+        mask_d = torch.from_numpy(mask).to(base.device, base.rdtype)
+        maps = [(base.generate_data(seed=int(9e6) + i) * mask_d).contiguous() for i in range(1 + nsims)] if gen else None
+        return base, bs, maps
+
+    base, bs, maps = setup(32, True)
+    out["n_triplets"] = int(bs._trip.shape[0])
+    bs.load_sims(lambda i: maps[1 + i], 1, preload=True)  # warm-up: plans, workspaces
+    bs.Bl_numerator(maps[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.load_sims(lambda i: maps[1 + i], nsims, preload=True)
+    torch.cuda.synchronize()
+    t_h = time.perf_counter() - t0
+    bs.kernel_events = []
+    t0 = time.perf_counter()
+    b32 = bs.Bl_numerator(maps[0])
+    torch.cuda.synchronize()
+    t_num = time.perf_counter() - t0
+    ev = bs.kernel_events
+    bs.kernel_events = None
+    tri_ms = float(np.mean([e0.elapsed_time(e1) for _, e0, e1 in ev]))
+    npix = 12 * nside * nside
+    flops = 42. * out["n_triplets"] * npix  # 3 terms x (2 complex products + 1 complex add) per pixel
+    out["batched_fp32"] = {"sims_per_s": nsims / (t_h + t_num), "s_H_maps": t_h, "s_numerator": t_num,
+                           "triple_sum_ms": tri_ms, "triple_sum_tflops": flops / (tri_ms * 1e-3) / 1e12,
+                           "triple_sum_launches": len(ev)}
+    log("bspec nside %d: batched fp32 %.2f sims/s (H maps %.2fs, numerator %.2fs, triple sum %.3f ms = %.1f TFLOP/s)"
+        % (nside, nsims / (t_h + t_num), t_h, t_num, tri_ms, flops / (tri_ms * 1e-3) / 1e12))
+    data32 = maps[0].double()
+    sims64 = [m.double() for m in maps[1:]]
+    del bs, maps
+    torch.cuda.empty_cache()
+    if plain:
+        base64, bs64, _ = setup(64, False, gen=False)
+        bs64.load_sims(lambda i: sims64[i], nsims, preload=False)
+        bs64.Bl_numerator(data32)  # warm-up (plans)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b64 = bs64.Bl_numerator(data32)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        out["plain_fp64"] = {"sims_per_s": nsims / t_plain, "s_numerator": t_plain}
+        out["speedup_batched_over_plain"] = (nsims / (t_h + t_num)) / (nsims / t_plain)
+        out["max_abs_diff_over_max_abs"] = float(np.abs(b32 - b64).max() / np.abs(b64).max())
+        log("bspec nside %d: plain fp64 %.2f sims/s; speedup %.1fx; max|diff|/max|B| %.2e"
+            % (nside, nsims / t_plain, out["speedup_batched_over_plain"], out["max_abs_diff_over_max_abs"]))
+        del bs64, base64
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(data_np, base, tm, nr_sample, lmax):
     """FP64 oracle on the host: S^-1 of the data plus `nr_sample` of the 40 radial
     nodes (2 syntheses each + gnl sums), extrapolated to one numerator."""
@@ -276,6 +363,10 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG S^-1 section")
     ap.add_argument("--learned-pairs", type=int, default=2000,
                     help="training pairs for the learned CG preconditioner (0 = plain CG only)")
+    ap.add_argument("--no-bspec", action="store_true", help="skip the binned-bispectrum section")
+    ap.add_argument("--bspec-nside", type=str, default="256",
+                    help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
+    ap.add_argument("--bspec-sims", type=int, default=16)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -391,6 +482,11 @@ def main():
     if not args.no_cg:
         log("CG S^-1: plain harmonic vs learned preconditioner ...")
         result["cg"] = bench_cg(nside, mask, learned_pairs=args.learned_pairs, nside_lo=min(128, nside))
+
+    if not args.no_bspec:
+        log("binned bispectrum ...")
+        result["bspec"] = [bench_bspec(int(n), args.bspec_sims, plain=not args.no_plain)
+                           for n in args.bspec_nside.split(",")]
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")

@@ -24,8 +24,14 @@ class BSpecBin:
     """Binned bispectrum estimation (bspec_bin.py:18-33): base, mask, applySinv, l_bins, ..."""
 
     def __init__(self, base, mask, applySinv, l_bins, l_bins_squeeze=[], include_partial_triangles=False,
-                 fields=['TTT', 'TTE', 'TEE', 'EEE', 'TBB', 'EBB', 'TTB', 'TEB', 'EEB', 'BBB'], parity='both'):
+                 fields=['TTT', 'TTE', 'TEE', 'EEE', 'TBB', 'EBB', 'TTB', 'TEB', 'EEB', 'BBB'], parity='both',
+                 batch_maps=True):
+        """batch_maps=True: every (bin, field) H map of a field in one spin-1 and one spin-2
+        synthesis launch.  batch_maps=False: one transform per (bin, field), as the reference's
+        compute_spin_transform_map loop (bspec_bin.py:336-337) issues them."""
         self.base = base
+        self.batch_maps = batch_maps
+        self.kernel_events = None  # set to a list to time every triple-sum launch
         self.mask = mask
         self.applySinv = applySinv
         self.l_bins = np.asarray(l_bins)
@@ -178,8 +184,16 @@ class BSpecBin:
         Nb, nf = self.Nl_squeeze, self.nf
         G = -(self._binmask[:, None, :] * hb[None, :, :]).reshape(Nb * nf, 1, -1)
         GC = torch.cat([G, torch.zeros_like(G)], dim=1).contiguous()  # (Nb nf, 2, nlm_b)
-        QU1 = self.base.plan(self.lmax_b, 1).synthesis(GC)
-        QU2 = self.base.plan(self.lmax_b, 2).synthesis(GC)
+        P1, P2 = self.base.plan(self.lmax_b, 1), self.base.plan(self.lmax_b, 2)
+        if self.batch_maps:
+            QU1 = P1.synthesis(GC)
+            QU2 = P2.synthesis(GC)
+        else:
+            QU1 = torch.empty((Nb * nf, 2, P1.npix), dtype=self.base.rdtype, device=GC.device)
+            QU2 = torch.empty_like(QU1)
+            for k in range(Nb * nf):
+                P1.synthesis(GC[k:k + 1], out=QU1[k:k + 1])
+                P2.synthesis(GC[k:k + 1], out=QU2[k:k + 1])
         self.base.n_SHTs_reverse += 2 * Nb * nf
         p1 = torch.complex(QU1[:, 0], QU1[:, 1]).contiguous()
         m2 = torch.complex(QU2[:, 0], -QU2[:, 1]).contiguous()
@@ -195,9 +209,15 @@ class BSpecBin:
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(need, dtype=torch.uint8, device=self.base.device)
         prec = 32 if A.dtype == torch.complex64 else 64
+        if self.kernel_events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         check(load().ps_bspec_triple_sums(prec, ptr(A), ptr(B), ptr(C), self.nmap, A.shape[-1], ptr(self._trip),
                                           ntrip, float(scale), ptr(self._scratch), need, ptr(out), int(accumulate),
                                           stream_handle()))
+        if self.kernel_events is not None:
+            e1.record()
+            self.kernel_events.append((A.shape[-1], e0, e1))
 
     def _assemble(self, S):
         """Per parity: Re S, or Re(i S) = -Im S when p_u chi = -1 (bspec_bin.py:358-364)."""
