@@ -720,6 +720,170 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
     }
 }
 
+// ------------------------------------- Legendre synthesis, spin s, FP32 matrix cores
+// The tiling of the spin-0 kernel with two recursions per ring (+s and (-1)^s (-s)):
+// A = lam+ = (f+ + f-)/2 or lam- = (f+ - f-)/2, B = the alm rows of 8 maps (16 complex
+// columns, map b and comp c (G = 0, C = 1) at 2 b + c, i.e. real column j = 4 b + 2 c + part).
+// Four accumulators: lam+ / lam- times even / odd l + m + s.  With q = j % 4 the output at
+// lane j of map b is
+//   q = 0: Re R = -(G_re lam+ - C_im lam-)     q = 1: Im R = -(G_im lam+ + C_re lam-)
+//   q = 2: Re I = -(C_re lam+ + G_im lam-)     q = 3: Im I = -(C_im lam+ - G_re lam-)
+// i.e. -(D+[j] - D-[j ^ 3]) for q = 0, 3 and -(D+[j] + D-[j ^ 3]) for q = 1, 2; the south
+// ring takes lam+ -> (-1)^(l+m+s) lam+, lam- -> -(-1)^(l+m+s) lam-.  R and I land in the
+// complex columns 2 b and 2 b + 1 of the phase buffer, the layout of the FP64 / few-map kernel.
+__global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int ncol, int ld,
+                                                           const cplx<float>* __restrict__ alm_t,
+                                                           cplx<float>* __restrict__ phase) {
+    constexpr int BCM = 16;
+    __shared__ __align__(16) float sa[LCH * 2 * BCM];
+    __shared__ double sco[LCH * 3];
+    __shared__ cplx<float> rot[128];
+    const int nrb = (d.npairs + 127) / 128;
+    const int nchunk = (ncol + BCM - 1) / BCM;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = lb % nrb;
+    const int chunk = (lb / nrb) % nchunk;
+    const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int jp = rb * 128 + wave * 32 + li32;
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    const int M1 = d.mmax + 1;
+    if (tid < 128) {
+        const int jr = min(rb * 128 + tid, d.npairs - 1);
+        const RingInfo ri = d.rings[jr];
+        rot[tid] = expi_pi<float>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    }
+
+    const double x = d.cth[jpc];
+    Rec rp, rm;
+    rec_seed(rp, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
+    rec_seed(rm, d.seedL[M1 + m] + d.seedA[M1 + m] * d.l2sh[jpc] + d.seedB[M1 + m] * d.l2ch[jpc],
+             d.seedS[M1 + m]);
+    v16f Pev = {0}, Pod = {0}, Mev = {0}, Mod = {0};
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const float* a = reinterpret_cast<const float*>(alm_t + (size_t)chunk * BCM);
+    const double* cf4 = d.coef + 4 * base;
+    const int lstart = m > d.spin ? m : d.spin;
+    // chunk staging as in the spin-0 kernel; columns at or past ld (the last chunk of a
+    // batch whose ld is not a multiple of 16) read as zero
+    static_assert(LCH * 2 * BCM == 4 * 256, "one float4 per thread per chunk");
+    const bool col_ok = chunk * BCM + 2 * (tid & 7) < ld;
+    float4 pa;
+    double pal = 0.0, pbe = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        const int nn = min(LCH, d.lmax + 1 - l0n);
+        const int li = tid >> 3;
+        pa = (li < nn && col_ok)
+                 ? *reinterpret_cast<const float4*>(a + (size_t)(base + l0n + li) * 2 * ld + 4 * (tid & 7))
+                 : float4{0.f, 0.f, 0.f, 0.f};
+        if (tid < nn) {
+            pal = cf4[4 * (l0n + tid)];
+            pbe = cf4[4 * (l0n + tid) + 1];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    fetch(lstart);
+    bool fast = false;
+    for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
+        const int n = min(LCH, d.lmax + 1 - l0);
+        __syncthreads();
+        *reinterpret_cast<float4*>(sa + 4 * tid) = pa;
+        if (tid < n) {
+            sco[3 * tid] = pal;
+            sco[3 * tid + 1] = pbe;
+            sco[3 * tid + 2] = pga;
+        }
+        __syncthreads();
+        if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
+        if (!fast) fast = __all(rp.k == 0 && rm.k == 0);
+        if (fast && n == LCH) {
+            // every scale exponent is zero and the chunk is full: straight-line groups
+#pragma unroll 2
+            for (int g = 0; g < LCH; g += 4) {
+                double vp[4], vm[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    vp[c] = 0.5 * (rp.f + rm.f);
+                    vm[c] = 0.5 * (rp.f - rm.f);
+                    const double al = sco[3 * (g + c)], be = sco[3 * (g + c) + 1], ga = sco[3 * (g + c) + 2];
+                    rec_fast(rp, al * (x + be), ga);
+                    rec_fast(rm, al * (x - be), ga);
+                }
+                const float bev = sa[(g + 2 * h) * 2 * BCM + li32];
+                const float bod = sa[(g + 1 + 2 * h) * 2 * BCM + li32];
+                Pev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[2] : vp[0]), bev, Pev, 0, 0, 0);
+                Pod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[3] : vp[1]), bod, Pod, 0, 0, 0);
+                Mev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[2] : vm[0]), bev, Mev, 0, 0, 0);
+                Mod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[3] : vm[1]), bod, Mod, 0, 0, 0);
+            }
+            continue;
+        }
+        for (int g = 0; g < n; g += 4) {
+            // recursion values at l0+g .. l0+g+3 (zero past the end of the chunk)
+            double vp[4], vm[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int li = g + c;
+                if (li < n) {
+                    const double fp = fast ? rp.f : rp.f * rp.cf, fm = fast ? rm.f : rm.f * rm.cf;
+                    vp[c] = 0.5 * (fp + fm);
+                    vm[c] = 0.5 * (fp - fm);
+                    const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                    if (fast) {
+                        rec_fast(rp, al * (x + be), ga);
+                        rec_fast(rm, al * (x - be), ga);
+                    } else {
+                        rec_step(rp, al * (x + be), ga);
+                        rec_step(rm, al * (x - be), ga);
+                    }
+                } else {
+                    vp[c] = vm[c] = 0.0;
+                }
+            }
+            if (!fast) {
+                rec_rescale(rp);
+                rec_rescale(rm);
+                // all values below FP32 range
+                if (!__any((rp.k >= scale_consts<float>::KMIN - 1) || (rm.k >= scale_consts<float>::KMIN - 1)))
+                    continue;
+            }
+            const int le = g + 2 * h, lo = g + 1 + 2 * h;
+            const float bev = le < n ? sa[le * 2 * BCM + li32] : 0.f;
+            const float bod = lo < n ? sa[lo * 2 * BCM + li32] : 0.f;
+            Pev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[2] : vp[0]), bev, Pev, 0, 0, 0);
+            Pod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[3] : vp[1]), bod, Pod, 0, 0, 0);
+            Mev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[2] : vm[0]), bev, Mev, 0, 0, 0);
+            Mod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[3] : vm[1]), bod, Mod, 0, 0, 0);
+        }
+    }
+    // ev accumulators hold l - l_start even, i.e. l + m + s even unless l_start + m + s is odd
+    const bool swp = ((lstart + m + d.spin) & 1) != 0;
+    const int col = chunk * BCM + (li32 >> 1);
+    const bool re = (li32 & 1) == 0;
+    const int q = li32 & 3;
+    const float sg = (q == 0 || q == 3) ? -1.f : 1.f;
+#pragma unroll
+    for (int vv = 0; vv < 16; vv++) {
+        const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+        const int jr = rb * 128 + wave * 32 + i;
+        const float pe = swp ? Pod[vv] : Pev[vv], po = swp ? Pev[vv] : Pod[vv];
+        const float me = swp ? Mod[vv] : Mev[vv], mo = swp ? Mev[vv] : Mod[vv];
+        const float mn3 = __shfl_xor(me + mo, 3), ms3 = __shfl_xor(mo - me, 3);
+        const float nv = -((pe + po) + sg * mn3), sv = -((pe - po) + sg * ms3);
+        const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
+        const cplx<float> w = rot[wave * 32 + i];
+        const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+        const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+        if (jr < d.npairs && col < ncol) {
+            const int rn = jr, rs = d.nrings - 1 - jr;
+            reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+            if (rs != rn) reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+        }
+    }
+}
+
 // -------------------------------------------------- Legendre synthesis, spin s
 // alm_t: [nlm][ld] complex with columns (map b, comp c) at 2 b + c (G, C).
 template <typename T, int BC>
@@ -1815,9 +1979,21 @@ static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* a
     return launch_synth_s0<T, BCMAX>(p, ncol, ld, alm_t, phase, st);
 }
 
+static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
+                                hipStream_t st) {
+    ShtDev<float> d = dev_view<float>(p);
+    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 15) / 16;
+    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    hipLaunchKernelGGL(k_leg_synth_sp_mfma, dim3(nb), dim3(256), 0, st, d, ncol, ld, (const cplx<float>*)alm_t,
+                       (cplx<float>*)phase);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 template <typename T>
 static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* alm_t, void* phase, hipStream_t st) {
     constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
+    if (sizeof(T) == 4 && nmap > 2) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
     if (nmap <= 1) return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
     if (BCMAX > 2 && nmap <= 2) return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
     return launch_synth_sp<T, BCMAX>(p, nmap, ld, alm_t, phase, st);

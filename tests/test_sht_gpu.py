@@ -57,7 +57,7 @@ def test_spin0_adjoint_matches_oracle(gpu, nside, lmax, prec, nmaps):
 @pytest.mark.parametrize("nside,lmax,prec,nmaps,spin", [
     (2, 5, 64, 1, 2), (8, 23, 64, 3, 2), (64, 191, 64, 1, 2), (8, 23, 32, 3, 2), (64, 128, 32, 2, 2),
     (128, 383, 32, 1, 2), (2, 5, 64, 1, 1), (8, 23, 64, 3, 1), (64, 191, 64, 2, 1), (64, 128, 32, 9, 1),
-    (128, 255, 32, 1, 1), (8, 23, 64, 2, 3)])
+    (128, 255, 32, 1, 1), (8, 23, 64, 2, 3), (128, 255, 32, 21, 2), (64, 128, 32, 5, 3), (256, 512, 32, 30, 1)])
 def test_spin_synthesis_and_adjoint_match_oracle(gpu, nside, lmax, prec, nmaps, spin):
     from polyspec_amd._abi import ShtPlan
     print(f"spin{spin} nside={nside} lmax={lmax} prec={prec} nmaps={nmaps}", flush=True)
