@@ -283,14 +283,18 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
     t_num = time.perf_counter() - t0
     ev = bs.kernel_events
     bs.kernel_events = None
-    tri_ms = float(np.mean([e0.elapsed_time(e1) for _, e0, e1 in ev]))
+    cub_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in ev if n == 3]))
+    lin_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in ev if n == 4]))
     npix = 12 * nside * nside
-    flops = 42. * out["n_triplets"] * npix  # 3 terms x (2 complex products + 1 complex add) per pixel
+    # algorithmic flops per (triplet, pixel): cubic 3 x (complex product + complex multiply-add) = 42;
+    # linear 3 x (2 complex products + 3 complex multiply-adds) = 108
+    cub_tf = 42. * out["n_triplets"] * npix / (cub_ms * 1e-3) / 1e12
+    lin_tf = 108. * out["n_triplets"] * npix / (lin_ms * 1e-3) / 1e12
     out["batched_fp32"] = {"sims_per_s": nsims / (t_h + t_num), "s_H_maps": t_h, "s_numerator": t_num,
-                           "triple_sum_ms": tri_ms, "triple_sum_tflops": flops / (tri_ms * 1e-3) / 1e12,
-                           "triple_sum_launches": len(ev)}
-    log("bspec nside %d: batched fp32 %.2f sims/s (H maps %.2fs, numerator %.2fs, triple sum %.3f ms = %.1f TFLOP/s)"
-        % (nside, nsims / (t_h + t_num), t_h, t_num, tri_ms, flops / (tri_ms * 1e-3) / 1e12))
+                           "cubic_sum_ms": cub_ms, "cubic_sum_tflops": cub_tf,
+                           "linear_sum_ms": lin_ms, "linear_sum_tflops": lin_tf}
+    log("bspec nside %d: batched fp32 %.2f sims/s (H maps %.2fs, numerator %.2fs; cubic sum %.3f ms = %.1f TFLOP/s, "
+        "linear sum %.3f ms = %.1f TFLOP/s)" % (nside, nsims / (t_h + t_num), t_h, t_num, cub_ms, cub_tf, lin_ms, lin_tf))
     data32 = maps[0].double()
     sims64 = [m.double() for m in maps[1:]]
     del bs, maps

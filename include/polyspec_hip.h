@@ -183,11 +183,21 @@ int ps_cg_update_p(int precision, long long n, const double *num, const double *
  *   out[t] (+)= scale * sum_p [A_i1 B_i2 C_i3 + A_i2 B_i3 C_i1 + A_i3 B_i1 C_i2](p)
  * for triplets t = (i1, i2, i3) (int32, ntrip x 3) of map indices into the complex
  * stacks A, B, C (nmap x npix, nmap <= 128, FP32 or FP64 complex); out is complex FP64.
+ * A == B (the cubic term, bspec_bin.py:339-364) runs a two-stack pass.
  * scratch: ps_bspec_scratch_bytes(ntrip) bytes. */
 long long ps_bspec_scratch_bytes(int ntrip);
 int ps_bspec_triple_sums(int precision, const void *A, const void *B, const void *C, int nmap, long long npix,
                          const int *triplets, int ntrip, double scale, void *scratch, long long scratch_bytes,
                          void *out, int accumulate, void *stream);
+/* The linear term of one simulation (bspec_bin.py:366-403: the three _analyze1 calls with
+ * (data, sim, sim), (sim, data, sim) and (sim, sim, data) H maps) in one pass:
+ *   out[t] (+)= scale * sum_p sum_cyc [(P_a Q_b + Q_a P_b) N_c + Q_a Q_b M_c](p)
+ * over the cyclic orders (a, b, c) of (i1, i2, i3), with P / M the data's (+1)H / (-2)H
+ * stacks and Q / N the simulation's; equal to ps_bspec_triple_sums on (P, Q, N), (Q, P, N)
+ * and (Q, Q, M) added.  Same layouts and scratch as ps_bspec_triple_sums. */
+int ps_bspec_linear_sums(int precision, const void *P, const void *Q, const void *M, const void *N, int nmap,
+                         long long npix, const int *triplets, int ntrip, double scale, void *scratch,
+                         long long scratch_bytes, void *out, int accumulate, void *stream);
 
 /* -------------------------------------------- learned CG preconditioner
  * Kernels of a 9-point-stencil convolutional network on HEALPix maps (FP32),

@@ -4,7 +4,8 @@ Mirrors BSpecBin's interface: the bin and triangle bookkeeping (bspec_bin.py:34-
 186-214) is the reference's logic restated; the H maps (bspec_bin.py:177-184, 216-228,
 335-337) are batched spin-1 and spin-2 syntheses of all bins and fields at once on the
 device; the map products summed over pixels by mp.Pool (_analyze3 / _analyze1,
-bspec_bin.py:340-403) are one ps_bspec_triple_sums launch per (A, B, C) assignment.
+bspec_bin.py:340-403) are one ps_bspec_triple_sums launch for the cubic term and one
+ps_bspec_linear_sums launch (the three (data, sim) assignments together) per simulation.
 
 H maps: for a harmonic field h (nf, nlm) and bin b with mask 1_b(l),
   (+1)H_b = sum_lm 1Y_lm 1_b conj(h)      (compute_spin_transform_map(., +1))
@@ -203,21 +204,25 @@ class BSpecBin:
         """S^-1 then H maps of one simulation (bspec_bin.py:216-228)."""
         return self.H_maps(self.applySinv(sim, input_type=input_type))
 
-    def _triple(self, A, B, C, scale, out, accumulate):
+    def _sums(self, stacks, scale, out, accumulate):
+        """ps_bspec_triple_sums on (A, B, C) = stacks, or ps_bspec_linear_sums on (P, Q, M, N)."""
         ntrip = self._trip.shape[0]
         need = int(load().ps_bspec_scratch_bytes(ntrip))
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(need, dtype=torch.uint8, device=self.base.device)
+        A = stacks[0]
+        for X in stacks:
+            assert X.dtype == A.dtype and X.shape == (self.nmap, A.shape[-1]) and X.is_contiguous()
         prec = 32 if A.dtype == torch.complex64 else 64
         if self.kernel_events is not None:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        check(load().ps_bspec_triple_sums(prec, ptr(A), ptr(B), ptr(C), self.nmap, A.shape[-1], ptr(self._trip),
-                                          ntrip, float(scale), ptr(self._scratch), need, ptr(out), int(accumulate),
-                                          stream_handle()))
+        fn = load().ps_bspec_triple_sums if len(stacks) == 3 else load().ps_bspec_linear_sums
+        check(fn(prec, *[ptr(X) for X in stacks], self.nmap, A.shape[-1], ptr(self._trip), ntrip, float(scale),
+                 ptr(self._scratch), need, ptr(out), int(accumulate), stream_handle()))
         if self.kernel_events is not None:
             e1.record()
-            self.kernel_events.append((A.shape[-1], e0, e1))
+            self.kernel_events.append((len(stacks), e0, e1))
 
     def _assemble(self, S):
         """Per parity: Re S, or Re(i S) = -Im S when p_u chi = -1 (bspec_bin.py:358-364)."""
@@ -297,7 +302,7 @@ class BSpecBin:
         p1, m2 = self.H_maps(h)
         ntrip = self._trip.shape[0]
         S3 = torch.zeros((ntrip, 2), dtype=torch.float64, device=self.base.device)
-        self._triple(p1, p1, m2, 1.0, S3, 0)
+        self._sums((p1, p1, m2), 1.0, S3, 0)
         b3 = self.base.A_pix * self._assemble(S3) / 3.
         b1 = np.zeros(self.N_b)
         if not include_linear_term:
@@ -311,8 +316,7 @@ class BSpecBin:
                     sp1, sm2 = self.p1_H_maps[ii], self.m2_H_maps[ii]
                 else:
                     sp1, sm2 = self.load_sim_data(ii)
-                self._triple(p1, sp1, sm2, 1.0, S1, 1)
-                self._triple(sp1, p1, sm2, 1.0, S1, 1)
-                self._triple(sp1, sp1, m2, 1.0, S1, 1)
+                # (p1, sp1, sm2) + (sp1, p1, sm2) + (sp1, sp1, m2) in one pass
+                self._sums((p1, sp1, m2, sm2), 1.0, S1, 1)
             b1 = -self.base.A_pix * self._assemble(S1) / 3. / self.N_it
         return b3 + b1

@@ -111,3 +111,46 @@ def test_gpu_bspec_numerator_matches_oracle(gpu, pol, prec, nsims):
     scale = np.abs(ref).max()
     tol = 1e-9 if prec == 64 else 2e-4
     assert np.abs(got - ref).max() < tol * scale, (np.abs(got - ref).max() / scale)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,nmap,npix,ntrip", [(64, 7, 5001, 300), (32, 7, 5001, 300), (32, 60, 20000, 4500),
+                                                  (64, 128, 3000, 70)])
+def test_gpu_triple_and_linear_sums_match_numpy(gpu, prec, nmap, npix, ntrip):
+    """ps_bspec_triple_sums (three stacks and A = B) and ps_bspec_linear_sums against direct
+    FP64 sums; ragged pixel tiles, more triplets than one chunk, the 128-map maximum."""
+    import torch
+    from polyspec_amd._abi import load, ptr, stream_handle, check
+    print(f"bspec sums prec={prec} nmap={nmap} npix={npix} ntrip={ntrip}", flush=True)
+    rng = np.random.default_rng(nmap + ntrip)
+    X = [rng.standard_normal((nmap, npix)) + 1j * rng.standard_normal((nmap, npix)) for _ in range(4)]
+    trip = rng.integers(0, nmap, (ntrip, 3)).astype(np.int32)
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    Xd = [torch.from_numpy(x).to(gpu, cdt).contiguous() for x in X]
+    Xr = [x.astype(np.complex64).astype(np.complex128) if prec == 32 else x for x in X]
+    td = torch.from_numpy(trip).to(gpu)
+    lib = load()
+    nb = int(lib.ps_bspec_scratch_bytes(ntrip))
+    scr = torch.empty(nb, dtype=torch.uint8, device=gpu)
+
+    def cyc(A, B, C):
+        a, b, c = trip[:, 0], trip[:, 1], trip[:, 2]
+        return (A[a] * B[b] * C[c] + A[b] * B[c] * C[a] + A[c] * B[a] * C[b]).sum(1)
+
+    def run(fn, stacks, scale=1.0):
+        out = torch.zeros((ntrip, 2), dtype=torch.float64, device=gpu)
+        check(fn(prec, *[ptr(s) for s in stacks], nmap, npix, ptr(td), ntrip, scale, ptr(scr), nb, ptr(out), 0,
+                 stream_handle()))
+        o = out.cpu().numpy()
+        return o[:, 0] + 1j * o[:, 1]
+    tol = 1e-11 if prec == 64 else 2e-5
+    P, Q, M, N = Xr
+    want3 = cyc(P, Q, M)
+    got3 = run(lib.ps_bspec_triple_sums, (Xd[0], Xd[1], Xd[2]), 1.0)
+    assert np.abs(got3 - want3).max() < tol * np.abs(want3).max()
+    wantc = cyc(P, P, M)
+    gotc = run(lib.ps_bspec_triple_sums, (Xd[0], Xd[0], Xd[2]), 1.0)
+    assert np.abs(gotc - wantc).max() < tol * np.abs(wantc).max()
+    wantl = cyc(P, Q, N) + cyc(Q, P, N) + cyc(Q, Q, M)
+    gotl = run(lib.ps_bspec_linear_sums, tuple(Xd), 0.5) / 0.5
+    assert np.abs(gotl - wantl).max() < tol * np.abs(wantl).max()
