@@ -199,6 +199,22 @@ int ps_bspec_linear_sums(int precision, const void *P, const void *Q, const void
                          long long npix, const int *triplets, int ntrip, double scale, void *scratch,
                          long long scratch_bytes, void *out, int accumulate, void *stream);
 
+/* -------------------------------------------- radial-node optimizer
+ * Exact gNL-loc Fisher-matrix derivative (fisher_deriv_gNL_loc, ideal_fisher.pyx:750-787,
+ * called from _compute_fisher_derivatives, tspec_template.py:3858-3870), all FP64:
+ *   D(r, s) = pref w_r w_s sum_mu w_mu [S^PP^3 S^QQ + 3 S^PQ S^QP S^PP^2](mu; r, s),
+ *   S^XY(mu; r, s) = sum_l legs[mu][l] (2(lmin+l)+1) sum_ij X[l][i][r] Cinv[i][j][l] Y[l][j][s],
+ *   pref = 6 / (4 pi)^2 (9/25)^2 / 2.
+ * P, Q: (nl, npol, nr) for l = lmin .. lmin+nl-1 (npol <= 3); Cinv: (npol, npol, nl), the
+ * beam-weighted inverse total spectrum; legs: (nmu, nl) Legendre P_l(mu_k); w_mu: (nmu)
+ * quadrature weights; weights: (nr); out: (nr, nr), symmetric.  All device pointers.
+ * work: ps_fisher_deriv_work_bytes(nl, npol, nr, nmu) bytes (4 nl nr^2 doubles dominate).
+ * Synchronises the stream before returning. */
+long long ps_fisher_deriv_work_bytes(int nl, int npol, int nr, int nmu);
+int ps_fisher_deriv_gnl_loc(const double *P, const double *Q, int nl, int npol, int nr, const double *Cinv,
+                            const double *legs, const double *w_mu, int nmu, const double *weights, int lmin,
+                            void *work, long long work_bytes, double *out, void *stream);
+
 /* -------------------------------------------- learned CG preconditioner
  * Kernels of a 9-point-stencil convolutional network on HEALPix maps (FP32),
  * used as the nonlinear preconditioner of a flexible CG (no counterpart in the

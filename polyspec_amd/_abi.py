@@ -57,6 +57,8 @@ SIGNATURES = {
     "ps_bspec_scratch_bytes": (_LL, [_I]),
     "ps_bspec_triple_sums": (_I, [_I, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),
     "ps_bspec_linear_sums": (_I, [_I, _P, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),
+    "ps_fisher_deriv_work_bytes": (_LL, [_I, _I, _I, _I]),
+    "ps_fisher_deriv_gnl_loc": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _I, _P, _I, _P, _LL, _P, _P]),
 }
 
 _lib = None

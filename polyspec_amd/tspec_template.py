@@ -143,6 +143,193 @@ class TSpecTemplate:
         self.mw_d = torch.from_numpy(1. + (self.base.m_arr[self.base.l_arr <= self.lmax] > 0)).to(dev, rdt)
         self._lmaj = {}
 
+    # ------------------------------------------------- radial-node optimizer
+    def _fisher_deriv_gnl_loc(self, weights):
+        """Exact gNL-loc Fisher derivative D(r, r') on the device (fisher_deriv_gNL_loc,
+        ideal_fisher.pyx:750-787) with the inputs of _compute_fisher_derivatives
+        (tspec_template.py:3858-3870): 2 lmax + 1 Gauss-Legendre mu nodes, P_l(mu) for
+        l = lmin..lmax, the beam-weighted inverse total spectrum and the weights."""
+        from scipy.special import eval_legendre, roots_legendre
+        from ._abi import check, load, ptr, stream_handle
+        mus, w_mus = roots_legendre(2 * self.lmax + 1)
+        ls = np.arange(self.lmin, self.lmax + 1)
+        legs = eval_legendre(ls[None, :], mus[:, None])
+        beam = np.asarray(self.base.beam)
+        Cinv = (beam[:, None, :] * beam[None, :, :] * self.base.inv_Cl_tot_mat)[:, :, self.lmin:self.lmax + 1]
+        dev = self.base.device
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+        P = t(self.plLXs[self.nmax][self.lmin:self.lmax + 1])
+        Q = t(self.qlXs[self.lmin:self.lmax + 1])
+        nl, npol, nr = P.shape
+        Cd, Ld, Wd, wd = t(Cinv), t(legs), t(w_mus), t(weights)
+        nb = int(load().ps_fisher_deriv_work_bytes(nl, npol, nr, len(mus)))
+        ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        out = torch.empty((nr, nr), dtype=torch.float64, device=dev)
+        check(load().ps_fisher_deriv_gnl_loc(ptr(P), ptr(Q), nl, npol, nr, ptr(Cd), ptr(Ld), ptr(Wd), len(mus),
+                                             ptr(wd), self.lmin, ptr(ws), nb, ptr(out), stream_handle()))
+        D = out.cpu().numpy()
+        del ws
+        return D
+
+    def _compute_fisher_derivatives(self, templates, N_fish_optim=None, verb=False):
+        """Contact-template branch of tspec_template.py:3842-3900: {template: (sum D, D)}."""
+        output = {}
+        t0 = time.time()
+        for t in templates:
+            if t != 'gNL-loc':
+                raise NotImplementedError("Fisher derivatives are provided for gNL-loc only "
+                                          "(the tauNL Monte-Carlo score is not part of the hip backend)")
+            if verb:
+                print("\tComputing gNL-loc Fisher matrix derivative exactly")
+            D = self._fisher_deriv_gnl_loc(self.quad_weights_1d)
+            output[t] = (np.sum(D), D)
+        self.timers['fish_deriv'] += time.time() - t0
+        return output
+
+    @staticmethod
+    def _schur(D, inds):
+        """G = D_NN - D_SN^T D_SS^-1 D_SN and w_quad = 1 + row sums of D_SS^-1 D_SN
+        (tspec_template.py:3569-3573, 3596-3600)."""
+        notinds = np.setdiff1d(np.arange(len(D)), inds)
+        inv = np.linalg.inv(D[np.ix_(inds, inds)])
+        DSN = D[np.ix_(inds, notinds)]
+        X = inv @ DSN
+        return D[np.ix_(notinds, notinds)] - DSN.T @ X, 1. + np.sum(X, axis=1), notinds
+
+    def optimize_radial_sampling_1d(self, reduce_r=1, tolerance=1e-3, N_fish_optim=1, stalled_iterations=np.inf,
+                                    N_split=None, split_index=None, initial_r_points=None, verb=False,
+                                    optimize_weights=False, optimize_skip=0, repeat_trials=True, r_guess=[],
+                                    weight_guess={}):
+        """Radial sampling points and weights by greedy optimisation of the Fisher distance
+        (tspec_template.py:3111-3672), for the contact template gNL-loc: the exact Fisher
+        derivative on the starting grid (one device launch sequence), then the reference's
+        Schur-complement selection on the host.  Returns (r_arr, r_weights) and re-runs the
+        k-space precompute on the selected nodes."""
+        for t in self.templates:
+            if t != 'gNL-loc':
+                raise NotImplementedError("optimize_radial_sampling_1d is provided for gNL-loc only")
+        if optimize_weights:
+            raise NotImplementedError("optimize_weights applies to tauNL templates only")
+        t_init = time.time()
+        assert reduce_r > 0, "reduce_r parameter must be positive"
+        if reduce_r < 0.5:
+            print("## Caution: very dense r-sampling requested; computation may be very slow")
+        if reduce_r > 3:
+            print("## Caution: very sparse r-sampling requested; computation may be inaccurate")
+        # starting grid (tspec_template.py:3139-3142)
+        r_raw = np.asarray(list(np.arange(1, self.r_star * 0.95, 50 * reduce_r)) +
+                           list(np.arange(self.r_star * 0.95, self.r_hor * 1.05, 5 * reduce_r)) +
+                           list(np.arange(self.r_hor * 1.05, self.r_hor + 5000, 50 * reduce_r)))
+        r_init = 0.5 * (r_raw[1:] + r_raw[:-1])
+        self.quad_weights_1d = r_init ** 2 * np.diff(r_raw)
+        if initial_r_points is not None:
+            assert split_index is None, "Cannot specify both initial_r_points and index_split"
+            assert len(initial_r_points) == len(np.unique(initial_r_points)), \
+                "initial_r_points cannot contain repeated points"
+            sel = np.asarray([np.where(r == r_init)[0][0] for r in initial_r_points])
+            r_init, self.quad_weights_1d = r_init[sel], self.quad_weights_1d[sel]
+        elif N_split is not None:
+            print("Partitioning sampling grid into %d pieces" % N_split)
+            r_init = r_init[split_index::N_split]
+            self.quad_weights_1d = self.quad_weights_1d[split_index::N_split]
+        self.r_arr = r_init
+        self.N_r = len(r_init)
+        self.r_weights = {t: self.quad_weights_1d.copy() for t in self.templates}
+        print("# Computing k integrals with fiducial radial grid")
+        self._prepare_templates()
+
+        inds_guess = []
+        for rg in r_guess:
+            index = np.where(rg == r_init)[0]
+            if len(index) == 0:
+                raise Exception("r_guess was computed with different settings! All points in r_guess must be in r_init.")
+            inds_guess.append(int(index[0]))
+        if len(weight_guess.keys()) > 0:
+            assert len(inds_guess) > 0, "Must supply r_guess!"
+        if verb:
+            print("Computing all Fisher matrix derivatives")
+        derivs = self._compute_fisher_derivatives(self.templates, N_fish_optim=N_fish_optim, verb=verb)
+        if not hasattr(self, 'ideal_fisher'):
+            self.ideal_fisher = {}
+        inds_init = np.arange(self.N_r)
+        r_weights = {}
+        for template in self.templates:
+            init_score, D = derivs[template]
+            self.ideal_fisher[template] = init_score
+            if verb:
+                print("\nRunning optimization for template %s\nInitial score: %.2e" % (template, init_score))
+            inds = list(inds_guess)
+            G = None
+            if len(inds):
+                G, w_quad, notinds = self._schur(D, inds)
+                if template in weight_guess.keys():
+                    n_guess = int(np.sum(weight_guess[template] != 0))
+                    _, w_quad_guess, _ = self._schur(D, inds[:n_guess])
+                    w_vals = w_quad + np.concatenate([weight_guess[template][:n_guess] /
+                                                      self.quad_weights_1d[inds[:n_guess]] - w_quad_guess,
+                                                      np.zeros(len(inds) - n_guess)])
+                else:
+                    w_vals = w_quad
+                score = float(np.sum(G))
+                if verb:
+                    print("Unoptimized relative score: %.2e" % (score / init_score))
+            else:
+                score, w_quad, w_vals = init_score, np.zeros(0), np.zeros(0)
+            stalled_score = score
+            if score / init_score >= tolerance:
+                if len(inds) == 0:
+                    next_inds = np.argsort(np.sum(D, axis=1) ** 2 / np.diag(D))[::-1]
+                else:
+                    next_inds = inds_init[notinds][np.argsort(np.sum(G, axis=1) ** 2 / np.diag(G))[::-1]]
+                inds.append(int(next_inds[0]))
+                w_vals_old, w_quad_old, score_old = w_vals, w_quad, score
+                for iit, iteration in enumerate(range(len(inds), self.N_r)):
+                    for trial in range(len(next_inds)):
+                        inds[-1] = int(next_inds[trial])
+                        G, w_quad, notinds = self._schur(D, inds)
+                        w_vals = np.append(w_vals_old, 0.) + w_quad - np.append(w_quad_old, 0.)
+                        score = float(np.sum(G))
+                        if verb:
+                            print("Iteration %d, trial %d, relative score: %.2e, old score: %.2e" %
+                                  (iteration, trial, score / init_score, score_old / init_score))
+                        if score < score_old or not repeat_trials:
+                            break
+                    if score < 0:
+                        print("## Score is negative; this indicates a numerical error!")
+                        break
+                    if score / init_score < tolerance:
+                        break
+                    if iit % stalled_iterations == 0 and iit > 0:
+                        if np.abs(score - stalled_score) / stalled_score < 0.05:
+                            print("Stalled at iteration %d" % iteration)
+                            break
+                        stalled_score = score
+                    w_vals_old, w_quad_old, score_old = w_vals, w_quad, score
+                    next_inds = inds_init[notinds][np.argsort(np.sum(G, axis=1) ** 2 / np.diag(G))[::-1]]
+                    inds.append(int(next_inds[0]))
+            if G is None or len(G) == 0:
+                raise Exception("Failed to converge after %d iterations; this indicates a bug!" % N_fish_optim)
+            if verb:
+                print("\nScore threshold met with %d indices" % len(inds))
+            w_opt = np.asarray(w_vals, dtype=np.float64)
+            fish = float(np.sum(np.outer(w_opt, w_opt) * D[np.ix_(inds, inds)]))
+            if verb:
+                print("Ideal %s Fisher: %.4e (initial), %.4e (optimized). Relative score: %.2e\n" %
+                      (template, init_score, fish, score / init_score))
+            self.optimization_score = {template: score / init_score}
+            r_weights[template] = w_opt * self.quad_weights_1d[inds]
+        self.r_arr = r_init[inds]
+        self.N_r = len(self.r_arr)
+        self.r_weights = {}
+        for template in self.templates:
+            self.r_weights[template] = np.zeros(len(w_opt))
+            self.r_weights[template][:len(r_weights[template])] = r_weights[template]
+        if verb:
+            print("Computing k integrals with optimized radial grid")
+        self._prepare_templates()
+        print("\nOptimization complete after %.2f seconds" % (time.time() - t_init))
+        return self.r_arr, self.r_weights
+
     # ---------------------------------------------------------------- timing
     def report_timings(self):
         print("\n## Timings ##")
