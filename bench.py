@@ -35,6 +35,7 @@ PEAK_FP32 = 157.3e12
 # + WRITE_SIZE x 1 (factors from a 1 GiB copy), profiles/leg_synth_s0_mfma_nside512_lmax1024_80maps_pmc.md
 LEG_TRAFFIC = {(512, 1024, 80): (186470.8 * 2 + 1311359.4) * 1024}
 PEAK_HBM = 8.0e12
+PEAK_FP64 = 78.6e12
 
 
 def log(*a):
@@ -318,6 +319,61 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
     return out
 
 
+def bench_radial(base, raw, mask, tables, lmax, tolerance=1e-3, cpu_nodes=24):
+    """gNL-loc radial-node optimizer (optimize_radial_sampling_1d) on the starting grid of
+    the reference (reduce_r = 1): k-integrals on every node, the exact Fisher derivative on
+    the device (FP64 matrix cores), greedy selection on the host.  Also times the device
+    Fisher derivative alone (wall clock around the call, table upload included) and the FP64
+    oracle on a bounded sample of the same tables (cpu_nodes nodes), per (r, r') pair."""
+    import torch
+    from oracle.radial import fisher_deriv_gnl_loc, mu_quadrature
+    tm, _ = make_template(base, raw, mask, tables, lmax, True)
+    t0 = time.perf_counter()
+    r_opt, w_opt = tm.optimize_radial_sampling_1d(tolerance=tolerance)
+    t_total = time.perf_counter() - t0
+    n_start = len(tm.quad_weights_1d)
+    out = {"lmax": lmax, "tolerance": tolerance, "n_start_nodes": n_start, "n_selected": int(len(r_opt)),
+           "relative_score": float(tm.optimization_score['gNL-loc']), "ideal_fisher": float(tm.ideal_fisher['gNL-loc']),
+           "s_total": t_total, "s_fisher_deriv_in_optimizer": tm.timers['fish_deriv']}
+    # device Fisher derivative alone, on the starting grid's tables
+    sg = tm.start_grid
+    qw = sg['weights']
+    tm.plLXs, tm.qlXs = sg['plXs'][None], sg['qlXs']
+    tm._fisher_deriv_gnl_loc(qw)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    D = tm._fisher_deriv_gnl_loc(qw)
+    torch.cuda.synchronize()
+    t_d = time.perf_counter() - t0
+    nl = lmax - 1
+    nmu = 2 * lmax + 1
+    npair = n_start * (n_start + 1) / 2
+    flops = 8. * nmu * nl * npair  # 4 tensors x (mu x l) x pairs, multiply-add = 2
+    out["fisher_deriv_s"] = t_d
+    out["fisher_deriv_tflops"] = flops / t_d / 1e12
+    out["fisher_deriv_frac_fp64_peak"] = flops / t_d / PEAK_FP64
+    log("radial optimizer: %d -> %d nodes (score %.2e) in %.1fs; Fisher derivative %.3fs = %.1f TFLOP/s FP64"
+        % (n_start, len(r_opt), out["relative_score"], t_total, t_d, flops / t_d / 1e12))
+    # FP64 oracle on a bounded sample of the same tables
+    _, w_mus, legs = mu_quadrature(tm.lmin, tm.lmax)
+    beam = np.asarray(tm.base.beam)
+    C = beam[:, None, :] * beam[None, :, :] * tm.base.inv_Cl_tot_mat
+    sel = np.linspace(0, n_start - 1, cpu_nodes).astype(int)
+    t0 = time.perf_counter()
+    Dc = fisher_deriv_gnl_loc(sg['plXs'][:, :, sel], sg['qlXs'][:, :, sel], qw[sel], C, legs, w_mus, tm.lmin, tm.lmax)
+    t_c = time.perf_counter() - t0
+    # the reference evaluates all nr^2 ordered pairs (ideal_fisher.pyx:781-785), as the oracle does
+    t_full = t_c * (n_start / cpu_nodes) ** 2
+    out["cpu_oracle"] = {"s_sample": t_c, "sample": "%d of the %d nodes (all %d ordered pairs)" %
+                         (cpu_nodes, n_start, cpu_nodes ** 2), "s_full_matrix_extrapolated": t_full,
+                         "cores": 1, "kind": "port"}
+    out["speedup_vs_cpu_oracle"] = t_full / t_d
+    out["max_rel_diff_vs_oracle_on_sample"] = float(np.abs(D[np.ix_(sel, sel)] - Dc).max() / np.abs(Dc).max())
+    del tm, D
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(data_np, base, tm, nr_sample, lmax):
     """FP64 oracle on the host: S^-1 of the data plus `nr_sample` of the 40 radial
     nodes (2 syntheses each + gnl sums), extrapolated to one numerator."""
@@ -368,6 +424,7 @@ def main():
     ap.add_argument("--learned-pairs", type=int, default=2000,
                     help="training pairs for the learned CG preconditioner (0 = plain CG only)")
     ap.add_argument("--no-bspec", action="store_true", help="skip the binned-bispectrum section")
+    ap.add_argument("--no-radial", action="store_true", help="skip the radial-node optimizer section")
     ap.add_argument("--bspec-nside", type=str, default="256",
                     help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
     ap.add_argument("--bspec-sims", type=int, default=16)
@@ -486,6 +543,10 @@ def main():
     if not args.no_cg:
         log("CG S^-1: plain harmonic vs learned preconditioner ...")
         result["cg"] = bench_cg(nside, mask, learned_pairs=args.learned_pairs, nside_lo=min(128, nside))
+
+    if not args.no_radial:
+        log("radial-node optimizer ...")
+        result["radial_optimizer"] = bench_radial(base32, raw, mask, tables, lmax)
 
     if not args.no_bspec:
         log("binned bispectrum ...")

@@ -13,13 +13,14 @@ fisher_deriv_gnl_loc: with z^{XY}_l(r, r') = (2l + 1) sum_ij X_l^i(r) C^-1_{l,ij
 (X, Y in {p, q}) and S^{XY}(mu; r, r') = sum_l P_l(mu) z^{XY}_l(r, r'),
   D(r, r') = pref w_r w_r' sum_mu w_mu [S^PP^3 S^QQ + 3 S^PQ S^QP S^PP^2](mu; r, r'),
   S^QP(r, r') = S^PQ(r', r),  pref = 6 / (4 pi)^2 (9/25)^2 / 2.
-Pinned in tests/test_radial.py: the mu quadrature against the closed form
+Pinned in tests/test_radial.py: P_l(mu) against 40-digit mpmath values at l up to 1024, the mu
+quadrature against the closed form
 int P_l^4 = 2 sum_L (2L+1) (l l L; 0 0 0)^4 (a single l), symmetry, and the
 Schur-complement score against the direct quadratic form; sum(D) against the
 Monte-Carlo gNL-loc Fisher of the estimator in the GPU tests.
 """
 import numpy as np
-from scipy.special import eval_legendre, roots_legendre
+from scipy.special import roots_legendre
 
 PREF_GNL_LOC = 6. / (4. * np.pi) ** 2 * (9. / 25.) ** 2 / 2.
 
@@ -35,10 +36,15 @@ def radial_grid(r_star, r_hor, reduce_r=1):
 
 def mu_quadrature(lmin, lmax):
     """Gauss-Legendre nodes / weights with 2 lmax + 1 points and P_l(mu) for l = lmin..lmax
-    (tspec_template.py:3863-3865; lpmn's m = 0 column is the Legendre polynomial)."""
+    (tspec_template.py:3863-3865; lpmn's m = 0 column is the Legendre polynomial, evaluated
+    like lpmn by the upward recursion (l+1) P_{l+1} = (2l+1) mu P_l - l P_{l-1})."""
     mus, w_mus = roots_legendre(2 * lmax + 1)
-    ls = np.arange(lmin, lmax + 1)
-    legs = eval_legendre(ls[None, :], mus[:, None])
+    legs = np.empty((len(mus), lmax - lmin + 1))
+    pm1, p0 = np.zeros_like(mus), np.ones_like(mus)
+    for l in range(lmax + 1):
+        if l >= lmin:
+            legs[:, l - lmin] = p0
+        pm1, p0 = p0, ((2 * l + 1) * mus * p0 - l * pm1) / (l + 1)
     return mus, w_mus, legs
 
 

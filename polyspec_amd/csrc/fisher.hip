@@ -88,11 +88,16 @@ __global__ __launch_bounds__(256) void k_fd_gemm(FdDims d, const double* __restr
     __shared__ double red[4][FD_PT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane & 15, g = lane >> 4;
-    const long tile = blockIdx.x;
-    const int mb = blockIdx.y;
+    // mu blocks of one pair tile are consecutive logical blocks on one XCD, so the tile's
+    // zeta columns come from HBM once and are re-read from that XCD's L2
+    const int nmb = d.NMUP / FD_MUB;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const long tile = lb / nmb;
+    const int mb = lb % nmb;
     const int2 rt = tiles[tile];
     const int r = rt.x, s0 = rt.y * FD_PT;
     const int mu0 = mb * FD_MUB + wave * 64;
+    const bool live = mu0 < d.nmu;  // waves past the last node only help staging
     const long zs = (long)d.NLP * d.nr * d.NRP;
     // staging: thread t loads 4 consecutive s of (l = t / 16, X = (t / 4) % 4)
     const int sl = tid >> 4, sX = (tid >> 2) & 3, sj = (tid & 3) * 4;
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(256) void k_fd_gemm(FdDims d, const double* __restr
 #pragma unroll
             for (int u = 0; u < 4; u++) pa[u][ks] = legsT[(long)(l0 + 4 * ks + g) * d.NMUP + mu0 + 16 * u + j];
     };
-    fetchA(0);
+    if (live) fetchA(0);
     v4d acc[4][4];  // [X][u]
 #pragma unroll
     for (int X = 0; X < 4; X++)
@@ -123,8 +128,9 @@ __global__ __launch_bounds__(256) void k_fd_gemm(FdDims d, const double* __restr
         __syncthreads();
         if (l0 + FD_LCH < d.NLP) {
             pb = *reinterpret_cast<const double4*>(zsrc + (long)(l0 + FD_LCH + sl) * lstride);
-            fetchA(l0 + FD_LCH);
+            if (live) fetchA(l0 + FD_LCH);
         }
+        if (!live) continue;
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
             double b[4];
@@ -257,9 +263,9 @@ int ps_fisher_deriv_gnl_loc(const double* P, const double* Q, int nl, int npol, 
     hipLaunchKernelGGL(k_fd_legs, dim3((unsigned)std::min<long>((nlg + 255) / 256, 65536)), dim3(256), 0, st, d, legs,
                        w_mu, legsT, wmup);
     PS_LAUNCH_CHECK();
-    PS_REQUIRE(d.ntile < 2147483647L, "ps_fisher_deriv_gnl_loc: too many radial nodes");
-    hipLaunchKernelGGL(k_fd_gemm, dim3((unsigned)d.ntile, (unsigned)(d.NMUP / FD_MUB)), dim3(256), 0, st, d, Z, legsT,
-                       wmup, tiles, part);
+    PS_REQUIRE(d.ntile * (d.NMUP / FD_MUB) < 2147483647L, "ps_fisher_deriv_gnl_loc: too many radial nodes");
+    hipLaunchKernelGGL(k_fd_gemm, dim3((unsigned)(d.ntile * (d.NMUP / FD_MUB))), dim3(256), 0, st, d, Z, legsT, wmup,
+                       tiles, part);
     PS_LAUNCH_CHECK();
     const long nn = (long)nr * nr;
     const double pref = 6.0 / (16.0 * M_PI * M_PI) * (81.0 / 625.0) / 2.0;

@@ -13,6 +13,9 @@ scipy.special.spherical_jn (the reference tabulates j_l on an x grid with GSL's
 Steed algorithm and interpolates linearly, k_integrals.pyx:28-110; direct
 evaluation removes that interpolation error).
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 from scipy.special import gammaln, hyp2f1, spherical_jn
 
@@ -22,13 +25,22 @@ def Pzeta(k, As=2.1e-9, ns=0.96, k_pivot=0.05):
     return 2. * np.pi ** 2 / k ** 3 * As * (k / k_pivot) ** (ns - 1)
 
 
-def jl_kr(ls, k_arr, r_arr):
-    """j_l(k r) as (len(ls), len(r), len(k))."""
+def jl_kr(ls, k_arr, r_arr, nthreads=None):
+    """j_l(k r) as (len(ls), len(r), len(k)); the values of l are spread over a thread pool
+    (scipy's ufunc releases the GIL on whole arrays)."""
     ls = np.asarray(ls)
     x = np.asarray(r_arr)[:, None] * np.asarray(k_arr)[None, :]
     out = np.empty((len(ls), len(r_arr), len(k_arr)))
-    for i, l in enumerate(ls):
-        out[i] = spherical_jn(int(l), x)
+
+    def one(i):
+        out[i] = spherical_jn(int(ls[i]), x)
+    n = nthreads or min(16, os.cpu_count() or 1)
+    if n > 1 and len(ls) > 1:
+        with ThreadPoolExecutor(n) as ex:
+            list(ex.map(one, range(len(ls))))
+    else:
+        for i in range(len(ls)):
+            one(i)
     return out
 
 
@@ -38,11 +50,13 @@ def _trapz_sum(k_arr, integrand):
     return np.sum(dk * (integrand[..., 1:] + integrand[..., :-1]), axis=-1) / 2.
 
 
-def q_integral(k_arr, Tl_arr, r_arr, lmin, lmax, nslot):
-    """qlXs (lmax+1, nslot, N_r): zero outside lmin <= l <= lmax and X >= len(Tl_arr)."""
+def q_integral(k_arr, Tl_arr, r_arr, lmin, lmax, nslot, jl=None):
+    """qlXs (lmax+1, nslot, N_r): zero outside lmin <= l <= lmax and X >= len(Tl_arr).
+    jl: j_l(k r) for l = lmin..lmax from jl_kr, if already computed."""
     k_arr = np.asarray(k_arr, dtype=np.float64)
     ls = np.arange(lmin, lmax + 1)
-    jl = jl_kr(ls, k_arr, r_arr)  # (nl, nr, nk)
+    if jl is None:
+        jl = jl_kr(ls, k_arr, r_arr)  # (nl, nr, nk)
     out = np.zeros((lmax + 1, nslot, len(r_arr)))
     kp = 2. / np.pi * k_arr ** 2
     sgn = (-1.) ** ls
@@ -52,11 +66,12 @@ def q_integral(k_arr, Tl_arr, r_arr, lmin, lmax, nslot):
     return out
 
 
-def p_integral_n0(k_arr, Pz, Tl_arr, r_arr, lmin, lmax, lmax_arr, nslot):
+def p_integral_n0(k_arr, Pz, Tl_arr, r_arr, lmin, lmax, lmax_arr, nslot, jl=None):
     """plLXs[nmax=0] (lmax_arr+1, nslot, N_r): zero outside lmin <= l <= lmax."""
     k_arr = np.asarray(k_arr, dtype=np.float64)
     ls = np.arange(lmin, lmax + 1)
-    jl = jl_kr(ls, k_arr, r_arr)
+    if jl is None:
+        jl = jl_kr(ls, k_arr, r_arr)
     out = np.zeros((lmax_arr + 1, nslot, len(r_arr)))
     kp = 2. / np.pi * k_arr ** 2 * np.asarray(Pz)
     sgn = (-1.) ** ls

@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import kernels as K
-from .k_integrals import F_integral_n0, Pzeta as _Pzeta, p_integral_n0, q_integral
+from .k_integrals import F_integral_n0, Pzeta as _Pzeta, jl_kr, p_integral_n0, q_integral
 
 SUPPORTED = ('gNL-loc', 'tauNL-loc')
 FOURPI15 = (4. * np.pi) ** 1.5
@@ -123,9 +123,11 @@ class TSpecTemplate:
         if hasattr(self, 't0_num'):
             delattr(self, 't0_num')
         nslot = 1 + 2 * self.pol
+        jl = jl_kr(np.arange(self.lmin, self.lmax + 1), self.k_arr, self.r_arr)  # shared by p and q
         self.plLXs = p_integral_n0(self.k_arr, self.Pzeta(self.k_arr), self.Tl_arr, self.r_arr, self.lmin, self.lmax,
-                                   self.lmax + 1, nslot)[None]
-        self.qlXs = q_integral(self.k_arr, self.Tl_arr, self.r_arr, self.lmin, self.lmax, nslot)
+                                   self.lmax + 1, nslot, jl=jl)[None]
+        self.qlXs = q_integral(self.k_arr, self.Tl_arr, self.r_arr, self.lmin, self.lmax, nslot, jl=jl)
+        del jl
         if 'tauNL-loc' in self.templates:
             self.FLLs = F_integral_n0(self.r_arr, self.Lmin, self.Lmax, self.ns, self.As, self.k_pivot)[None]
         self._upload_tables()
@@ -149,11 +151,16 @@ class TSpecTemplate:
         ideal_fisher.pyx:750-787) with the inputs of _compute_fisher_derivatives
         (tspec_template.py:3858-3870): 2 lmax + 1 Gauss-Legendre mu nodes, P_l(mu) for
         l = lmin..lmax, the beam-weighted inverse total spectrum and the weights."""
-        from scipy.special import eval_legendre, roots_legendre
+        from scipy.special import roots_legendre
         from ._abi import check, load, ptr, stream_handle
         mus, w_mus = roots_legendre(2 * self.lmax + 1)
-        ls = np.arange(self.lmin, self.lmax + 1)
-        legs = eval_legendre(ls[None, :], mus[:, None])
+        # P_l(mu) by the upward three-term recursion (l+1) P_{l+1} = (2l+1) mu P_l - l P_{l-1}
+        legs = np.empty((len(mus), self.lmax - self.lmin + 1))
+        pm1, p0 = np.zeros_like(mus), np.ones_like(mus)
+        for l in range(self.lmax + 1):
+            if l >= self.lmin:
+                legs[:, l - self.lmin] = p0
+            pm1, p0 = p0, ((2 * l + 1) * mus * p0 - l * pm1) / (l + 1)
         beam = np.asarray(self.base.beam)
         Cinv = (beam[:, None, :] * beam[None, :, :] * self.base.inv_Cl_tot_mat)[:, :, self.lmin:self.lmax + 1]
         dev = self.base.device
@@ -249,6 +256,10 @@ class TSpecTemplate:
         if verb:
             print("Computing all Fisher matrix derivatives")
         derivs = self._compute_fisher_derivatives(self.templates, N_fish_optim=N_fish_optim, verb=verb)
+        # the starting grid and its tables, kept for diagnostics once the nodes are replaced
+        self.start_grid = {'r': r_init.copy(), 'weights': self.quad_weights_1d.copy(),
+                           'plXs': self.plLXs[self.nmax].copy(), 'qlXs': self.qlXs.copy(),
+                           'fisher_deriv': {t: derivs[t][1] for t in derivs}}
         if not hasattr(self, 'ideal_fisher'):
             self.ideal_fisher = {}
         inds_init = np.arange(self.N_r)

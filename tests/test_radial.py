@@ -101,6 +101,16 @@ def test_optimizer_meets_tolerance():
     assert abs(fish - v @ D @ v) <= 1e-9 * init
 
 
+def test_legendre_table_against_mpmath():
+    import mpmath
+    mpmath.mp.dps = 40
+    mus, _, legs = mu_quadrature(2, 1024)
+    for l in (2, 17, 500, 1024):
+        for i in (0, 5, 700, 1024, 2048):
+            want = float(mpmath.legendre(l, mpmath.mpf(mus[i])))
+            assert abs(legs[i, l - 2] - want) < 1e-12
+
+
 def test_radial_grid_shape():
     r, w = radial_grid(13882.607764400758, 14163.032903769747)
     assert len(r) == len(w) and np.all(np.diff(r) > 0) and np.all(w > 0)
