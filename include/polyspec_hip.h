@@ -209,7 +209,7 @@ int ps_bspec_linear_sums(int precision, const void *P, const void *Q, const void
  * beam-weighted inverse total spectrum; legs: (nmu, nl) Legendre P_l(mu_k); w_mu: (nmu)
  * quadrature weights; weights: (nr); out: (nr, nr), symmetric.  All device pointers.
  * work: ps_fisher_deriv_work_bytes(nl, npol, nr, nmu) bytes (4 nl nr^2 doubles dominate).
- * Synchronises the stream before returning. */
+ */
 long long ps_fisher_deriv_work_bytes(int nl, int npol, int nr, int nmu);
 int ps_fisher_deriv_gnl_loc(const double *P, const double *Q, int nl, int npol, int nr, const double *Cinv,
                             const double *legs, const double *w_mu, int nmu, const double *weights, int lmin,

@@ -15,7 +15,6 @@
 // the upper triangle s >= r run (D is symmetric); the last pass adds the mu-block partials
 // in a fixed order, scales by pref w_r w_s and mirrors.
 #include <algorithm>
-#include <vector>
 
 #include "polyspec_hip.h"
 #include "ps_common.h"
@@ -161,15 +160,30 @@ __global__ __launch_bounds__(256) void k_fd_gemm(FdDims d, const double* __restr
         part[((long)mb * d.ntile + tile) * FD_PT + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
+// First tile of row r: sum over r' < r of the tiles of row r', (NRP/16 - r'/16).
+__host__ __device__ inline long fd_row_tile0(int r, int K) {
+    const long q = r / FD_PT, rem = r % FD_PT;
+    return (long)r * K - (FD_PT * q * (q - 1) / 2 + rem * q);
+}
+
+// tile list: row r owns (r, s-tile) for s-tiles r/16 .. NRP/16 - 1
+__global__ void k_fd_tiles(FdDims d, int2* __restrict__ tiles) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.nr) return;
+    const int K = d.NRP / FD_PT;
+    long t = fd_row_tile0(r, K);
+    for (int st = r / FD_PT; st < K; st++) tiles[t++] = int2{r, st};
+}
+
 // D[r][s] = D[s][r] = pref w_r w_s sum_mb part[mb][tile(r, s / 16)][s % 16] for s >= r
-__global__ void k_fd_finish(FdDims d, const double* __restrict__ part, const long* __restrict__ row_tile0,
-                            const double* __restrict__ w, double pref, double* __restrict__ out) {
+__global__ void k_fd_finish(FdDims d, const double* __restrict__ part, const double* __restrict__ w, double pref,
+                            double* __restrict__ out) {
     const long n = (long)d.nr * d.nr;
     const int nmb = d.NMUP / FD_MUB;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
         const int r = (int)(e / d.nr), s = (int)(e % d.nr);
         if (s < r) continue;
-        const long tile = row_tile0[r] + (s / FD_PT - r / FD_PT);
+        const long tile = fd_row_tile0(r, d.NRP / FD_PT) + (s / FD_PT - r / FD_PT);
         double v = 0.0;
         for (int mb = 0; mb < nmb; mb++) v += part[((long)mb * d.ntile + tile) * FD_PT + s % FD_PT];
         v *= pref * w[r] * w[s];
@@ -195,7 +209,7 @@ static FdDims fd_dims(int nl, int npol, int nr, int nmu, int lmin) {
 }
 
 struct FdWork {
-    size_t z, legs, wmu, tiles, rowt, part, total;
+    size_t z, legs, wmu, tiles, part, total;
 };
 
 static FdWork fd_work(const FdDims& d) {
@@ -209,8 +223,6 @@ static FdWork fd_work(const FdDims& d) {
     o += al((size_t)d.NMUP * sizeof(double));
     w.tiles = o;
     o += al((size_t)d.ntile * sizeof(int2));
-    w.rowt = o;
-    o += al((size_t)d.nr * sizeof(long));
     w.part = o;
     o += al((size_t)(d.NMUP / FD_MUB) * d.ntile * FD_PT * sizeof(double));
     w.total = o;
@@ -243,18 +255,9 @@ int ps_fisher_deriv_gnl_loc(const double* P, const double* Q, int nl, int npol, 
     double* legsT = (double*)(base + wk.legs);
     double* wmup = (double*)(base + wk.wmu);
     int2* tiles = (int2*)(base + wk.tiles);
-    long* rowt = (long*)(base + wk.rowt);
     double* part = (double*)(base + wk.part);
-    // tile list (r, s-tile) for s-tiles reaching s >= r, and the first tile of every row
-    std::vector<int2> ht((size_t)d.ntile);
-    std::vector<long> hr((size_t)nr);
-    long t = 0;
-    for (int r = 0; r < nr; r++) {
-        hr[r] = t;
-        for (int stl = r / FD_PT; stl < d.NRP / FD_PT; stl++) ht[t++] = int2{r, stl};
-    }
-    PS_CHECK_HIP(hipMemcpyAsync(tiles, ht.data(), ht.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    PS_CHECK_HIP(hipMemcpyAsync(rowt, hr.data(), hr.size() * sizeof(long), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fd_tiles, dim3((nr + 255) / 256), dim3(256), 0, st, d, tiles);
+    PS_LAUNCH_CHECK();
     const long nz = (long)d.NLP * d.nr * d.NRP;
     hipLaunchKernelGGL(k_fd_zeta, dim3((unsigned)std::min<long>((nz + 255) / 256, 65536)), dim3(256), 0, st, d, P,
                        Q, Cinv, Z);
@@ -270,10 +273,8 @@ int ps_fisher_deriv_gnl_loc(const double* P, const double* Q, int nl, int npol, 
     const long nn = (long)nr * nr;
     const double pref = 6.0 / (16.0 * M_PI * M_PI) * (81.0 / 625.0) / 2.0;
     hipLaunchKernelGGL(k_fd_finish, dim3((unsigned)std::min<long>((nn + 255) / 256, 65536)), dim3(256), 0, st, d, part,
-                       rowt, weights, pref, out);
+                       weights, pref, out);
     PS_LAUNCH_CHECK();
-    // the host tables must outlive the asynchronous copies
-    PS_CHECK_HIP(hipStreamSynchronize(st));
     return 0;
 }
 
