@@ -558,8 +558,12 @@ def main():
         dnp = datas[0].to(torch.float64).cpu().numpy()
         result["cpu_baseline"] = cpu_baseline(dnp, base32, tm32, args.cpu_nodes, lmax)
         log("cpu: %s" % result["cpu_baseline"]["sample"])
-    print(json.dumps(result), flush=True)
+    return result
 
 
 if __name__ == "__main__":
-    main()
+    # the library prints progress as the reference does; keep stdout for the one JSON line
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):
+        res = main()
+    print(json.dumps(res), flush=True)

@@ -155,16 +155,25 @@ __global__ __launch_bounds__(BS_NT) void k_bspec_sums(const cplx<T>* __restrict_
     }
 }
 
-// out[t] (+)= scale * sum_b part[b][t]
+// out[t] (+)= scale * sum_b part[b][t]: 32 lanes per triplet, lane q adds blocks q, q + 32, ...
+// in order, then a fixed butterfly over the 32 lanes (bit-reproducible)
 __global__ void k_bspec_sum(const cplx<double>* __restrict__ part, int nblk, int ntrip, double scale,
                             int accumulate, cplx<double>* __restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntrip) return;
+    const int q = threadIdx.x & 31;
+    const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const bool ok = t < ntrip;
     double re = 0.0, im = 0.0;
-    for (int b = 0; b < nblk; b++) {
-        re += part[(size_t)b * ntrip + t].re;
-        im += part[(size_t)b * ntrip + t].im;
+    if (ok)
+        for (int b = q; b < nblk; b += 32) {
+            re += part[(size_t)b * ntrip + t].re;
+            im += part[(size_t)b * ntrip + t].im;
+        }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        re += __shfl_xor(re, o, 32);
+        im += __shfl_xor(im, o, 32);
     }
+    if (!ok || q != 0) return;
     if (accumulate) {
         out[t].re += scale * re;
         out[t].im += scale * im;
@@ -218,8 +227,8 @@ static int bs_run(int precision, int mode, const void* X0, const void* X1, const
            : mode == BS_CUBIC  ? bs_launch<double, BS_CUBIC>(X0, X1, X2, X3, nmap, npix, triplets, ntrip, part, ppb, nblk, st)
                                : bs_launch<double, BS_LINEAR>(X0, X1, X2, X3, nmap, npix, triplets, ntrip, part, ppb, nblk, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_bspec_sum, dim3((ntrip + 255) / 256), dim3(256), 0, st, (const cplx<double>*)part, (int)nblk,
-                       ntrip, scale, accumulate, (cplx<double>*)out);
+    hipLaunchKernelGGL(k_bspec_sum, dim3((unsigned)(((long)ntrip * 32 + 255) / 256)), dim3(256), 0, st,
+                       (const cplx<double>*)part, (int)nblk, ntrip, scale, accumulate, (cplx<double>*)out);
     PS_LAUNCH_CHECK();
     return 0;
 }

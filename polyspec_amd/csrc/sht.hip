@@ -587,15 +587,18 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
 //   A[i][k] at lane i + 32 k;  B[k][j] at lane j + 32 k;
 //   D[i][j] at lane j + 32 ((i / 4) % 2), register 4 (i / 8) + i % 4.
 // Lanes i and i + 32 run the same FP64 recursion (ring i) and feed l, l + 1 (k = 0) and
-// l + 2, l + 3 (k = 1).  One block = 4 waves = 128 ring pairs, one m, one 16-column chunk;
-// the chunk's alm rows are staged in LDS.
+// l + 2, l + 3 (k = 1).  One block = 4 waves = 128 ring pairs, one m, NT tiles of 16 complex
+// columns: every recursion value feeds 2 NT MFMAs, so the FP64 recursion is shared by up to
+// 80 columns.  The chunk's alm rows are staged in LDS.
 typedef float v16f __attribute__((ext_vector_type(16)));
 
+template <int NT>
 __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int ncol, int ld,
                                                            const cplx<float>* __restrict__ alm_t,
                                                            cplx<float>* __restrict__ phase) {
-    constexpr int BCM = 16;
-    __shared__ __align__(16) float sa[LCH * 2 * BCM];
+    constexpr int BCM = 16 * NT;   // complex columns per block
+    constexpr int RW = 2 * BCM;    // floats per staged row
+    __shared__ __align__(16) float sa[LCH * RW];
     __shared__ double sco[LCH * 2];
     __shared__ cplx<float> rot[128];
     const int nrb = (d.npairs + 127) / 128;
@@ -617,21 +620,27 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
     const double x = d.cth[jpc];
     Rec r;
     rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
-    v16f Dev = {0}, Dod = {0};
+    v16f Dev[NT], Dod[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) Dev[c] = Dod[c] = v16f{0};
 
     const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
     const float* a = reinterpret_cast<const float*>(alm_t + (size_t)chunk * BCM);
     const double* cf4 = d.coef + 4 * base;
-    // chunk staging: thread t holds 4 floats of alm row t / 8 and the (alpha, gamma) of row t;
-    // the next chunk is fetched into registers while the current one is processed
-    static_assert(LCH * 2 * BCM == 4 * 256, "one float4 per thread per chunk");
-    float4 pa;
+    // chunk staging: float4 e of thread t is q = t + 256 e = (row q / (2 NT * 4), float4 q % (8 NT));
+    // columns at or past ld read as zero; the next chunk is fetched while the current one runs
+    static_assert(LCH * RW == 4 * 256 * NT, "NT float4 per thread per chunk");
+    float4 pa[NT];
     double pal = 0.0, pga = 0.0;
     auto fetch = [&](int l0n) {
         const int nn = min(LCH, d.lmax + 1 - l0n);
-        const int li = tid >> 3;
-        pa = (li < nn) ? *reinterpret_cast<const float4*>(a + (size_t)(base + l0n + li) * 2 * ld + 4 * (tid & 7))
+#pragma unroll
+        for (int e = 0; e < NT; e++) {
+            const int q = tid + 256 * e, li = q / (8 * NT), c4 = q % (8 * NT);
+            const bool ok = li < nn && chunk * BCM + 2 * c4 < ld;
+            pa[e] = ok ? *reinterpret_cast<const float4*>(a + (size_t)(base + l0n + li) * 2 * ld + 4 * c4)
                        : float4{0.f, 0.f, 0.f, 0.f};
+        }
         if (tid < nn) {
             pal = cf4[4 * (l0n + tid)];
             pga = cf4[4 * (l0n + tid) + 2];
@@ -642,7 +651,8 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
     for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
         const int n = min(LCH, d.lmax + 1 - l0);
         __syncthreads();
-        *reinterpret_cast<float4*>(sa + 4 * tid) = pa;
+#pragma unroll
+        for (int e = 0; e < NT; e++) *reinterpret_cast<float4*>(sa + 4 * (tid + 256 * e)) = pa[e];
         if (tid < n) {
             sco[2 * tid] = pal;
             sco[2 * tid + 1] = pga;
@@ -664,10 +674,13 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
                 rec_fast(r, sco[2 * g + 6] * x, sco[2 * g + 7]);
                 const float aev = (float)(h ? v2 : v0);
                 const float aod = (float)(h ? v3 : v1);
-                const float bev = sa[(g + 2 * h) * 2 * BCM + li32];
-                const float bod = sa[(g + 1 + 2 * h) * 2 * BCM + li32];
-                Dev = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev, 0, 0, 0);
-                Dod = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod, 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < NT; c++) {
+                    const float bev = sa[(g + 2 * h) * RW + 32 * c + li32];
+                    const float bod = sa[(g + 1 + 2 * h) * RW + 32 * c + li32];
+                    Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
+                    Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
+                }
             }
             continue;
         }
@@ -692,30 +705,37 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
             const float aev = (float)(h ? v[2] : v[0]);
             const float aod = (float)(h ? v[3] : v[1]);
             const int le = g + 2 * h, lo = g + 1 + 2 * h;
-            const float bev = le < n ? sa[le * 2 * BCM + li32] : 0.f;
-            const float bod = lo < n ? sa[lo * 2 * BCM + li32] : 0.f;
-            Dev = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev, 0, 0, 0);
-            Dod = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod, 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < NT; c++) {
+                const float bev = le < n ? sa[le * RW + 32 * c + li32] : 0.f;
+                const float bod = lo < n ? sa[lo * RW + 32 * c + li32] : 0.f;
+                Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
+                Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
+            }
         }
     }
-    // D[i][j]: ring i of the wave, real column j = lane % 32 (complex column j / 2, part j & 1)
-    const int col = chunk * BCM + (li32 >> 1);
+    // D[i][j]: ring i of the wave, real column j = lane % 32 of tile c (complex column j / 2)
     const bool re = (li32 & 1) == 0;
 #pragma unroll
-    for (int vv = 0; vv < 16; vv++) {
-        const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
-        const int jr = rb * 128 + wave * 32 + i;
-        const float e = Dev[vv], o = Dod[vv];
-        float nv = e + o, sv = e - o;
-        const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
-        const cplx<float> w = rot[wave * 32 + i];
-        // (re + i im)(w.re + i w.im)
-        const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
-        const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
-        if (jr < d.npairs && col < ncol) {
-            const int rn = jr, rs = d.nrings - 1 - jr;
-            reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
-            if (rs != rn) reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+    for (int c = 0; c < NT; c++) {
+        const int col = chunk * BCM + 16 * c + (li32 >> 1);
+#pragma unroll
+        for (int vv = 0; vv < 16; vv++) {
+            const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+            const int jr = rb * 128 + wave * 32 + i;
+            const float e = Dev[c][vv], o = Dod[c][vv];
+            float nv = e + o, sv = e - o;
+            const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
+            const cplx<float> w = rot[wave * 32 + i];
+            // (re + i im)(w.re + i w.im)
+            const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+            const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+            if (jr < d.npairs && col < ncol) {
+                const int rn = jr, rs = d.nrings - 1 - jr;
+                reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+                if (rs != rn)
+                    reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+            }
         }
     }
 }
@@ -1959,11 +1979,24 @@ size_t sht_phase_bytes(const ShtPlan& p, int nmap) {
 // the batch (a single map uses BC = 1 instead of wasting the padding columns).
 static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
                                 hipStream_t st) {
+    // One tile of 16 complex columns per block.  Sharing the recursion over up to 5 tiles
+    // (NT = 5, 418 registers, one wave per SIMD) measured slower at Nside 512 / 80 maps
+    // (3.69 ms against 3.57 ms), and so did v_mfma_f32_32x32x1f32 with one ring per lane
+    // (no duplicated recursion, 4.57 ms): the kernel is bound by its VALU instruction stream
+    // (about 32 VALU instructions per MFMA) and latency, not by the recursion's arithmetic.
     ShtDev<float> d = dev_view<float>(p);
-    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 15) / 16;
+    const int NT = 1;
+    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * NT - 1) / (16 * NT);
     const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
-    hipLaunchKernelGGL(k_leg_synth_s0_mfma, dim3(nb), dim3(256), 0, st, d, ncol, ld, (const cplx<float>*)alm_t,
-                       (cplx<float>*)phase);
+    const cplx<float>* A = (const cplx<float>*)alm_t;
+    cplx<float>* P = (cplx<float>*)phase;
+    switch (NT) {
+        case 1: hipLaunchKernelGGL(k_leg_synth_s0_mfma<1>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
+        case 2: hipLaunchKernelGGL(k_leg_synth_s0_mfma<2>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
+        case 3: hipLaunchKernelGGL(k_leg_synth_s0_mfma<3>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
+        case 4: hipLaunchKernelGGL(k_leg_synth_s0_mfma<4>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
+        default: hipLaunchKernelGGL(k_leg_synth_s0_mfma<5>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
+    }
     PS_LAUNCH_CHECK();
     return 0;
 }

@@ -26,7 +26,8 @@ def relerr(x, ref):
 
 @pytest.mark.parametrize("nside,lmax,prec,nmaps", [
     (1, 2, 64, 1), (2, 5, 64, 3), (8, 23, 64, 2), (64, 128, 64, 10), (64, 191, 64, 1),
-    (8, 23, 32, 17), (64, 128, 32, 10), (128, 383, 32, 3), (256, 512, 32, 20)])
+    (8, 23, 32, 17), (64, 128, 32, 10), (128, 383, 32, 3), (256, 512, 32, 20), (64, 128, 32, 80),
+    (64, 191, 32, 100), (32, 64, 32, 33)])
 def test_spin0_synthesis_matches_oracle(gpu, nside, lmax, prec, nmaps):
     from polyspec_amd._abi import ShtPlan
     print(f"spin0 synthesis nside={nside} lmax={lmax} prec={prec} nmaps={nmaps}", flush=True)
