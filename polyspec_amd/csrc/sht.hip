@@ -1985,18 +1985,11 @@ static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     // (no duplicated recursion, 4.57 ms): the kernel is bound by its VALU instruction stream
     // (about 32 VALU instructions per MFMA) and latency, not by the recursion's arithmetic.
     ShtDev<float> d = dev_view<float>(p);
-    const int NT = 1;
+    constexpr int NT = 1;
     const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * NT - 1) / (16 * NT);
     const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
-    const cplx<float>* A = (const cplx<float>*)alm_t;
-    cplx<float>* P = (cplx<float>*)phase;
-    switch (NT) {
-        case 1: hipLaunchKernelGGL(k_leg_synth_s0_mfma<1>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
-        case 2: hipLaunchKernelGGL(k_leg_synth_s0_mfma<2>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
-        case 3: hipLaunchKernelGGL(k_leg_synth_s0_mfma<3>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
-        case 4: hipLaunchKernelGGL(k_leg_synth_s0_mfma<4>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
-        default: hipLaunchKernelGGL(k_leg_synth_s0_mfma<5>, dim3(nb), dim3(256), 0, st, d, ncol, ld, A, P); break;
-    }
+    hipLaunchKernelGGL(k_leg_synth_s0_mfma<NT>, dim3(nb), dim3(256), 0, st, d, ncol, ld,
+                       (const cplx<float>*)alm_t, (cplx<float>*)phase);
     PS_LAUNCH_CHECK();
     return 0;
 }
