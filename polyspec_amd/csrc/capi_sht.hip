@@ -52,6 +52,7 @@ int ps_sht_plan_create(int nside, int lmax, int mmax, int spin, int precision, p
     h->p.prec = precision / 8;
     hipGetDevice(&h->p.device);
     int rc = h->p.build();
+    if (!rc) rc = sht_mark_dead(h->p);
     if (rc) {
         delete h;
         return rc;

@@ -27,10 +27,15 @@ struct ShtPlan {
     double *d_l2s = nullptr, *d_l2sh = nullptr, *d_l2ch = nullptr, *d_seedL = nullptr, *d_seedS = nullptr;
     int *d_seedA = nullptr, *d_seedB = nullptr;
     void *d_cth = nullptr, *d_tw = nullptr, *d_blue = nullptr, *d_coef = nullptr;
+    // dead[m * npairs + jp] = 1: |lambda_lm| (both spin signs) < 2^-200 at ring pair jp for every l,
+    // so the pair contributes nothing at order m (the evanescent region of high m near the poles)
+    unsigned char* d_dead = nullptr;
     int build();
     ~ShtPlan();
 };
 
+// fills p.d_dead (one kernel on the null stream, synchronised); called once after build()
+int sht_mark_dead(ShtPlan& p);
 int sht_synth_lmajor_ld_multiple(const ShtPlan& p);
 size_t sht_phase_bytes(const ShtPlan& p, int nmap);
 int sht_adjoint_nsplit(const ShtPlan& p, int nmap);

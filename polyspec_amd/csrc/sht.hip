@@ -278,7 +278,7 @@ int ShtPlan::build() {
 
 ShtPlan::~ShtPlan() {
     void* ptrs[] = {d_rings, d_l2s, d_l2sh, d_l2ch, d_seedL, d_seedS, d_seedA,
-                    d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist};
+                    d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist, d_dead};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -296,6 +296,7 @@ struct ShtDev {
     const cplx<T>* tw;
     const cplx<T>* blue;
     const double* coef;  // (alpha, beta, gamma, 0) per idx(l, m)
+    const unsigned char* dead;  // (mmax+1) x npairs, see ShtPlan::d_dead
     long maxM;
 };
 
@@ -311,6 +312,7 @@ static ShtDev<T> dev_view(const ShtPlan& p) {
     d.npix = p.npix;
     d.nlm = p.nlm;
     d.rings = p.d_rings;
+    d.dead = p.d_dead;
     d.l2s = p.d_l2s;
     d.l2sh = p.d_l2sh;
     d.l2ch = p.d_l2ch;
@@ -578,6 +580,53 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
     }
 }
 
+// ------------------------------------------------------------ dead (m, ring) pairs
+// One thread per (m, ring pair): runs the recursion(s) of the synthesis kernels and records
+// whether the true magnitude |f| 2^(64 k) stays below 2^-200 for every l.  Such values are
+// exactly zero after the FP32 scaling (k <= -3) and below 1e-60 in FP64, so a block whose
+// ring pairs are all dead at its m may skip its l loop.
+__global__ void k_mark_dead(ShtDev<double> d, unsigned char* __restrict__ dead) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(d.mmax + 1) * d.npairs) return;
+    const int m = (int)(idx / d.npairs), jp = (int)(idx % d.npairs);
+    const int M1 = d.mmax + 1;
+    const double x = d.cth[jp];
+    Rec rp, rm;
+    rec_seed(rp, d.seedL[m] + d.seedA[m] * d.l2sh[jp] + d.seedB[m] * d.l2ch[jp], d.seedS[m]);
+    const bool sp = d.spin != 0;
+    if (sp)
+        rec_seed(rm, d.seedL[M1 + m] + d.seedA[M1 + m] * d.l2sh[jp] + d.seedB[M1 + m] * d.l2ch[jp], d.seedS[M1 + m]);
+    else
+        rm = rp;
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const int lstart = m > d.spin ? m : d.spin;
+    bool alive = false;
+    for (int l = lstart; l <= d.lmax && !alive; l++) {
+        const double lg = fmax(log2(fabs(rp.f)) + 64.0 * rp.k, log2(fabs(rm.f)) + 64.0 * rm.k);
+        alive = lg >= -200.0;
+        const double* c = d.coef + 4 * (base + l);
+        rec_step(rp, c[0] * (x + c[1]), c[2]);
+        if (sp) rec_step(rm, c[0] * (x - c[1]), c[2]);
+        else rm = rp;
+        if (((l - lstart) & 3) == 3) {
+            rec_rescale(rp);
+            if (sp) rec_rescale(rm);
+            else rm = rp;
+        }
+    }
+    dead[idx] = alive ? 0 : 1;
+}
+
+int sht_mark_dead(ShtPlan& p) {
+    const size_t n = (size_t)(p.mmax + 1) * p.npairs;
+    PS_CHECK_HIP(hipMalloc((void**)&p.d_dead, n));
+    ShtDev<double> d = dev_view<double>(p);
+    hipLaunchKernelGGL(k_mark_dead, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, d, p.d_dead);
+    PS_LAUNCH_CHECK();
+    PS_CHECK_HIP(hipStreamSynchronize(0));
+    return 0;
+}
+
 // ------------------------------------- Legendre synthesis, spin 0, FP32 matrix cores
 // The contraction F_m(ring, col) = sum_l lam_lm(ring) a_lm(col) of one m for 32 ring pairs
 // and 16 complex (32 real) columns runs on v_mfma_f32_32x32x2f32: A = lam (32 rings x 2
@@ -646,9 +695,11 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
             pga = cf4[4 * (l0n + tid) + 2];
         }
     };
-    fetch(m);
+    // a block whose ring pairs are all dead at this m writes zero phases without the l loop
+    const bool skip = __syncthreads_and(d.dead[(size_t)m * d.npairs + jpc]);
+    if (!skip) fetch(m);
     bool fast = false;
-    for (int l0 = m; l0 <= d.lmax; l0 += LCH) {
+    for (int l0 = m; l0 <= d.lmax && !skip; l0 += LCH) {
         const int n = min(LCH, d.lmax + 1 - l0);
         __syncthreads();
 #pragma unroll
@@ -804,9 +855,11 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int 
             pga = cf4[4 * (l0n + tid) + 2];
         }
     };
-    fetch(lstart);
+    // a block whose ring pairs are all dead at this m writes zero phases without the l loop
+    const bool skip = __syncthreads_and(d.dead[(size_t)m * d.npairs + jpc]);
+    if (!skip) fetch(lstart);
     bool fast = false;
-    for (int l0 = lstart; l0 <= d.lmax; l0 += LCH) {
+    for (int l0 = lstart; l0 <= d.lmax && !skip; l0 += LCH) {
         const int n = min(LCH, d.lmax + 1 - l0);
         __syncthreads();
         *reinterpret_cast<float4*>(sa + 4 * tid) = pa;
