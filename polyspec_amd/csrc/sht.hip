@@ -1174,11 +1174,13 @@ __device__ void fft_dit(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long 
     }
 }
 
-// LDS element index of entry i of a transform buffer: one pad slot per 32 entries keeps
-// bit-reversed reads and short butterfly strides off a single bank; buffers are
-// spaced fft_pitch(M) apart (odd pitch: the NB buffers of a fold start on distinct banks).
-__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
-__host__ __device__ __forceinline__ int fft_pitch(int M) { return M + (M >> 5) + 1; }
+// LDS element index of entry i of a transform buffer: bits 2..5 of i are XORed into bits
+// 0..3 (a permutation inside each aligned group of 16 entries, so no padding).  With
+// radix-4 butterflies a quarter-wave of 16 lanes then touches 32 distinct 4-byte banks in
+// every pass, the short-stride passes included (a pad slot per 32 entries left the last
+// passes 2-4 way conflicted).  Buffers are spaced fft_pitch(M) apart.
+__device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 2) & 15); }
+__host__ __device__ __forceinline__ int fft_pitch(int M) { return M + 1; }
 
 // Batched transforms over NB padded buffers of length M <= 256 * 4 * FFT_MAXQ.  Two
 // radix-2 stages are fused into one radix-2^2 pass in registers (half the LDS traffic,
@@ -1355,14 +1357,53 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
     __shared__ cplx<T> part[256];
 
     const int pitch = fft_pitch(M);
-    if (!pow2)  // Bluestein: the input is zero-padded from n to M
+    if (!pow2) {  // Bluestein: the input is zero-padded from n to M
         for (int i = tid; i < NB * pitch; i += 256) buf[i] = cplx<T>{T(0), T(0)};
+        __syncthreads();  // the fold below writes entries that other waves zero here
+    }
     // Fold, owner computes: item (j, q) sums m = j + k n (direct) and m = (n - j) + k n or,
     // for j = 0, n + k n (mirrored, conjugated), split over S slices of k when the ring
     // has fewer than 256 items; slices are added in a fixed order.
     const int nitem = n * NB;
-    const int S = nitem >= 256 ? 1 : 256 / nitem;
+    const bool nofold = n > d.mmax;  // no aliasing: one direct and one mirrored term at most
+    const int S = (nofold || nitem >= 256) ? 1 : 256 / nitem;
     const int step = S * n;
+    if (nofold) {
+        // no aliasing (n > mmax): item (j, q) takes m = j and the mirror m = n - j at most.
+        // Batches of FB items issue all their loads before any use, so the global-memory
+        // latency is paid once per batch rather than once per item.
+        constexpr int FB = 8;
+        for (int i0 = tid; i0 < nitem; i0 += 256 * FB) {
+            cplx<T> A[FB], B[FB], Am[FB], Bm[FB];
+#pragma unroll
+            for (int u = 0; u < FB; u++) {
+                const int i = i0 + 256 * u;
+                const int j = i / NB, q = i - (i / NB) * NB;
+                const bool in = i < nitem;
+                const bool hasA = in && c0 + 2 * q < ncol, hasB = in && c0 + 2 * q + 1 < ncol;
+                const cplx<T>* src = P + 2 * q;
+                const bool dir = j <= d.mmax, mir = j > 0 && n - j <= d.mmax;
+                const cplx<T> z{T(0), T(0)};
+                A[u] = (hasA && dir) ? src[(size_t)j * mstride] : z;
+                B[u] = (hasB && dir) ? src[(size_t)j * mstride + 1] : z;
+                Am[u] = (hasA && mir) ? src[(size_t)(n - j) * mstride] : z;
+                Bm[u] = (hasB && mir) ? src[(size_t)(n - j) * mstride + 1] : z;
+            }
+#pragma unroll
+            for (int u = 0; u < FB; u++) {
+                const int i = i0 + 256 * u;
+                if (i >= nitem) break;
+                const int j = i / NB, q = i - (i / NB) * NB;
+                cplx<T> a = A[u], b = B[u];
+                if (j == 0) a.im = b.im = T(0);
+                cplx<T> acc{a.re - b.im, a.im + b.re};  // A + i B
+                acc.re += Am[u].re + Bm[u].im;          // conj(A) + i conj(B) of the mirror
+                acc.im += Bm[u].re - Am[u].im;
+                if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));  // Bluestein pre-chirp
+                buf[q * pitch + pidx(j)] = acc;
+            }
+        }
+    } else
     for (int i = tid; i < nitem * S; i += 256) {
         const int sl = i / nitem, r = i - sl * nitem;
         const int j = r / NB, q = r - j * NB;
