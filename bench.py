@@ -33,7 +33,7 @@ METRIC = "gNL-loc numerators/s at Nside 512 (batched vs plain GPU); CG iters to 
 PEAK_FP32 = 157.3e12
 # HBM bytes of one k_leg_synth_s0_mfma launch (80 maps, Nside 512, lmax 1024): rocprofv3 FETCH_SIZE x 2
 # + WRITE_SIZE x 1 (factors from a 1 GiB copy), profiles/leg_synth_s0_mfma_nside512_lmax1024_80maps_pmc.md
-LEG_TRAFFIC = {(512, 1024, 80): (186470.8 * 2 + 1311359.4) * 1024}
+LEG_TRAFFIC = {(512, 1024, 80): (189299.2 * 2 + 1311359.4) * 1024}
 PEAK_HBM = 8.0e12
 PEAK_FP64 = 78.6e12
 
@@ -323,8 +323,9 @@ def bench_radial(base, raw, mask, tables, lmax, tolerance=1e-3, cpu_nodes=24):
     """gNL-loc radial-node optimizer (optimize_radial_sampling_1d) on the starting grid of
     the reference (reduce_r = 1): k-integrals on every node, the exact Fisher derivative on
     the device (FP64 matrix cores), greedy selection on the host.  Also times the device
-    Fisher derivative alone (wall clock around the call, table upload included) and the FP64
-    oracle on a bounded sample of the same tables (cpu_nodes nodes), per (r, r') pair."""
+    Fisher derivative alone (HIP events around its kernels; the wall clock with the host
+    set-up is reported beside it) and the FP64 oracle on a bounded sample of the same tables
+    (cpu_nodes nodes)."""
     import torch
     from oracle.radial import fisher_deriv_gnl_loc, mu_quadrature
     tm, _ = make_template(base, raw, mask, tables, lmax, True)
@@ -341,15 +342,18 @@ def bench_radial(base, raw, mask, tables, lmax, tolerance=1e-3, cpu_nodes=24):
     tm.plLXs, tm.qlXs = sg['plXs'][None], sg['qlXs']
     tm._fisher_deriv_gnl_loc(qw)  # warm-up
     torch.cuda.synchronize()
+    ev = []
     t0 = time.perf_counter()
-    D = tm._fisher_deriv_gnl_loc(qw)
+    D = tm._fisher_deriv_gnl_loc(qw, events=ev)
     torch.cuda.synchronize()
-    t_d = time.perf_counter() - t0
+    t_wall = time.perf_counter() - t0
+    t_d = ev[0][0].elapsed_time(ev[0][1]) * 1e-3  # device time of its kernels (zeta, GEMM, finish)
     nl = lmax - 1
     nmu = 2 * lmax + 1
     npair = n_start * (n_start + 1) / 2
     flops = 8. * nmu * nl * npair  # 4 tensors x (mu x l) x pairs, multiply-add = 2
     out["fisher_deriv_s"] = t_d
+    out["fisher_deriv_wall_s"] = t_wall  # with the host quadrature set-up and table uploads
     out["fisher_deriv_tflops"] = flops / t_d / 1e12
     out["fisher_deriv_frac_fp64_peak"] = flops / t_d / PEAK_FP64
     log("radial optimizer: %d -> %d nodes (score %.2e) in %.1fs; Fisher derivative %.3fs = %.1f TFLOP/s FP64"

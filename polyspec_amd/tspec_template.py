@@ -146,7 +146,7 @@ class TSpecTemplate:
         self._lmaj = {}
 
     # ------------------------------------------------- radial-node optimizer
-    def _fisher_deriv_gnl_loc(self, weights):
+    def _fisher_deriv_gnl_loc(self, weights, events=None):
         """Exact gNL-loc Fisher derivative D(r, r') on the device (fisher_deriv_gNL_loc,
         ideal_fisher.pyx:750-787) with the inputs of _compute_fisher_derivatives
         (tspec_template.py:3858-3870): 2 lmax + 1 Gauss-Legendre mu nodes, P_l(mu) for
@@ -172,8 +172,14 @@ class TSpecTemplate:
         nb = int(load().ps_fisher_deriv_work_bytes(nl, npol, nr, len(mus)))
         ws = torch.empty(nb, dtype=torch.uint8, device=dev)
         out = torch.empty((nr, nr), dtype=torch.float64, device=dev)
+        if events is not None:  # device time of the derivative's kernels, on torch's current stream
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         check(load().ps_fisher_deriv_gnl_loc(ptr(P), ptr(Q), nl, npol, nr, ptr(Cd), ptr(Ld), ptr(Wd), len(mus),
                                              ptr(wd), self.lmin, ptr(ws), nb, ptr(out), stream_handle()))
+        if events is not None:
+            e1.record()
+            events.append((e0, e1))
         D = out.cpu().numpy()
         del ws
         return D
