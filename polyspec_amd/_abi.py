@@ -149,8 +149,9 @@ class ShtPlan:
     def work_bytes(self, nmaps, lmajor):
         return int(load().ps_sht_workspace_bytes(self._h, int(nmaps), int(lmajor)))
 
-    def synthesis(self, alm, out=None):
-        """alm (nmaps, ncomp, nlm) complex -> maps (nmaps, ncomp, npix) real."""
+    def synthesis(self, alm, out=None, ws=None):
+        """alm (nmaps, ncomp, nlm) complex -> maps (nmaps, ncomp, npix) real.  ws: optional
+        uint8 device buffer of at least work_bytes(nmaps, 0) bytes (see adjoint)."""
         require_device(alm, self.cdtype, "alm")
         assert alm.dim() == 3 and alm.shape[1] == self.ncomp and alm.shape[2] == self.nlm, "Wrong alm shape"
         nm = alm.shape[0]
@@ -158,7 +159,9 @@ class ShtPlan:
             out = torch.empty((nm, self.ncomp, self.npix), dtype=self.rdtype, device=alm.device)
         require_device(out, self.rdtype, "map")
         wb = self.work_bytes(nm, 0)
-        ws = workspace(wb, alm.device)
+        if ws is None:
+            ws = workspace(wb, alm.device)
+        assert ws.numel() >= wb, "workspace too small"
         check(load().ps_sht_synthesis(self._h, nm, ptr(alm), ptr(out), ptr(ws), wb, stream_handle()))
         return out
 
@@ -195,8 +198,10 @@ class ShtPlan:
         check(load().ps_sht_ring_synthesis(self._h, int(nmaps), ptr(phase), ptr(out), stream_handle()))
         return out
 
-    def adjoint(self, maps, scale, out=None):
-        """maps (nmaps, ncomp, npix) real -> scale * adjoint (nmaps, ncomp, nlm) complex."""
+    def adjoint(self, maps, scale, out=None, ws=None):
+        """maps (nmaps, ncomp, npix) real -> scale * adjoint (nmaps, ncomp, nlm) complex.
+        ws: a uint8 device buffer of at least work_bytes(nmaps, 1) bytes to use instead of the
+        shared per-device workspace (needed when transforms run concurrently on other streams)."""
         require_device(maps, self.rdtype, "map")
         assert maps.dim() == 3 and maps.shape[1] == self.ncomp and maps.shape[2] == self.npix, "Wrong map shape"
         nm = maps.shape[0]
@@ -204,6 +209,8 @@ class ShtPlan:
             out = torch.empty((nm, self.ncomp, self.nlm), dtype=self.cdtype, device=maps.device)
         require_device(out, self.cdtype, "alm")
         wb = self.work_bytes(nm, 1)
-        ws = workspace(wb, maps.device)
+        if ws is None:
+            ws = workspace(wb, maps.device)
+        assert ws.numel() >= wb, "workspace too small"
         check(load().ps_sht_adjoint(self._h, nm, ptr(maps), ptr(out), float(scale), ptr(ws), wb, stream_handle()))
         return out

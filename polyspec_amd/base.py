@@ -167,11 +167,43 @@ class PolySpec:
         sc = np.pi / (3. * self.Nside ** 2)
         p0 = self.plan(lmax, 0)
         out = torch.empty((len(input_map), p0.nlm), dtype=self.cdtype, device=self.device)
-        p0.adjoint(m[0:1].reshape(1, 1, -1), sc, out=out[0:1].view(1, 1, -1))
         if len(input_map) == 3:
+            # T (spin 0) on a side stream with its own workspace, Q/U (spin 2) on the current
+            # stream: the two single-map adjoints are latency-bound and overlap
             p2 = self.plan(lmax, 2)
+            main = torch.cuda.current_stream(self.device)
+            side = self._side_stream()
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.wait_event(ready)
+            ws = self._side_workspace(p0.work_bytes(1, 1))
+            m.record_stream(side)  # used on the side stream: not reused before its work ends
+            out.record_stream(side)
+            with torch.cuda.stream(side):
+                p0.adjoint(m[0:1].reshape(1, 1, -1), sc, out=out[0:1].view(1, 1, -1), ws=ws)
+                done = torch.cuda.Event()
+                done.record(side)
             p2.adjoint(m[1:3].reshape(1, 2, -1), sc, out=out[1:3].view(1, 2, -1))
+            main.wait_event(done)
+        else:
+            p0.adjoint(m[0:1].reshape(1, 1, -1), sc, out=out[0:1].view(1, 1, -1))
         return self._ret(out, npy)
+
+    def _side_stream(self):
+        """A second stream (default priority) for transforms that overlap with the current one."""
+        if getattr(self, '_side', None) is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        return self._side
+
+    def _side_workspace(self, nbytes):
+        """Workspace of the side stream (the shared one belongs to the current stream)."""
+        buf = getattr(self, '_side_ws', None)
+        if buf is None or buf.numel() < nbytes:
+            self._side_ws = None
+            # allocated on the side stream so the caching allocator orders its reuse there
+            with torch.cuda.stream(self._side_stream()):
+                self._side_ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=self.device)
+        return self._side_ws
 
     @timer_func
     def to_map(self, input_lm, lmax=None):
@@ -184,9 +216,26 @@ class PolySpec:
         npy = not isinstance(input_lm, torch.Tensor)
         a = self._to_dev(input_lm, True)
         out = torch.empty((len(input_lm), self.Npix), dtype=self.rdtype, device=self.device)
-        self.plan(lmax, 0).synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1))
+        p0 = self.plan(lmax, 0)
         if len(input_lm) == 3:
-            self.plan(lmax, 2).synthesis(a[1:3].reshape(1, 2, -1), out=out[1:3].view(1, 2, -1))
+            # T on a side stream with its own workspace, E/B on the current stream (as in to_lm)
+            p2 = self.plan(lmax, 2)
+            main = torch.cuda.current_stream(self.device)
+            side = self._side_stream()
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.wait_event(ready)
+            ws = self._side_workspace(p0.work_bytes(1, 0))
+            a.record_stream(side)
+            out.record_stream(side)
+            with torch.cuda.stream(side):
+                p0.synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1), ws=ws)
+                done = torch.cuda.Event()
+                done.record(side)
+            p2.synthesis(a[1:3].reshape(1, 2, -1), out=out[1:3].view(1, 2, -1))
+            main.wait_event(done)
+        else:
+            p0.synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1))
         return self._ret(out, npy)
 
     @timer_func
