@@ -115,13 +115,13 @@ def build(nside, lmax, nr, precision, batch, nk=200):
     return base, raw, tot, (k, np.asarray([Tl, Tl]), r, w)
 
 
-def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None):
+def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None, graph=False):
     from polyspec_amd.tspec_template import TSpecTemplate
     from polyspec_amd.weighting import Weightings
     k, Tl, r, w = tables
     W = Weightings(base, mask, dict(raw))
     return TSpecTemplate(base, mask, W.applySinv_ideal, ['gNL-loc'], k, Tl, 2, lmax, r_values=r,
-                         r_weights={'gNL-loc': w}, batch_sims=batch), W
+                         r_weights={'gNL-loc': w}, batch_sims=batch, capture_graph=graph), W
 
 
 def lognormal_noise(base, mask, sigma_ln=0.5, fwhm_deg=5., seed=11, noise_uK_arcmin=30.):
@@ -428,6 +428,8 @@ def main():
     ap.add_argument("--learned-pairs", type=int, default=2000,
                     help="training pairs for the learned CG preconditioner (0 = plain CG only)")
     ap.add_argument("--no-bspec", action="store_true", help="skip the binned-bispectrum section")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the second timing of the batched numerator replayed from its captured HIP graph")
     ap.add_argument("--no-radial", action="store_true", help="skip the radial-node optimizer section")
     ap.add_argument("--bspec-nside", type=str, default="256",
                     help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
@@ -507,6 +509,24 @@ def main():
                      "algorithmic_flops_per_launch": flops},
         "mc_precompute_s": t_mc,
     }
+
+    if not args.no_graph:
+        # the same numerators replayed from a captured HIP graph (capture_graph=True)
+        tm32.capture_graph = True
+        for i in range(args.warmup + 1):
+            tm32.Tl_numerator(datas[i % ndata])  # first call: eager warm-up + capture
+        torch.cuda.synchronize()
+        gvals = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            gvals.append(tm32.Tl_numerator(datas[i % ndata])[0])
+        torch.cuda.synchronize()
+        dtg = time.perf_counter() - t0
+        tm32.capture_graph = False
+        result["graph_replay"] = {"numerators_per_s": args.steps / dtg, "ms_per_step": dtg / args.steps * 1e3,
+                                  "max_rel_diff_vs_eager": float(np.max(np.abs(np.array(gvals) - vals[:len(gvals)]) /
+                                                                        np.abs(vals[:len(gvals)])))}
+        log("graph replay: %.2f ms/numerator, %.1f numerators/s" % (dtg / args.steps * 1e3, args.steps / dtg))
 
     if args.copy_calib:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)

@@ -169,7 +169,8 @@ class PolySpec:
         out = torch.empty((len(input_map), p0.nlm), dtype=self.cdtype, device=self.device)
         if len(input_map) == 3:
             # T (spin 0) on a side stream with its own workspace, Q/U (spin 2) on the current
-            # stream: the two single-map adjoints are latency-bound and overlap
+            # stream: the two single-map adjoints are latency-bound and overlap.  Under graph
+            # capture the fork and join become graph dependencies.
             p2 = self.plan(lmax, 2)
             main = torch.cuda.current_stream(self.device)
             side = self._side_stream()
@@ -177,8 +178,9 @@ class PolySpec:
             ready.record(main)
             side.wait_event(ready)
             ws = self._side_workspace(p0.work_bytes(1, 1))
-            m.record_stream(side)  # used on the side stream: not reused before its work ends
-            out.record_stream(side)
+            if not torch.cuda.is_current_stream_capturing():
+                m.record_stream(side)  # used on the side stream: not reused before its work ends
+                out.record_stream(side)
             with torch.cuda.stream(side):
                 p0.adjoint(m[0:1].reshape(1, 1, -1), sc, out=out[0:1].view(1, 1, -1), ws=ws)
                 done = torch.cuda.Event()
@@ -226,8 +228,9 @@ class PolySpec:
             ready.record(main)
             side.wait_event(ready)
             ws = self._side_workspace(p0.work_bytes(1, 0))
-            a.record_stream(side)
-            out.record_stream(side)
+            if not torch.cuda.is_current_stream_capturing():
+                a.record_stream(side)
+                out.record_stream(side)
             with torch.cuda.stream(side):
                 p0.synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1), ws=ws)
                 done = torch.cuda.Event()

@@ -41,7 +41,7 @@ class TSpecTemplate:
     def __init__(self, base, mask, applySinv, templates, k_arr, Tl_arr, lmin, lmax, Lmin=None, Lmax=None,
                  Lmin_lens=None, Lmax_lens=None, ns=0.96, As=2.1e-9, k_pivot=0.05, r_values=[], r_weights={},
                  rtau_values=[], rtau_weights={}, C_phi=[], C_lens_weight={}, K_coll=0.1, k_coll=0.1, r_star=None,
-                 r_hor=None, batch_sims=True, sim_batch=None):
+                 r_hor=None, batch_sims=True, sim_batch=None, capture_graph=False):
         for t in templates:
             if t not in SUPPORTED:
                 raise NotImplementedError("Template %s is not provided by the hip backend (only %s)" % (t, SUPPORTED))
@@ -56,6 +56,12 @@ class TSpecTemplate:
         self.ns, self.As, self.k_pivot = ns, As, k_pivot
         self.batch_sims = batch_sims
         self.sim_batch = sim_batch
+        # capture_graph: with batch_sims, the device part of each numerator (S^-1, radial
+        # filtering, syntheses, fused sums) is captured once into a HIP graph and replayed;
+        # applySinv must then be capturable (no host synchronisation, e.g. applySinv_ideal)
+        self.capture_graph = capture_graph
+        self._graph = None
+        self._sims_version = 0
         self.kernel_events = None  # set to a list to time the Legendre stage of every filter call
         self.Pzeta = lambda k: _Pzeta(k, self.As, self.ns, self.k_pivot)
 
@@ -498,6 +504,7 @@ class TSpecTemplate:
                 self.t0_num[ii] += 1. / 24. * tau * 3 / self.N_it * FOURPI15
 
     def _ingest_sims(self, get_pair, N_pairs, preload, verb):
+        self._sims_version += 1  # invalidates a captured numerator graph
         self.N_it = N_pairs
         print("Using %d pairs of Monte Carlo simulations" % self.N_it)
         keep_maps = preload and (not self.batch_sims or 'tauNL-loc' in self.templates)
@@ -558,34 +565,16 @@ class TSpecTemplate:
                 "Data must contain T only!"
         compute_t0 = not (hasattr(self, 't0_num') and include_disconnected_term)
 
-        t0 = time.time()
-        h = self._h_dev(self.applySinv(data, input_type=input_type, lmax=self.lmax))
-        self.timers['Sinv'] += time.time() - t0
-        maps = self._filter([h])[0]
-        P, Q = maps[0], maps[1]
-
         nT = len(self.templates)
-        t4 = [None] * nT
         t2 = [0.] * nT
-        A_dd = None
         collapsed = include_disconnected_term and self.batch_sims
-        for ii, t in enumerate(self.templates):
-            if t == 'gNL-loc':
-                t1 = time.time()
-                c4 = 9. / 100. * 4. * self.base.A_pix * FOURPI15
-                if collapsed:
-                    c2 = -54. / 100. * self.base.A_pix * FOURPI15 / self.N_it
-                    t4[ii] = K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4, self.U_d, self.V_d, c2)
-                else:
-                    t4[ii] = K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4)
-                self.timers['gNL_summation'] += time.time() - t1
-            if t == 'tauNL-loc':
-                A_dd = self._A_maps(P, Q)
-                t4[ii] = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)
-                if collapsed:
-                    t4[ii] = t4[ii] + self._tau_sum(self.SA_d, A_dd, t, scale=-1. / 24. * 6. / self.N_it / 2. *
-                                                    FOURPI15 * 4.)
-        t4 = np.array([float(x) for x in t4])
+        graph_ok = (self.capture_graph and collapsed and input_type == 'map' and isinstance(data, torch.Tensor)
+                    and data.is_cuda)
+        if graph_ok:
+            t4_d, P, Q, A_dd = self._numerator_graph(data)
+        else:
+            t4_d, P, Q, A_dd = self._numerator_device(data, input_type, collapsed)
+        t4 = t4_d.cpu().numpy()
 
         if include_disconnected_term:
             t2 = np.zeros(nT)
@@ -633,6 +622,56 @@ class TSpecTemplate:
             out = t4
         self.timers['numerator'] += time.time() - t_start
         return out
+
+    def _numerator_device(self, data, input_type, collapsed):
+        """The data-dependent device work of a numerator: S^-1, radial filtering, syntheses and
+        the t4 sums (with the collapsed Monte-Carlo terms when `collapsed`).  Returns the (nT,)
+        FP64 tensor of t4 values and the maps P, Q and A_dd (tauNL-loc) that the remaining
+        terms use; no host synchronisation."""
+        t0 = time.time()
+        h = self._h_dev(self.applySinv(data, input_type=input_type, lmax=self.lmax))
+        self.timers['Sinv'] += time.time() - t0
+        maps = self._filter([h])[0]
+        P, Q = maps[0], maps[1]
+        t4 = []
+        A_dd = None
+        for t in self.templates:
+            if t == 'gNL-loc':
+                t1 = time.time()
+                c4 = 9. / 100. * 4. * self.base.A_pix * FOURPI15
+                if collapsed:
+                    c2 = -54. / 100. * self.base.A_pix * FOURPI15 / self.N_it
+                    t4.append(K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4, self.U_d, self.V_d, c2))
+                else:
+                    t4.append(K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4))
+                self.timers['gNL_summation'] += time.time() - t1
+            if t == 'tauNL-loc':
+                A_dd = self._A_maps(P, Q)
+                v = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)
+                if collapsed:
+                    v = v + self._tau_sum(self.SA_d, A_dd, t, scale=-1. / 24. * 6. / self.N_it / 2. *
+                                          FOURPI15 * 4.)
+                t4.append(v)
+        return torch.stack([x.reshape(()) for x in t4]), P, Q, A_dd
+
+    def _numerator_graph(self, data):
+        """_numerator_device through a captured HIP graph: captured on the first call for this
+        data shape and set of simulations (after one eager warm-up run that sizes every cached
+        workspace), then replayed with the data copied into the graph's input buffer."""
+        key = (tuple(data.shape), data.dtype, self._sims_version)
+        if self._graph is None or self._graph_key != key:
+            self._graph = None
+            self._numerator_device(data, 'map', True)  # warm-up: plans, workspaces, scratch
+            torch.cuda.synchronize()
+            self._g_in = data.detach().clone()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._g_out = self._numerator_device(self._g_in, 'map', True)
+            self._graph, self._graph_key = g, key
+        self._g_in.copy_(data)
+        self._graph.replay()
+        t4, P, Q, A_dd = self._g_out
+        return t4.clone(), P, Q, A_dd
 
     # --------------------------------------------------------------- Fisher
     def _transform_maps(self, maps, f, w, out, scale):

@@ -84,3 +84,39 @@ def test_fisher_matches_oracle(gpu, nside, lmax, pol, nr, templates, precision):
     ref = orc.fisher_from_grfs(a_maps)
     tol = 1e-8 if precision == 64 else 2e-4
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (got, ref)
+
+
+@pytest.mark.parametrize("pol,precision", [(True, 32), (True, 64), (False, 32)])
+def test_graph_numerator_matches_eager(gpu, pol, precision):
+    """capture_graph=True: the numerator's device work replayed from a HIP graph equals the eager
+    numerator on every replay (three data maps, each twice, eager work in between)."""
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.tspec_template import TSpecTemplate
+    from polyspec_amd.weighting import Weightings
+    print(f"graph numerator pol={pol} prec={precision}", flush=True)
+    nside, lmax, nr = 16, 24, 3
+    templates = ['gNL-loc', 'tauNL-loc']
+    raw, tot, beam = spectra(nside, pol)
+    rng = np.random.default_rng(31)
+    L = 3 * nside - 1
+    sims = [(grf_alm(rng, tot, lmax, pol), grf_alm(rng, tot, lmax, pol)) for _ in range(2)]
+    k, Tl = transfer(lmax, pol)
+    r, w = radial_nodes(nr)
+    tms = []
+    for graph in (False, True):
+        base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=precision)
+        mask = np.ones(12 * nside ** 2)
+        W = Weightings(base, mask, dict(raw))
+        tm = TSpecTemplate(base, mask, W.applySinv_ideal, templates, k, Tl, 2, lmax, r_values=r,
+                           r_weights={t: w for t in templates}, batch_sims=True, capture_graph=graph)
+        tm.load_sims(lambda ii: sims[ii], len(sims), input_type='harmonic')
+        tms.append(tm)
+    rdt = torch.float32 if precision == 32 else torch.float64
+    datas = [torch.from_numpy(oracle.to_map(grf_alm(rng, tot, L, pol), nside, L)).to(gpu, rdt).reshape(-1, 12 * nside ** 2)
+             for _ in range(3)]
+    for rep in range(2):
+        for d in datas:
+            want = tms[0].Tl_numerator(d)
+            got = tms[1].Tl_numerator(d)
+            assert np.allclose(got, want, rtol=1e-12, atol=0), (rep, got, want)
+    assert tms[1]._graph is not None
