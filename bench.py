@@ -42,6 +42,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def autotune_report(base):
+    """Launch shapes the autotuner kept for each SHT plan and batch width, with the kept and
+    built-in shapes' milliseconds per launch of every stage."""
+    out = {}
+    for (lmax, spin), plan in base._plans.items():
+        for nmaps, times in plan._tuned.items():
+            if times is None:
+                continue
+            out["lmax%d_spin%d_%dmaps" % (lmax, spin, nmaps)] = {
+                "shapes": dict(zip(plan.SHAPE_STAGES, plan.shapes(nmaps))),
+                "ms_kept_vs_builtin": {k: [round(v[0], 4), round(v[1], 4)] for k, v in times.items()}}
+    return out
+
+
 def spectra(nside, fwhm=30., noise=30.):
     L = 3 * nside - 1
     l = np.arange(L + 1)
@@ -508,6 +522,7 @@ def main():
                                "dense FP32 matrix peak)" % (ncol, leg_ms),
                      "algorithmic_flops_per_launch": flops},
         "mc_precompute_s": t_mc,
+        "autotune": autotune_report(base32),
     }
 
     if not args.no_graph:

@@ -81,6 +81,27 @@ int ps_sht_ring_synthesis(const ps_sht_plan *plan, int nmaps, const void *phase,
 int ps_sht_adjoint(const ps_sht_plan *plan, int nmaps, const void *map, void *alm, double scale, void *work,
                    long long work_bytes, void *stream);
 
+/* Launch-shape tuning (the plan's stages: 0 Legendre synthesis, 1 ring FFTs of power-of-two
+ * rings, 2 ring FFTs of Bluestein rings, 3 Legendre adjoint).  No reference counterpart: ducc0
+ * picks its own blocking inside ducc0.sht (base.py:172-176, 193 call it), so these tune what
+ * the transforms above launch for one batch width nmaps*ncomp; results of every shape agree
+ * with the built-in choice to rounding.  Shapes are stored in the plan: set or tune them
+ * before, not during, transforms on other threads.
+ *
+ * candidates: the shape ids of `stage` (count returned; at most max_ids written).
+ * get_shapes: shapes[4] in use for nmaps (tuned or set: returns 1; built-in rule: 0).
+ * set_shapes: shapes[4], -1 keeps the built-in rule for a stage.
+ * autotune: times every candidate (one untimed launch, then `reps` launches between HIP events)
+ * on synthetic data in `work` (ps_sht_autotune_work_bytes bytes) and keeps the fastest when it
+ * beats the built-in choice by 3%; synchronises `stream` (not capturable).  times (optional,
+ * 8 doubles): per stage, ms per launch of the kept and of the built-in shape. */
+int ps_sht_shape_candidates(const ps_sht_plan *plan, int nmaps, int stage, int *ids, int max_ids);
+int ps_sht_get_shapes(const ps_sht_plan *plan, int nmaps, int *shapes);
+int ps_sht_set_shapes(ps_sht_plan *plan, int nmaps, const int *shapes);
+long long ps_sht_autotune_work_bytes(const ps_sht_plan *plan, int nmaps);
+int ps_sht_autotune(ps_sht_plan *plan, int nmaps, void *work, long long work_bytes, int reps, double *times,
+                    void *stream);
+
 
 /* ------------------------------------------------------- radial kernels
  * n = 0 loop nests of polyspec/cython/tauNL_utils.pyx and ideal_fisher.pyx.

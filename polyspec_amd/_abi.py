@@ -28,6 +28,11 @@ SIGNATURES = {
     "ps_sht_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P, _LL, _P]),
     "ps_sht_adjoint": (_I, [_P, _I, _P, _P, _D, _P, _LL, _P]),
     "ps_sht_phase_bytes": (_I, [_P, _I, ctypes.POINTER(_LL)]),
+    "ps_sht_shape_candidates": (_I, [_P, _I, _I, ctypes.POINTER(_I), _I]),
+    "ps_sht_get_shapes": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "ps_sht_set_shapes": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "ps_sht_autotune_work_bytes": (_LL, [_P, _I]),
+    "ps_sht_autotune": (_I, [_P, _I, _P, _LL, _I, ctypes.POINTER(_D), _P]),
     "ps_sht_legendre_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P]),
     "ps_sht_ring_synthesis": (_I, [_P, _I, _P, _P, _P]),
     "ps_reduce_scratch_bytes": (_LL, []),
@@ -124,10 +129,19 @@ def workspace(nbytes, device):
 
 
 class ShtPlan:
-    """Immutable HEALPix SHT plan (nside, lmax, spin, precision) on the current device."""
+    """HEALPix SHT plan (nside, lmax, spin, precision) on the current device.
 
-    def __init__(self, nside, lmax, spin=0, precision=64):
+    The plan also holds the launch shapes of its four stages per batch width (SHAPE_STAGES):
+    the built-in rule unless set_shapes() or autotune() chose others.  With autotune=True the
+    first transform of each batch width outside graph capture times every candidate first
+    (autotune(); it synchronises the stream once)."""
+
+    SHAPE_STAGES = ("synthesis", "ring_pow2", "ring_bluestein", "adjoint")
+
+    def __init__(self, nside, lmax, spin=0, precision=64, autotune=False):
         lib = load()
+        self.autotune_on_first_use = bool(autotune)
+        self._tuned = {}
         self.nside, self.lmax, self.spin, self.precision = int(nside), int(lmax), int(spin), int(precision)
         h = ctypes.c_void_p()
         check(lib.ps_sht_plan_create(self.nside, self.lmax, self.lmax, self.spin, self.precision, ctypes.byref(h)))
@@ -149,12 +163,54 @@ class ShtPlan:
     def work_bytes(self, nmaps, lmajor):
         return int(load().ps_sht_workspace_bytes(self._h, int(nmaps), int(lmajor)))
 
+    # ------------------------------------------------------------ launch shapes
+    def shape_candidates(self, nmaps, stage):
+        """Shape ids of one stage (index or name in SHAPE_STAGES) for a batch of nmaps maps."""
+        st = self.SHAPE_STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        lib = load()
+        n = lib.ps_sht_shape_candidates(self._h, int(nmaps), st, None, 0)
+        check(min(n, 0))
+        ids = (_I * max(n, 1))()
+        check(min(lib.ps_sht_shape_candidates(self._h, int(nmaps), st, ids, n), 0))
+        return list(ids[:n])
+
+    def shapes(self, nmaps):
+        """Shapes in use for nmaps maps, one id per stage."""
+        out = (_I * 4)()
+        check(min(load().ps_sht_get_shapes(self._h, int(nmaps), out), 0))
+        return list(out)
+
+    def set_shapes(self, nmaps, shapes):
+        """Fix the shapes for nmaps maps (a list of 4 ids, -1 = built-in rule)."""
+        arr = (_I * 4)(*[int(x) for x in shapes])
+        check(load().ps_sht_set_shapes(self._h, int(nmaps), arr))
+        self._tuned[int(nmaps)] = None
+
+    def autotune(self, nmaps, reps=3):
+        """Time every candidate shape of every stage for nmaps maps and keep the fastest (when
+        3% faster than the built-in one).  Returns {stage: (kept ms, built-in ms)} per launch."""
+        lib = load()
+        nb = int(lib.ps_sht_autotune_work_bytes(self._h, int(nmaps)))
+        check(min(nb, 0))
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.device)
+        times = (_D * 8)()
+        check(lib.ps_sht_autotune(self._h, int(nmaps), ptr(ws), nb, int(reps), times, stream_handle()))
+        res = {name: (times[2 * i], times[2 * i + 1]) for i, name in enumerate(self.SHAPE_STAGES)}
+        self._tuned[int(nmaps)] = res
+        return res
+
+    def _maybe_tune(self, nmaps):
+        if self.autotune_on_first_use and int(nmaps) not in self._tuned and \
+                not torch.cuda.is_current_stream_capturing():
+            self.autotune(nmaps)
+
     def synthesis(self, alm, out=None, ws=None):
         """alm (nmaps, ncomp, nlm) complex -> maps (nmaps, ncomp, npix) real.  ws: optional
         uint8 device buffer of at least work_bytes(nmaps, 0) bytes (see adjoint)."""
         require_device(alm, self.cdtype, "alm")
         assert alm.dim() == 3 and alm.shape[1] == self.ncomp and alm.shape[2] == self.nlm, "Wrong alm shape"
         nm = alm.shape[0]
+        self._maybe_tune(nm)
         if out is None:
             out = torch.empty((nm, self.ncomp, self.npix), dtype=self.rdtype, device=alm.device)
         require_device(out, self.rdtype, "map")
@@ -170,6 +226,7 @@ class ShtPlan:
         require_device(alm_t, self.cdtype, "alm_t")
         assert alm_t.dim() == 2 and alm_t.shape[0] == self.nlm
         ld = alm_t.shape[1]
+        self._maybe_tune(nmaps)
         if out is None:
             out = torch.empty((nmaps, self.ncomp, self.npix), dtype=self.rdtype, device=alm_t.device)
         require_device(out, self.rdtype, "map")
@@ -187,12 +244,14 @@ class ShtPlan:
     def legendre_synthesis_lmajor(self, alm_t, nmaps, phase):
         """Legendre stage only: alm_t (nlm, ld) -> ring phases (uint8 buffer of phase_buffer())."""
         require_device(alm_t, self.cdtype, "alm_t")
+        self._maybe_tune(nmaps)
         check(load().ps_sht_legendre_synthesis_lmajor(self._h, int(nmaps), ptr(alm_t), alm_t.shape[1], ptr(phase),
                                                       stream_handle()))
         return phase
 
     def ring_synthesis(self, phase, nmaps, out=None):
         """Ring-FFT stage only: phases -> maps (nmaps, ncomp, npix)."""
+        self._maybe_tune(nmaps)
         if out is None:
             out = torch.empty((nmaps, self.ncomp, self.npix), dtype=self.rdtype, device=phase.device)
         check(load().ps_sht_ring_synthesis(self._h, int(nmaps), ptr(phase), ptr(out), stream_handle()))
@@ -205,6 +264,7 @@ class ShtPlan:
         require_device(maps, self.rdtype, "map")
         assert maps.dim() == 3 and maps.shape[1] == self.ncomp and maps.shape[2] == self.npix, "Wrong map shape"
         nm = maps.shape[0]
+        self._maybe_tune(nm)
         if out is None:
             out = torch.empty((nm, self.ncomp, self.nlm), dtype=self.cdtype, device=maps.device)
         require_device(out, self.cdtype, "alm")

@@ -36,11 +36,15 @@ class PolySpec:
     """Base class for PolySpec (polyspec/base.py:9-117) with the HIP SHT backend.
 
     Inputs: Nside, Cl_tot (dict of fiducial total spectra), beam, pol, backend,
-    nthreads -- as in the reference -- plus ``precision`` (64 | 32) and
-    ``device`` (HIP device index) for the GPU backend.
+    nthreads -- as in the reference -- plus ``precision`` (64 | 32),
+    ``device`` (HIP device index) for the GPU backend and ``autotune``: time the
+    launch-shape candidates of each SHT plan on its first transform of a batch width
+    and keep the fastest (ShtPlan.autotune; results agree with the built-in shapes to
+    rounding).
     """
 
-    def __init__(self, Nside, Cl_tot, beam=[], pol=True, backend='hip', nthreads=0, precision=64, device=None):
+    def __init__(self, Nside, Cl_tot, beam=[], pol=True, backend='hip', nthreads=0, precision=64, device=None,
+                 autotune=True):
         if backend != 'hip':
             raise Exception("This package implements the 'hip' backend only ('healpix' and 'ducc' "
                             "are provided by the reference package)")
@@ -48,6 +52,7 @@ class PolySpec:
         self.pol = pol
         self.backend = backend
         self.precision = int(precision)
+        self.autotune = bool(autotune)
         assert self.precision in (32, 64), "precision must be 32 or 64"
         if not torch.cuda.is_available():
             raise RuntimeError("backend='hip' needs a visible MI355X (HIP) device")
@@ -133,7 +138,7 @@ class PolySpec:
         p = self._plans.get(key)
         if p is None:
             with torch.cuda.device(self.device):
-                p = ShtPlan(self.Nside, lmax, spin, self.precision)
+                p = ShtPlan(self.Nside, lmax, spin, self.precision, autotune=self.autotune)
             self._plans[key] = p
         return p
 

@@ -156,4 +156,37 @@ int ps_sht_adjoint(const ps_sht_plan* plan, int nmaps, const void* map, void* al
     return sht_adjoint(p, nmaps, map, p.npix, alm, p.nlm, scale, work, (hipStream_t)stream);
 }
 
+int ps_sht_shape_candidates(const ps_sht_plan* plan, int nmaps, int stage, int* ids, int max_ids) {
+    PS_REQUIRE(plan && nmaps >= 1, "ps_sht_shape_candidates: bad argument");
+    PS_REQUIRE(stage >= 0 && stage < SHT_NSTAGE, "ps_sht_shape_candidates: stage must be 0..3");
+    return sht_shape_candidates(plan->p, nmaps, stage, ids, ids ? max_ids : 0);
+}
+
+int ps_sht_get_shapes(const ps_sht_plan* plan, int nmaps, int* shapes) {
+    PS_REQUIRE(plan && shapes && nmaps >= 1, "ps_sht_get_shapes: bad argument");
+    const ShtShapes* s = plan->p.shapes_for(nmaps * plan->p.ncomp);
+    for (int st = 0; st < SHT_NSTAGE; st++) {
+        const int id = s ? s->id[st] : -1;
+        shapes[st] = id >= 0 ? id : sht_default_shape(plan->p, nmaps, st);
+    }
+    return s ? 1 : 0;
+}
+
+int ps_sht_set_shapes(ps_sht_plan* plan, int nmaps, const int* shapes) {
+    PS_REQUIRE(plan && shapes && nmaps >= 1, "ps_sht_set_shapes: bad argument");
+    return sht_set_shapes(plan->p, nmaps, shapes);
+}
+
+long long ps_sht_autotune_work_bytes(const ps_sht_plan* plan, int nmaps) {
+    if (!plan || nmaps < 1) return -1;
+    return (long long)sht_autotune_work_bytes(plan->p, nmaps);
+}
+
+int ps_sht_autotune(ps_sht_plan* plan, int nmaps, void* work, long long work_bytes, int reps, double* times,
+                    void* stream) {
+    PS_REQUIRE(plan && work && nmaps >= 1, "ps_sht_autotune: bad argument");
+    PS_REQUIRE(work_bytes >= 0, "ps_sht_autotune: negative workspace size");
+    return sht_autotune(plan->p, nmaps, work, (size_t)work_bytes, reps, (hipStream_t)stream, times);
+}
+
 }  // extern "C"

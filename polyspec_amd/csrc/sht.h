@@ -15,6 +15,15 @@ struct RingInfo {
     int voff;    // offset of the Bluestein kernel in the table, -1 for power-of-two rings
 };
 
+// Launch shapes for one batch width (ncol = nmaps * ncomp), chosen by timing in sht_autotune
+// or set by the caller; -1 keeps the built-in rule for that stage.
+enum { SHT_STAGE_SYNTH = 0, SHT_STAGE_RING_POW2 = 1, SHT_STAGE_RING_BLUE = 2, SHT_STAGE_ADJ = 3, SHT_NSTAGE = 4 };
+struct ShtShapes {
+    int ncol = 0;
+    int id[SHT_NSTAGE] = {-1, -1, -1, -1};
+};
+constexpr int SHT_MAX_SHAPES = 16;
+
 struct ShtPlan {
     int nside = 0, lmax = 0, mmax = 0, spin = 0, prec = 4;  // prec: bytes per real (4 or 8)
     int nrings = 0, npairs = 0, ncomp = 1;
@@ -30,6 +39,14 @@ struct ShtPlan {
     // dead[m * npairs + jp] = 1: |lambda_lm| (both spin signs) < 2^-200 at ring pair jp for every l,
     // so the pair contributes nothing at order m (the evanescent region of high m near the poles)
     unsigned char* d_dead = nullptr;
+    // per-batch-width launch shapes (not thread-safe against concurrent transforms on this plan)
+    ShtShapes shapes[SHT_MAX_SHAPES];
+    int nshapes = 0;
+    const ShtShapes* shapes_for(int ncol) const {
+        for (int i = 0; i < nshapes; i++)
+            if (shapes[i].ncol == ncol) return &shapes[i];
+        return nullptr;
+    }
     int build();
     ~ShtPlan();
 };
@@ -50,5 +67,20 @@ int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stri
 // work: sht_adjoint_work_bytes(p, nmap) bytes (ring phases, then ring-slice partial sums)
 int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, void* alm, long alm_stride,
                 double scale, void* work, hipStream_t st);
+
+// Launch-shape candidates of one stage for nmap maps (ids usable by sht_set_shapes); returns
+// their count, writing at most `max` ids.  The built-in rule's choice is reported by
+// sht_default_shape.
+int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int max);
+int sht_default_shape(const ShtPlan& p, int nmap, int stage);
+// ids[SHT_NSTAGE]; -1 entries keep the built-in rule
+int sht_set_shapes(ShtPlan& p, int nmap, const int* ids);
+size_t sht_autotune_work_bytes(const ShtPlan& p, int nmap);
+// Times every candidate of every stage on synthetic data in `work` (reps launches each, after
+// one untimed launch) and keeps the fastest; a candidate replaces the built-in choice only when
+// it is at least 3% faster.  Synchronises the stream.  times (optional): per stage, the
+// winning and the built-in shape's milliseconds per launch.
+int sht_autotune(ShtPlan& p, int nmap, void* work, size_t work_bytes, int reps, hipStream_t st,
+                 double* times);
 
 }  // namespace ps

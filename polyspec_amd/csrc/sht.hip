@@ -1919,29 +1919,47 @@ static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, lo
     return 0;
 }
 
+// Ring-synthesis shape: rings per block (NB FFT buffers of two real columns each).  The
+// built-in rule fills 64 KB of LDS with as many buffers as the batch can use.
+static int ring_nb_rule(long maxM, int ncol, int prec) {
+    const long per = maxM * 2L * prec;
+    if (ncol >= 8 && 4 * per <= 65536) return 4;
+    if (ncol >= 4 && 2 * per <= 65536) return 2;
+    return 1;
+}
+
+static bool ring_nb_valid(long maxM, int ncol, int prec, int nb) {
+    const long per = maxM * 2L * prec;
+    // one buffer always (beyond 64 KB it takes the dynamic-LDS attribute); more only within 64 KB
+    return nb == 1 || ((nb == 2 || nb == 4) && nb * per <= 65536 && ncol > nb);
+}
+
 template <typename T>
 static int launch_ring_synth_list(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
-                                  const void* phase, void* map, long map_stride, hipStream_t st) {
+                                  const void* phase, void* map, long map_stride, hipStream_t st, int nb) {
     if (nlist == 0) return 0;
     if (maxM > 4L * 256 * FFT_MAXQ) {
         set_error("ring synthesis: FFT length above 8192 is not supported");
         return -1;
     }
-    const long per = maxM * (long)sizeof(cplx<T>);
-    if (ncol >= 8 && 4 * per <= 65536)
-        return launch_ring_synth_nb<T, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
-    if (ncol >= 4 && 2 * per <= 65536)
-        return launch_ring_synth_nb<T, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    if (nb < 0) nb = ring_nb_rule(maxM, ncol, (int)sizeof(T));
+    if (!ring_nb_valid(maxM, ncol, (int)sizeof(T), nb)) {
+        set_error("ring synthesis: invalid launch shape");
+        return -1;
+    }
+    if (nb == 4) return launch_ring_synth_nb<T, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    if (nb == 2) return launch_ring_synth_nb<T, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
     return launch_ring_synth_nb<T, 1>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
 }
 
 template <typename T>
 static int launch_ring_synth(const ShtPlan& p, int ncol, const void* phase, void* map, long map_stride,
-                             hipStream_t st) {
-    int rc = launch_ring_synth_list<T>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st);
+                             hipStream_t st, int nb_pow2, int nb_blue) {
+    int rc = launch_ring_synth_list<T>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st,
+                                       nb_pow2);
     if (rc) return rc;
     return launch_ring_synth_list<T>(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol, phase, map,
-                                     map_stride, st);
+                                     map_stride, st, nb_blue);
 }
 
 template <typename T>
@@ -1997,44 +2015,72 @@ static int launch_adj_cfg(const ShtPlan& p, int ncol, const void* phase, void* a
     return 0;
 }
 
-// Latency shape (few columns, ring slices): up to 8 ring pairs per thread, one column
-// chunk of 2 (spin 0) or 1 map (spin s) per block.
-template <typename T>
-static int launch_adj_latency(const ShtPlan& p, int ncol, const void* phase, void* dst, long dstride,
-                              double scale, int ns, int pps, hipStream_t st) {
-    if (p.spin == 0) {
-        constexpr int BC = 2;
-        if (pps <= 64) return launch_adj_cfg<T, 1, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-        if (pps <= 128) return launch_adj_cfg<T, 2, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-        if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-        if (pps <= 512) return launch_adj_cfg<T, 8, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-        if (pps <= 1024) return launch_adj_cfg<T, 8, 2, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-        return launch_adj_cfg<T, 8, 4, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    }
-    constexpr int BC = 1;
-    if (pps <= 64) return launch_adj_cfg<T, 1, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    if (pps <= 128) return launch_adj_cfg<T, 2, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    if (pps <= 512) return launch_adj_cfg<T, 8, 1, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    if (pps <= 1024) return launch_adj_cfg<T, 8, 2, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
-    return launch_adj_cfg<T, 8, 4, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+// Adjoint Legendre shapes: NRT ring pairs per thread, NW waves per block, BC batch columns
+// (spin 0) or maps (spin s) per block; a block covers 64 NRT NW ring pairs of its ring slice.
+struct AdjShape {
+    int nrt, nw, bc;
+    bool s0, sp;  // instantiated for spin 0 / spin s
+};
+static constexpr AdjShape ADJ_SHAPES[] = {
+    {1, 1, 2, true, false}, {2, 1, 2, true, false}, {4, 1, 2, true, true},  {8, 1, 2, true, false},
+    {8, 2, 2, true, false}, {8, 4, 2, true, false}, {4, 2, 2, true, true},  {4, 4, 2, true, true},
+    {4, 8, 2, true, true},  {1, 1, 1, false, true}, {2, 1, 1, false, true}, {4, 1, 1, false, true},
+    {8, 1, 1, false, true}, {8, 2, 1, false, true}, {8, 4, 1, false, true}};
+static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES[0]));
+
+static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
+    if (id < 0 || id >= N_ADJ_SHAPES) return false;
+    const AdjShape& a = ADJ_SHAPES[id];
+    return (p.spin == 0 ? a.s0 : a.sp) && 64 * a.nrt * a.nw >= pps;
 }
 
-// Throughput shape (many columns): 4 ring pairs per thread and more waves per block,
-// wider column chunks.
+// Built-in rule.  Latency (few columns, ring slices): up to 8 ring pairs per thread, one column
+// chunk of 2 (spin 0) or 1 map (spin s) per block.  Throughput (many columns): 4 ring pairs per
+// thread and more waves per block.
+static int adj_shape_rule(const ShtPlan& p, bool many, int pps) {
+    const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
+    if (many) {
+        static constexpr int thr[6] = {2, 2, 2, 6, 7, 8};
+        return thr[lvl];
+    }
+    static constexpr int lat_s0[6] = {0, 1, 2, 3, 4, 5}, lat_sp[6] = {9, 10, 11, 12, 13, 14};
+    return p.spin == 0 ? lat_s0[lvl] : lat_sp[lvl];
+}
+
 template <typename T>
-static int launch_adj_throughput(const ShtPlan& p, int ncol, const void* phase, void* dst, long dstride,
-                                 double scale, int pps, hipStream_t st) {
-    constexpr int BC = 2;
-    if (pps <= 256) return launch_adj_cfg<T, 4, 1, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
-    if (pps <= 512) return launch_adj_cfg<T, 4, 2, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
-    if (pps <= 1024) return launch_adj_cfg<T, 4, 4, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
-    return launch_adj_cfg<T, 4, 8, BC>(p, ncol, phase, dst, dstride, scale, 1, pps, st);
+static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phase, void* dst, long dstride,
+                            double scale, int ns, int pps, hipStream_t st) {
+    if (!adj_shape_valid(p, id, pps)) {
+        set_error("adjoint: invalid launch shape");
+        return -1;
+    }
+#define PS_ADJ_CASE(I, NRT, NW, BC) \
+    case I: return launch_adj_cfg<T, NRT, NW, BC>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    switch (id) {
+        PS_ADJ_CASE(0, 1, 1, 2)
+        PS_ADJ_CASE(1, 2, 1, 2)
+        PS_ADJ_CASE(2, 4, 1, 2)
+        PS_ADJ_CASE(3, 8, 1, 2)
+        PS_ADJ_CASE(4, 8, 2, 2)
+        PS_ADJ_CASE(5, 8, 4, 2)
+        PS_ADJ_CASE(6, 4, 2, 2)
+        PS_ADJ_CASE(7, 4, 4, 2)
+        PS_ADJ_CASE(8, 4, 8, 2)
+        PS_ADJ_CASE(9, 1, 1, 1)
+        PS_ADJ_CASE(10, 2, 1, 1)
+        PS_ADJ_CASE(11, 4, 1, 1)
+        PS_ADJ_CASE(12, 8, 1, 1)
+        PS_ADJ_CASE(13, 8, 2, 1)
+        PS_ADJ_CASE(14, 8, 4, 1)
+    }
+#undef PS_ADJ_CASE
+    set_error("adjoint: invalid launch shape");
+    return -1;
 }
 
 template <typename T>
 static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride, double scale,
-                      void* part, hipStream_t st) {
+                      void* part, hipStream_t st, int shape) {
     if (p.npairs > 2048) {
         set_error("adjoint: nside above 1024 is not supported");
         return -1;
@@ -2045,8 +2091,8 @@ static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, 
     void* dst = ns > 1 ? part : alm;
     const long dstride = ns > 1 ? p.nlm : alm_stride;
     const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
-    int rc = many ? launch_adj_throughput<T>(p, ncol, phase, dst, dstride, scale, pps, st)
-                  : launch_adj_latency<T>(p, ncol, phase, dst, dstride, scale, ns, pps, st);
+    if (shape < 0) shape = adj_shape_rule(p, many, pps);
+    int rc = launch_adj_shape<T>(p, shape, ncol, phase, dst, dstride, scale, ns, pps, st);
     if (rc || ns == 1) return rc;
     const long n = (long)ncol * p.nlm;
     hipLaunchKernelGGL(k_adj_sum_splits<T>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, st,
@@ -2088,15 +2134,39 @@ static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     return 0;
 }
 
+// Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32 only), otherwise the batch
+// columns (spin 0) or maps (spin s) per block of the vector kernel.
 template <typename T>
-static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st) {
-    constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
-    if (sizeof(T) == 4 && ncol > 8) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
-    if (ncol <= 1) return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
-    if (ncol <= 2) return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
-    if (ncol <= 4) return launch_synth_s0<T, 4>(p, ncol, ld, alm_t, phase, st);
-    if (BCMAX > 8 && ncol <= 8) return launch_synth_s0<T, 8>(p, ncol, ld, alm_t, phase, st);
-    return launch_synth_s0<T, BCMAX>(p, ncol, ld, alm_t, phase, st);
+static int synth_shape_rule(const ShtPlan& p, int ncol) {
+    if (p.spin == 0) {
+        constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
+        if (sizeof(T) == 4 && ncol > 8) return 0;
+        return ncol <= 1 ? 1 : ncol <= 2 ? 2 : ncol <= 4 ? 4 : (BCMAX > 8 && ncol <= 8) ? 8 : BCMAX;
+    }
+    constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
+    const int nmap = ncol / p.ncomp;
+    if (sizeof(T) == 4 && nmap > 2) return 0;
+    return nmap <= 1 ? 1 : (BCMAX > 2 && nmap <= 2) ? 2 : BCMAX;
+}
+
+template <typename T>
+static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st,
+                               int shape) {
+    if (shape < 0) shape = synth_shape_rule<T>(p, ncol);
+    switch (shape) {
+        case 0:
+            if constexpr (sizeof(T) == 4) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
+            break;
+        case 1: return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
+        case 2: return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
+        case 4: return launch_synth_s0<T, 4>(p, ncol, ld, alm_t, phase, st);
+        case 8: return launch_synth_s0<T, 8>(p, ncol, ld, alm_t, phase, st);
+        case 16:
+            if constexpr (sizeof(T) == 4) return launch_synth_s0<T, 16>(p, ncol, ld, alm_t, phase, st);
+            break;
+    }
+    set_error("synthesis: invalid launch shape");
+    return -1;
 }
 
 static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
@@ -2111,12 +2181,55 @@ static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* 
 }
 
 template <typename T>
-static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* alm_t, void* phase, hipStream_t st) {
-    constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
-    if (sizeof(T) == 4 && nmap > 2) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
-    if (nmap <= 1) return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
-    if (BCMAX > 2 && nmap <= 2) return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
-    return launch_synth_sp<T, BCMAX>(p, nmap, ld, alm_t, phase, st);
+static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* alm_t, void* phase, hipStream_t st,
+                               int shape) {
+    if (shape < 0) shape = synth_shape_rule<T>(p, nmap * p.ncomp);
+    switch (shape) {
+        case 0:
+            if constexpr (sizeof(T) == 4) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
+            break;
+        case 1: return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
+        case 2: return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
+        case 4:
+            if constexpr (sizeof(T) == 4) return launch_synth_sp<T, 4>(p, nmap, ld, alm_t, phase, st);
+            break;
+    }
+    set_error("synthesis: invalid launch shape");
+    return -1;
+}
+
+static int shape_of(const ShtPlan& p, int ncol, int stage) {
+    const ShtShapes* s = p.shapes_for(ncol);
+    return s ? s->id[stage] : -1;
+}
+
+// n: batch columns (spin 0) or maps (spin s)
+static int legendre_synthesis(const ShtPlan& p, int n, const void* alm_t, int ld, void* phase, hipStream_t st,
+                              int shape) {
+    if (p.prec == 4)
+        return p.spin == 0 ? launch_synth_s0_any<float>(p, n, ld, alm_t, phase, st, shape)
+                           : launch_synth_sp_any<float>(p, n, ld, alm_t, phase, st, shape);
+    return p.spin == 0 ? launch_synth_s0_any<double>(p, n, ld, alm_t, phase, st, shape)
+                       : launch_synth_sp_any<double>(p, n, ld, alm_t, phase, st, shape);
+}
+
+static int ring_synthesis(const ShtPlan& p, int ncol, const void* phase, void* map, long map_stride, hipStream_t st,
+                          int nb_pow2, int nb_blue) {
+    if (p.prec == 4) return launch_ring_synth<float>(p, ncol, phase, map, map_stride, st, nb_pow2, nb_blue);
+    return launch_ring_synth<double>(p, ncol, phase, map, map_stride, st, nb_pow2, nb_blue);
+}
+
+// one ring class of the ring synthesis (maps of stride npix)
+static int ring_synth_list(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol, const void* phase,
+                           void* map, hipStream_t st, int nb) {
+    if (p.prec == 4) return launch_ring_synth_list<float>(p, list, nlist, maxM, ncol, phase, map, p.npix, st, nb);
+    return launch_ring_synth_list<double>(p, list, nlist, maxM, ncol, phase, map, p.npix, st, nb);
+}
+
+static int legendre_adjoint(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
+                            double scale, void* part, hipStream_t st, int shape) {
+    if (p.prec == 4) return launch_adj<float>(p, ncol, phase, alm, alm_stride, scale, part, st, shape);
+    return launch_adj<double>(p, ncol, phase, alm, alm_stride, scale, part, st, shape);
 }
 
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
@@ -2129,17 +2242,15 @@ int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, 
 int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
                                   hipStream_t st) {
     const int ncol = nmap * p.ncomp;
-    if (p.prec == 4)
-        return p.spin == 0 ? launch_synth_s0_any<float>(p, ncol, ld, alm_t, phase, st)
-                           : launch_synth_sp_any<float>(p, nmap, ld, alm_t, phase, st);
-    return p.spin == 0 ? launch_synth_s0_any<double>(p, ncol, ld, alm_t, phase, st)
-                       : launch_synth_sp_any<double>(p, nmap, ld, alm_t, phase, st);
+    // the spin-0 kernels take batch columns, the spin-s kernels maps
+    return legendre_synthesis(p, p.spin == 0 ? ncol : nmap, alm_t, ld, phase, st,
+                              shape_of(p, ncol, SHT_STAGE_SYNTH));
 }
 
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st) {
     const int ncol = nmap * p.ncomp;
-    if (p.prec == 4) return launch_ring_synth<float>(p, ncol, phase, map, map_stride, st);
-    return launch_ring_synth<double>(p, ncol, phase, map, map_stride, st);
+    return ring_synthesis(p, ncol, phase, map, map_stride, st, shape_of(p, ncol, SHT_STAGE_RING_POW2),
+                          shape_of(p, ncol, SHT_STAGE_RING_BLUE));
 }
 
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
@@ -2161,15 +2272,242 @@ int sht_adjoint(const ShtPlan& p, int nmap, const void* map, long map_stride, vo
     const int ncol = nmap * p.ncomp;
     void* phase = work;
     void* part = (char*)work + ((sht_phase_bytes(p, nmap) + 255) & ~(size_t)255);
-    int rc;
-    if (p.prec == 4) {
-        rc = launch_ring_adjoint<float>(p, ncol, map, map_stride, phase, st);
-        if (rc) return rc;
-        return launch_adj<float>(p, ncol, phase, alm, alm_stride, scale, part, st);
-    }
-    rc = launch_ring_adjoint<double>(p, ncol, map, map_stride, phase, st);
+    int rc = p.prec == 4 ? launch_ring_adjoint<float>(p, ncol, map, map_stride, phase, st)
+                         : launch_ring_adjoint<double>(p, ncol, map, map_stride, phase, st);
     if (rc) return rc;
-    return launch_adj<double>(p, ncol, phase, alm, alm_stride, scale, part, st);
+    return legendre_adjoint(p, ncol, phase, alm, alm_stride, scale, part, st, shape_of(p, ncol, SHT_STAGE_ADJ));
+}
+
+// ------------------------------------------------------------ launch-shape tuning
+
+int sht_default_shape(const ShtPlan& p, int nmap, int stage) {
+    const int ncol = nmap * p.ncomp;
+    switch (stage) {
+        case SHT_STAGE_SYNTH:
+            return p.prec == 4 ? synth_shape_rule<float>(p, ncol) : synth_shape_rule<double>(p, ncol);
+        case SHT_STAGE_RING_POW2: return ring_nb_rule(p.maxM_pow2, ncol, p.prec);
+        case SHT_STAGE_RING_BLUE: return ring_nb_rule(p.maxM_blue, ncol, p.prec);
+        case SHT_STAGE_ADJ: {
+            const int ns = sht_adjoint_nsplit(p, nmap);
+            const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
+            return adj_shape_rule(p, many, (p.npairs + ns - 1) / ns);
+        }
+    }
+    return -1;
+}
+
+int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int max) {
+    const int ncol = nmap * p.ncomp;
+    int buf[64], n = 0;
+    auto add = [&](int id) {
+        if (n < 64) buf[n++] = id;
+    };
+    switch (stage) {
+        case SHT_STAGE_SYNTH: {
+            // vector kernels whose chunk is between half and all of the (power-of-two rounded)
+            // batch, capped at the widest instantiated chunk; the matrix-core kernel in FP32
+            const int w = p.spin == 0 ? ncol : nmap;
+            const int bcmax = p.spin == 0 ? (p.prec == 4 ? SYN_BC_S0_F : SYN_BC_S0_D)
+                                          : (p.prec == 4 ? SYN_BC_SP_F : SYN_BC_SP_D);
+            int top = 1;
+            while (top < w && top < bcmax) top *= 2;
+            if (p.prec == 4) add(0);
+            for (int bc = std::max(1, top / 2); bc <= top; bc *= 2) add(bc);
+            break;
+        }
+        case SHT_STAGE_RING_POW2:
+        case SHT_STAGE_RING_BLUE: {
+            const long maxM = stage == SHT_STAGE_RING_POW2 ? p.maxM_pow2 : p.maxM_blue;
+            for (int nb = 1; nb <= 4; nb *= 2)
+                if (ring_nb_valid(maxM, ncol, p.prec, nb)) add(nb);
+            break;
+        }
+        case SHT_STAGE_ADJ: {
+            // shapes covering the ring slice with at most 4x idle threads
+            const int ns = sht_adjoint_nsplit(p, nmap);
+            const int pps = (p.npairs + ns - 1) / ns;
+            for (int id = 0; id < N_ADJ_SHAPES; id++) {
+                const int cover = 64 * ADJ_SHAPES[id].nrt * ADJ_SHAPES[id].nw;
+                if (adj_shape_valid(p, id, pps) && (cover <= 4 * pps || cover == 64)) add(id);
+            }
+            break;
+        }
+        default: set_error("launch shapes: stage must be 0..3"); return -1;
+    }
+    // the built-in choice is always a candidate
+    const int def = sht_default_shape(p, nmap, stage);
+    bool has_def = false;
+    for (int i = 0; i < n; i++) has_def = has_def || buf[i] == def;
+    if (!has_def) add(def);
+    for (int i = 0; ids && i < n && i < max; i++) ids[i] = buf[i];
+    return n;
+}
+
+static bool shape_allowed(const ShtPlan& p, int nmap, int stage, int id) {
+    int ids[64];
+    const int n = sht_shape_candidates(p, nmap, stage, ids, 64);
+    for (int i = 0; i < n && i < 64; i++)
+        if (ids[i] == id) return true;
+    return id == sht_default_shape(p, nmap, stage);
+}
+
+int sht_set_shapes(ShtPlan& p, int nmap, const int* ids) {
+    const int ncol = nmap * p.ncomp;
+    for (int st = 0; st < SHT_NSTAGE; st++)
+        if (ids[st] >= 0 && !shape_allowed(p, nmap, st, ids[st])) {
+            set_error("launch shapes: shape " + std::to_string(ids[st]) + " is not a candidate of stage " +
+                      std::to_string(st) + " for this batch");
+            return -1;
+        }
+    ShtShapes* e = nullptr;
+    for (int i = 0; i < p.nshapes; i++)
+        if (p.shapes[i].ncol == ncol) e = &p.shapes[i];
+    if (!e) {
+        // a full table drops the oldest batch width, which returns to the built-in rule
+        if (p.nshapes == SHT_MAX_SHAPES) {
+            for (int i = 1; i < SHT_MAX_SHAPES; i++) p.shapes[i - 1] = p.shapes[i];
+            p.nshapes--;
+        }
+        e = &p.shapes[p.nshapes++];
+    }
+    e->ncol = ncol;
+    for (int st = 0; st < SHT_NSTAGE; st++) e->id[st] = ids[st];
+    return 0;
+}
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t sht_autotune_work_bytes(const ShtPlan& p, int nmap) {
+    const int ncol = nmap * p.ncomp;
+    const int mult = sht_synth_lmajor_ld_multiple(p);
+    const long ld = (ncol + mult - 1) / mult * mult;
+    return al256((size_t)ld * p.nlm * 2 * p.prec) + al256((size_t)ncol * p.npix * p.prec) +
+           al256((size_t)ncol * p.nlm * 2 * p.prec) +
+           al256(std::max(sht_phase_bytes(p, nmap), sht_adjoint_work_bytes(p, nmap)));
+}
+
+// Bounded synthetic coefficients in [-0.5, 0.5) (a multiplicative hash of the index); the
+// padding columns of the l-major layout are zero.
+template <typename T>
+__global__ void k_tune_fill(cplx<T>* x, long n, int ld, int ncol) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const unsigned h = (unsigned)i * 2654435761u;
+        const bool live = (int)(i % ld) < ncol;
+        x[i] = live ? cplx<T>{T((h >> 8) * (1.0 / 16777216.0) - 0.5), T(((h ^ 0x9e3779b9u) >> 8) * (1.0 / 16777216.0) - 0.5)}
+                    : cplx<T>{T(0), T(0)};
+    }
+}
+
+namespace {
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
+// one untimed launch, then reps launches between two events; milliseconds per launch
+template <class F>
+static int time_shape(F&& launch, int reps, hipStream_t st, EventPair& ev, double* ms) {
+    int rc = launch();
+    if (rc) return rc;
+    PS_CHECK_HIP(hipEventRecord(ev.a, st));
+    for (int i = 0; i < reps; i++)
+        if ((rc = launch())) return rc;
+    PS_CHECK_HIP(hipEventRecord(ev.b, st));
+    PS_CHECK_HIP(hipEventSynchronize(ev.b));
+    float t = 0.f;
+    PS_CHECK_HIP(hipEventElapsedTime(&t, ev.a, ev.b));
+    *ms = (double)t / reps;
+    return 0;
+}
+
+int sht_autotune(ShtPlan& p, int nmap, void* work, size_t work_bytes, int reps, hipStream_t st, double* times) {
+    if (work_bytes < sht_autotune_work_bytes(p, nmap)) {
+        set_error("autotune: workspace too small");
+        return -1;
+    }
+    if (reps < 1) reps = 3;
+    const int ncol = nmap * p.ncomp;
+    const int mult = sht_synth_lmajor_ld_multiple(p);
+    const int ld = (ncol + mult - 1) / mult * mult;
+    char* w = (char*)work;
+    void* alm_t = w;
+    w += al256((size_t)ld * p.nlm * 2 * p.prec);
+    void* map = w;
+    w += al256((size_t)ncol * p.npix * p.prec);
+    void* alm = w;
+    w += al256((size_t)ncol * p.nlm * 2 * p.prec);
+    void* phase = w;
+    void* part = (char*)phase + al256(sht_phase_bytes(p, nmap));
+
+    const long nel = (long)ld * p.nlm;
+    const unsigned nbf = (unsigned)std::min<long>((nel + 255) / 256, 16384);
+    if (p.prec == 4)
+        hipLaunchKernelGGL(k_tune_fill<float>, dim3(nbf), dim3(256), 0, st, (cplx<float>*)alm_t, nel, ld, ncol);
+    else
+        hipLaunchKernelGGL(k_tune_fill<double>, dim3(nbf), dim3(256), 0, st, (cplx<double>*)alm_t, nel, ld, ncol);
+    PS_LAUNCH_CHECK();
+
+    EventPair ev;
+    PS_CHECK_HIP(hipEventCreate(&ev.a));
+    PS_CHECK_HIP(hipEventCreate(&ev.b));
+    int chosen[SHT_NSTAGE];
+    const int n_syn = p.spin == 0 ? ncol : nmap;
+    for (int stage = 0; stage < SHT_NSTAGE; stage++) {
+        int ids[64];
+        const int nc = std::min(64, sht_shape_candidates(p, nmap, stage, ids, 64));
+        const int def = sht_default_shape(p, nmap, stage);
+        chosen[stage] = def;
+        if (times) times[2 * stage] = times[2 * stage + 1] = 0.0;
+        const bool empty = (stage == SHT_STAGE_RING_POW2 && p.n_pow2 == 0) ||
+                           (stage == SHT_STAGE_RING_BLUE && p.nrings == p.n_pow2);
+        if (empty) continue;
+        if (stage == SHT_STAGE_ADJ) {
+            // phases of the adjoint layout, once, from the synthesised maps
+            int rc = p.prec == 4 ? launch_ring_adjoint<float>(p, ncol, map, p.npix, phase, st)
+                                 : launch_ring_adjoint<double>(p, ncol, map, p.npix, phase, st);
+            if (rc) return rc;
+        }
+        auto run = [&](int id) -> int {
+            switch (stage) {
+                case SHT_STAGE_SYNTH: return legendre_synthesis(p, n_syn, alm_t, ld, phase, st, id);
+                case SHT_STAGE_RING_POW2:
+                    return ring_synth_list(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, st, id);
+                case SHT_STAGE_RING_BLUE:
+                    return ring_synth_list(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol,
+                                           phase, map, st, id);
+                default: return legendre_adjoint(p, ncol, phase, alm, p.nlm, 1.0, part, st, id);
+            }
+        };
+        double t_def = 0.0;
+        int rc = time_shape([&] { return run(def); }, reps, st, ev, &t_def);
+        if (rc) return rc;
+        double t_best = t_def;
+        int best = def;
+        for (int i = 0; i < nc; i++) {
+            if (ids[i] == def) continue;
+            double t = 0.0;
+            if ((rc = time_shape([&] { return run(ids[i]); }, reps, st, ev, &t))) return rc;
+            if (t < t_best) {
+                t_best = t;
+                best = ids[i];
+            }
+        }
+        if (best != def && t_best < 0.97 * t_def) chosen[stage] = best;
+        else t_best = t_def;
+        if (times) {
+            times[2 * stage] = t_best;
+            times[2 * stage + 1] = t_def;
+        }
+        if (stage == SHT_STAGE_SYNTH) {
+            // the ring stages read phases of the chosen synthesis
+            if ((rc = run(chosen[stage]))) return rc;
+        }
+    }
+    return sht_set_shapes(p, nmap, chosen);
 }
 
 }  // namespace ps

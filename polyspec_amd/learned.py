@@ -247,8 +247,9 @@ class LearnedPreconditioner:
         self.Llo = int(min(self.L, 2 * nside_lo if lmax_lo is None else lmax_lo))
         dev = self.base.device
         with torch.cuda.device(dev):
-            self.p0 = ShtPlan(nside_lo, self.Llo, 0, self.base.precision)
-            self.p2 = ShtPlan(nside_lo, self.Llo, 2, self.base.precision) if self.nf == 3 else None
+            tune = getattr(self.base, 'autotune', False)
+            self.p0 = ShtPlan(nside_lo, self.Llo, 0, self.base.precision, autotune=tune)
+            self.p2 = ShtPlan(nside_lo, self.Llo, 2, self.base.precision, autotune=tune) if self.nf == 3 else None
         self.npix_lo = 12 * self.nside_lo ** 2
         self.sel = torch.from_numpy(lm_select(self.L, self.Llo)).to(dev)
         nbr, rev = ring_stencil(self.nside_lo)
