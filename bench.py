@@ -183,7 +183,7 @@ def _cg_trace(W, d, precond, nstep, thresh_rz=1e-6, rb_marks=(1e-3, 1e-4)):
     return out
 
 
-def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=2000, nside_lo=128):
+def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=2000, nside_lo=128, batch=16):
     """Inverse-variance CG (applySinv_optimal's solver) at base_lmax: plain harmonic CG against the
     flexible CG with the learned preconditioner, both judged by rz/rz0 < thresh."""
     import torch
@@ -212,6 +212,30 @@ def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=20
     plain["preconditioner"] = "harmonic (Sigma_lm, weighting.py:304-305)"
     log("cg plain: %s" % plain)
     res = {"plain": plain, "setup": info}
+    if batch:
+        # batch maps solved in one lock-step CG against the same maps solved one by one
+        ds = torch.stack([data(100 + i) for i in range(batch)])
+        W.applySinv_optimal_batch(ds, preconditioner='harmonic', nstep=1, thresh=thresh)  # warm-up, tuning
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outb = W.applySinv_optimal_batch(ds, preconditioner='harmonic', nstep=nstep, thresh=thresh)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        its = W.last_cg.iterations.copy()
+        t0 = time.perf_counter()
+        singles = [W.applySinv_optimal(ds[i], preconditioner='harmonic', nstep=nstep, thresh=thresh)
+                   for i in range(batch)]
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        diff = max(float((outb[i] - singles[i]).abs().max() / singles[i].abs().max()) for i in range(batch))
+        res["batched"] = {"maps": batch, "solves_per_s": batch / tb, "single_solves_per_s": batch / ts,
+                          "speedup": ts / tb, "iterations_min_max": [int(its.min()), int(its.max())],
+                          "max_rel_diff_vs_single": diff,
+                          "solver": "BatchCGD: harmonic preconditioner, every map its own step lengths and "
+                                    "stopping iteration, transforms of all maps batched"}
+        log("cg batched: %s" % res["batched"])
+        del ds, outb, singles
+        torch.cuda.empty_cache()
     if learned_pairs:
         LP = LearnedPreconditioner(W, nside_lo=nside_lo)
         torch.cuda.synchronize()
@@ -439,6 +463,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-nodes", type=int, default=2)
     ap.add_argument("--no-cg", action="store_true", help="skip the CG S^-1 section")
+    ap.add_argument("--cg-batch", type=int, default=16, help="maps of the batched CG measurement (0: skip)")
     ap.add_argument("--learned-pairs", type=int, default=2000,
                     help="training pairs for the learned CG preconditioner (0 = plain CG only)")
     ap.add_argument("--no-bspec", action="store_true", help="skip the binned-bispectrum section")
@@ -581,7 +606,8 @@ def main():
 
     if not args.no_cg:
         log("CG S^-1: plain harmonic vs learned preconditioner ...")
-        result["cg"] = bench_cg(nside, mask, learned_pairs=args.learned_pairs, nside_lo=min(128, nside))
+        result["cg"] = bench_cg(nside, mask, learned_pairs=args.learned_pairs, nside_lo=min(128, nside),
+                                batch=args.cg_batch)
 
     if not args.no_radial:
         log("radial-node optimizer ...")

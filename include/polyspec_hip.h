@@ -148,6 +148,10 @@ int ps_mc_accumulate(int precision, const void *P, const void *Q, int nsim, long
  * 3 a*(2 b c + a d) (tauNL_utils.multiply_asym, 116-124). */
 int ps_pixel_products(int precision, int mode, const void *a, const void *b, const void *c, const void *d,
                       long long n, double scale, void *out, void *stream);
+/* The same over a batch: a and out are (nbatch, n), b, c, d (n) shared by every
+ * stack (the weight maps of the batched CG below). */
+int ps_pixel_products_batch(int precision, int mode, int nbatch, const void *a, const void *b, const void *c,
+                            const void *d, long long n, double scale, void *out, void *stream);
 
 /* tauNL_utils.radial_sum (tauNL_utils.pyx:220-235) over the full m-major set,
  * zero below lmin: out[X, lm] (+)= scale sum_r lm_map[r, lm] f[l,X,r] w_r, X < npol. */
@@ -197,6 +201,21 @@ int ps_cg_update_xr(int precision, long long n, const double *rz, const double *
                     void *x, void *r, void *stream);
 int ps_cg_update_p(int precision, long long n, const double *num, const double *den, const void *z, void *p,
                    void *stream);
+
+/* Batched CG: nbatch independent systems solved in lock step (the reference solves one
+ * map at a time, weighting.py:148-184 called per simulation; these run the same
+ * iteration for every map of a batch).  Vectors are (nbatch, nf, nlm) contiguous;
+ * scalars are per system (double[nbatch]); active[b] == 0 leaves system b unchanged
+ * (converged).  mdot_batch: out[b] = dot of system b; scratch ps_mdot_batch_scratch_bytes. */
+int ps_lmat_apply_batch(int precision, int nbatch, int nf, int lmax, const double *mat, const double *pre,
+                        const double *post, const void *in, void *out, double alpha, int accumulate, void *stream);
+long long ps_mdot_batch_scratch_bytes(int nbatch);
+int ps_mdot_batch(int precision, int nbatch, int nf, int lmax, const void *x, const void *y, double *scratch,
+                  double *out, void *stream);
+int ps_cg_update_xr_batch(int precision, int nbatch, long long n, const double *rz, const double *pAp,
+                          const int *active, const void *p, const void *Ap, void *x, void *r, void *stream);
+int ps_cg_update_p_batch(int precision, int nbatch, long long n, const double *num, const double *den,
+                         const int *active, const void *z, void *p, void *stream);
 
 /* -------------------------------------------- binned bispectrum (BSpecBin)
  * Pixel reductions of the binned-bispectrum numerator (bspec_bin.py:339-403, the

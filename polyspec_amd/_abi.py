@@ -50,6 +50,12 @@ SIGNATURES = {
     "ps_mdot": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     "ps_cg_update_xr": (_I, [_I, _LL, _P, _P, _P, _P, _P, _P, _P]),
     "ps_cg_update_p": (_I, [_I, _LL, _P, _P, _P, _P, _P]),
+    "ps_pixel_products_batch": (_I, [_I, _I, _I, _P, _P, _P, _P, _LL, _D, _P, _P]),
+    "ps_lmat_apply_batch": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _I, _P]),
+    "ps_mdot_batch_scratch_bytes": (_LL, [_I]),
+    "ps_mdot_batch": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
+    "ps_cg_update_xr_batch": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ps_cg_update_p_batch": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P]),
     "ps_nn_im2col9": (_I, [_P, _P, _I, _I, _I, _P, _P]),
     "ps_nn_col2im9": (_I, [_P, _P, _I, _I, _I, _I, _P, _P]),
     "ps_nn_gemm": (_I, [_I, _I, _I, _I, _I, _P, _LL, _P, _LL, _P, _LL, _P, _I, _I, _P, _P]),

@@ -246,6 +246,38 @@ class PolySpec:
             p0.synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1))
         return self._ret(out, npy)
 
+    def to_lm_batch(self, maps, lmax=None):
+        """to_lm of a stack of map sets, (B, nf, Npix) -> (B, nf, nlm), nf = 1 or 3: one batched
+        adjoint per spin over all B sets (the reference transforms one set per call)."""
+        B, nf = maps.shape[0], maps.shape[1]
+        assert nf in (1, 3), "Wrong input shape supplied!"
+        lmax = self.lmax if lmax is None else lmax
+        self.n_SHTs_forward += B * (1 + (nf == 3))
+        m = self._to_dev(maps, False).reshape(B, nf, self.Npix)
+        sc = np.pi / (3. * self.Nside ** 2)
+        p0 = self.plan(lmax, 0)
+        if nf == 1:
+            return p0.adjoint(m, sc)
+        out = torch.empty((B, 3, p0.nlm), dtype=self.cdtype, device=self.device)
+        out[:, 0:1] = p0.adjoint(m[:, 0:1].contiguous(), sc)
+        out[:, 1:3] = self.plan(lmax, 2).adjoint(m[:, 1:3].contiguous(), sc)
+        return out
+
+    def to_map_batch(self, alms, lmax=None):
+        """to_map of a stack of harmonic sets, (B, nf, nlm) -> (B, nf, Npix), nf = 1 or 3."""
+        B, nf = alms.shape[0], alms.shape[1]
+        assert nf in (1, 3), "Wrong input shape supplied!"
+        lmax = self.lmax if lmax is None else lmax
+        self.n_SHTs_reverse += B * (1 + (nf == 3))
+        a = self._to_dev(alms, True)
+        p0 = self.plan(lmax, 0)
+        if nf == 1:
+            return p0.synthesis(a)
+        out = torch.empty((B, 3, self.Npix), dtype=self.rdtype, device=self.device)
+        out[:, 0:1] = p0.synthesis(a[:, 0:1].contiguous())
+        out[:, 1:3] = self.plan(lmax, 2).synthesis(a[:, 1:3].contiguous())
+        return out
+
     @timer_func
     def to_lm_vec(self, input_vec, spin=0, lmax=None):
         """Batched map -> harmonic transform, base.py:178-209.

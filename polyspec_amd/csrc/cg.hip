@@ -11,11 +11,15 @@
 
 namespace ps {
 
-// out[a] = post[a,l] sum_b mat[a,b,l] pre[b,l] in[b]   (accumulate: out += ...)
+// out[a] = post[a,l] sum_b mat[a,b,l] pre[b,l] in[b]   (accumulate: out += ...);
+// blockIdx.y selects a vector of a batch (vectors in_bstride / out_bstride elements apart)
 template <typename T>
 __global__ void k_lmat(int nf, int lmax, const double* __restrict__ mat, const double* __restrict__ pre,
                        const double* __restrict__ post, const cplx<T>* __restrict__ in, long in_stride,
-                       cplx<T>* __restrict__ out, long out_stride, double alpha, int accumulate) {
+                       cplx<T>* __restrict__ out, long out_stride, double alpha, int accumulate, long in_bstride,
+                       long out_bstride) {
+    in += (long)blockIdx.y * in_bstride;
+    out += (long)blockIdx.y * out_bstride;
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long lm = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (lm >= nlm) return;
@@ -62,12 +66,16 @@ __device__ __forceinline__ double block_sum256(double v, double* sh) {
 }
 
 // Re sum_{a, lm} x[a,lm] conj(y[a,lm]) (1 + [m > 0])   (weighting.py:255)
+// (blockIdx.y: vector of a batch, nf * nlm elements apart; partial sums gridDim.x apart)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mdot(int nf, int lmax, const cplx<T>* __restrict__ x,
                                               const cplx<T>* __restrict__ y, double* __restrict__ part) {
     __shared__ double sh[256];
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long n = nlm * nf;
+    x += (long)blockIdx.y * n;
+    y += (long)blockIdx.y * n;
+    part += (long)blockIdx.y * gridDim.x;
     double acc = 0.0;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -79,20 +87,31 @@ __global__ __launch_bounds__(256) void k_mdot(int nf, int lmax, const cplx<T>* _
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// one block per vector of the batch: out[b] = sum of its n partial sums, fixed order
 __global__ void k_dot_finish(const double* __restrict__ part, int n, double* __restrict__ out) {
     __shared__ double sh[256];
+    part += (long)blockIdx.x * n;
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) s += part[i];
     s = block_sum256(s, sh);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
-// CG step (weighting.py:329-332): alpha = rz / pAp; x += alpha p; r -= alpha Ap
+// CG step (weighting.py:329-332): alpha = rz / pAp; x += alpha p; r -= alpha Ap.
+// blockIdx.y: system of a batch (vectors n elements apart, its own scalars); systems with
+// active[b] == 0 (converged) are left as they are.
 template <typename T>
 __global__ void k_cg_xr(long n, const double* __restrict__ rz, const double* __restrict__ pAp,
-                        const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ Ap, cplx<T>* __restrict__ x,
-                        cplx<T>* __restrict__ r) {
-    const double alpha = *rz / *pAp;
+                        const int* __restrict__ active, const cplx<T>* __restrict__ p,
+                        const cplx<T>* __restrict__ Ap, cplx<T>* __restrict__ x, cplx<T>* __restrict__ r) {
+    const int bs = blockIdx.y;
+    if (active && !active[bs]) return;
+    const double alpha = rz[bs] / pAp[bs];
+    const long off = (long)bs * n;
+    p += off;
+    Ap += off;
+    x += off;
+    r += off;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         x[i].re = (T)((double)x[i].re + alpha * (double)p[i].re);
@@ -106,8 +125,12 @@ __global__ void k_cg_xr(long n, const double* __restrict__ rz, const double* __r
 // the Polak-Ribiere numerator.
 template <typename T>
 __global__ void k_cg_p(long n, const double* __restrict__ num, const double* __restrict__ den,
-                       const cplx<T>* __restrict__ z, cplx<T>* __restrict__ p) {
-    const double beta = *num / *den;
+                       const int* __restrict__ active, const cplx<T>* __restrict__ z, cplx<T>* __restrict__ p) {
+    const int bs = blockIdx.y;
+    if (active && !active[bs]) return;
+    const double beta = num[bs] / den[bs];
+    z += (long)bs * n;
+    p += (long)bs * n;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         p[i].re = (T)((double)z[i].re + beta * (double)p[i].re);
@@ -119,81 +142,138 @@ __global__ void k_cg_p(long n, const double* __restrict__ num, const double* __r
 
 using namespace ps;
 
+static int lmat_launch(int precision, int nbatch, int nf, int lmax, const double* mat, const double* pre,
+                       const double* post, const void* in, long in_stride, long in_bstride, void* out,
+                       long out_stride, long out_bstride, double alpha, int accumulate, hipStream_t st) {
+    PS_REQUIRE(nf >= 1 && nf <= 3, "lmat_apply: nf must be 1..3");
+    PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "lmat_apply: nbatch must be 1..65535");
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const dim3 grid((unsigned)((nlm + 255) / 256), (unsigned)nbatch);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_lmat<float>, grid, dim3(256), 0, st, nf, lmax, mat, pre, post, (const cplx<float>*)in,
+                           in_stride, (cplx<float>*)out, out_stride, alpha, accumulate, in_bstride, out_bstride);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_lmat<double>, grid, dim3(256), 0, st, nf, lmax, mat, pre, post, (const cplx<double>*)in,
+                           in_stride, (cplx<double>*)out, out_stride, alpha, accumulate, in_bstride, out_bstride);
+    else
+        PS_REQUIRE(false, "lmat_apply: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+// partial-sum blocks per vector: DOT_BLOCKS for one vector, fewer per vector for batches
+static int dot_blocks(int nbatch) { return nbatch == 1 ? DOT_BLOCKS : 64; }
+
+static int mdot_launch(int precision, int nbatch, int nf, int lmax, const void* x, const void* y, double* scratch,
+                       double* out, hipStream_t st) {
+    PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "mdot: nbatch must be 1..65535");
+    const int nbk = dot_blocks(nbatch);
+    const dim3 grid((unsigned)nbk, (unsigned)nbatch);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_mdot<float>, grid, dim3(256), 0, st, nf, lmax, (const cplx<float>*)x,
+                           (const cplx<float>*)y, scratch);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_mdot<double>, grid, dim3(256), 0, st, nf, lmax, (const cplx<double>*)x,
+                           (const cplx<double>*)y, scratch);
+    else
+        PS_REQUIRE(false, "mdot: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dot_finish, dim3((unsigned)nbatch), dim3(256), 0, st, (const double*)scratch, nbk, out);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+static int cg_xr_launch(int precision, int nbatch, long n, const double* rz, const double* pAp, const int* active,
+                        const void* p, const void* Ap, void* x, void* r, hipStream_t st) {
+    PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "cg_update_xr: nbatch must be 1..65535");
+    const dim3 grid((unsigned)std::min<long>((n + 255) / 256, nbatch == 1 ? 4096 : 1024), (unsigned)nbatch);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_cg_xr<float>, grid, dim3(256), 0, st, n, rz, pAp, active, (const cplx<float>*)p,
+                           (const cplx<float>*)Ap, (cplx<float>*)x, (cplx<float>*)r);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_cg_xr<double>, grid, dim3(256), 0, st, n, rz, pAp, active, (const cplx<double>*)p,
+                           (const cplx<double>*)Ap, (cplx<double>*)x, (cplx<double>*)r);
+    else
+        PS_REQUIRE(false, "cg_update_xr: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+static int cg_p_launch(int precision, int nbatch, long n, const double* num, const double* den, const int* active,
+                       const void* z, void* p, hipStream_t st) {
+    PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "cg_update_p: nbatch must be 1..65535");
+    const dim3 grid((unsigned)std::min<long>((n + 255) / 256, nbatch == 1 ? 4096 : 1024), (unsigned)nbatch);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_cg_p<float>, grid, dim3(256), 0, st, n, num, den, active, (const cplx<float>*)z,
+                           (cplx<float>*)p);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_cg_p<double>, grid, dim3(256), 0, st, n, num, den, active, (const cplx<double>*)z,
+                           (cplx<double>*)p);
+    else
+        PS_REQUIRE(false, "cg_update_p: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 extern "C" {
 
 int ps_lmat_apply(int precision, int nf, int lmax, const double* mat, const double* pre, const double* post,
                   const void* in, long long in_stride, void* out, long long out_stride, double alpha, int accumulate,
                   void* stream) {
     PS_REQUIRE(in && out, "ps_lmat_apply: NULL argument");
-    PS_REQUIRE(nf >= 1 && nf <= 3, "ps_lmat_apply: nf must be 1..3");
+    return lmat_launch(precision, 1, nf, lmax, mat, pre, post, in, (long)in_stride, 0, out, (long)out_stride, 0,
+                       alpha, accumulate, (hipStream_t)stream);
+}
+
+int ps_lmat_apply_batch(int precision, int nbatch, int nf, int lmax, const double* mat, const double* pre,
+                        const double* post, const void* in, void* out, double alpha, int accumulate, void* stream) {
+    PS_REQUIRE(in && out, "ps_lmat_apply_batch: NULL argument");
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    unsigned nb = (unsigned)((nlm + 255) / 256);
-    hipStream_t st = (hipStream_t)stream;
-    if (precision == 32)
-        hipLaunchKernelGGL(k_lmat<float>, dim3(nb), dim3(256), 0, st, nf, lmax, mat, pre, post, (const cplx<float>*)in,
-                           (long)in_stride, (cplx<float>*)out, (long)out_stride, alpha, accumulate);
-    else if (precision == 64)
-        hipLaunchKernelGGL(k_lmat<double>, dim3(nb), dim3(256), 0, st, nf, lmax, mat, pre, post,
-                           (const cplx<double>*)in, (long)in_stride, (cplx<double>*)out, (long)out_stride, alpha,
-                           accumulate);
-    else
-        PS_REQUIRE(false, "ps_lmat_apply: precision must be 32 or 64");
-    PS_LAUNCH_CHECK();
-    return 0;
+    return lmat_launch(precision, nbatch, nf, lmax, mat, pre, post, in, nlm, nf * nlm, out, nlm, nf * nlm, alpha,
+                       accumulate, (hipStream_t)stream);
 }
 
 int ps_mdot(int precision, int nf, int lmax, const void* x, const void* y, double* scratch, double* out,
             void* stream) {
     PS_REQUIRE(x && y && scratch && out, "ps_mdot: NULL argument");
-    hipStream_t st = (hipStream_t)stream;
-    if (precision == 32)
-        hipLaunchKernelGGL(k_mdot<float>, dim3(DOT_BLOCKS), dim3(256), 0, st, nf, lmax, (const cplx<float>*)x,
-                           (const cplx<float>*)y, scratch);
-    else if (precision == 64)
-        hipLaunchKernelGGL(k_mdot<double>, dim3(DOT_BLOCKS), dim3(256), 0, st, nf, lmax, (const cplx<double>*)x,
-                           (const cplx<double>*)y, scratch);
-    else
-        PS_REQUIRE(false, "ps_mdot: precision must be 32 or 64");
-    PS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(256), 0, st, (const double*)scratch, DOT_BLOCKS, out);
-    PS_LAUNCH_CHECK();
-    return 0;
+    return mdot_launch(precision, 1, nf, lmax, x, y, scratch, out, (hipStream_t)stream);
+}
+
+int ps_mdot_batch(int precision, int nbatch, int nf, int lmax, const void* x, const void* y, double* scratch,
+                  double* out, void* stream) {
+    PS_REQUIRE(x && y && scratch && out, "ps_mdot_batch: NULL argument");
+    return mdot_launch(precision, nbatch, nf, lmax, x, y, scratch, out, (hipStream_t)stream);
 }
 
 int ps_cg_update_xr(int precision, long long n, const double* rz, const double* pAp, const void* p, const void* Ap,
                     void* x, void* r, void* stream) {
     PS_REQUIRE(rz && pAp && p && Ap && x && r, "ps_cg_update_xr: NULL argument");
-    hipStream_t st = (hipStream_t)stream;
-    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 4096);
-    if (precision == 32)
-        hipLaunchKernelGGL(k_cg_xr<float>, dim3(nb), dim3(256), 0, st, (long)n, rz, pAp, (const cplx<float>*)p,
-                           (const cplx<float>*)Ap, (cplx<float>*)x, (cplx<float>*)r);
-    else if (precision == 64)
-        hipLaunchKernelGGL(k_cg_xr<double>, dim3(nb), dim3(256), 0, st, (long)n, rz, pAp, (const cplx<double>*)p,
-                           (const cplx<double>*)Ap, (cplx<double>*)x, (cplx<double>*)r);
-    else
-        PS_REQUIRE(false, "ps_cg_update_xr: precision must be 32 or 64");
-    PS_LAUNCH_CHECK();
-    return 0;
+    return cg_xr_launch(precision, 1, (long)n, rz, pAp, nullptr, p, Ap, x, r, (hipStream_t)stream);
+}
+
+int ps_cg_update_xr_batch(int precision, int nbatch, long long n, const double* rz, const double* pAp,
+                          const int* active, const void* p, const void* Ap, void* x, void* r, void* stream) {
+    PS_REQUIRE(rz && pAp && active && p && Ap && x && r, "ps_cg_update_xr_batch: NULL argument");
+    return cg_xr_launch(precision, nbatch, (long)n, rz, pAp, active, p, Ap, x, r, (hipStream_t)stream);
 }
 
 int ps_cg_update_p(int precision, long long n, const double* num, const double* den, const void* z, void* p,
                    void* stream) {
     PS_REQUIRE(num && den && z && p, "ps_cg_update_p: NULL argument");
-    hipStream_t st = (hipStream_t)stream;
-    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 4096);
-    if (precision == 32)
-        hipLaunchKernelGGL(k_cg_p<float>, dim3(nb), dim3(256), 0, st, (long)n, num, den, (const cplx<float>*)z,
-                           (cplx<float>*)p);
-    else if (precision == 64)
-        hipLaunchKernelGGL(k_cg_p<double>, dim3(nb), dim3(256), 0, st, (long)n, num, den, (const cplx<double>*)z,
-                           (cplx<double>*)p);
-    else
-        PS_REQUIRE(false, "ps_cg_update_p: precision must be 32 or 64");
-    PS_LAUNCH_CHECK();
-    return 0;
+    return cg_p_launch(precision, 1, (long)n, num, den, nullptr, z, p, (hipStream_t)stream);
+}
+
+int ps_cg_update_p_batch(int precision, int nbatch, long long n, const double* num, const double* den,
+                         const int* active, const void* z, void* p, void* stream) {
+    PS_REQUIRE(num && den && active && z && p, "ps_cg_update_p_batch: NULL argument");
+    return cg_p_launch(precision, nbatch, (long)n, num, den, active, z, p, (hipStream_t)stream);
 }
 
 long long ps_mdot_scratch_bytes(void) { return (long long)DOT_BLOCKS * sizeof(double); }
+
+long long ps_mdot_batch_scratch_bytes(int nbatch) {
+    if (nbatch < 1) return -1;
+    return (long long)dot_blocks(nbatch) * nbatch * sizeof(double);
+}
 
 }  // extern "C"

@@ -148,6 +148,9 @@ __global__ void k_mc_accumulate(const T* __restrict__ P, const T* __restrict__ Q
 template <typename T>
 __global__ void k_products(int mode, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
                            const T* __restrict__ d, long n, T scale, T* __restrict__ out) {
+    // blockIdx.y: map stack of a batch (a and out n elements apart; b, c, d shared)
+    a += (long)blockIdx.y * n;
+    out += (long)blockIdx.y * n;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         T v;
@@ -391,24 +394,34 @@ int ps_mc_accumulate(int precision, const void* P, const void* Q, int nsim, long
     return 0;
 }
 
-int ps_pixel_products(int precision, int mode, const void* a, const void* b, const void* c, const void* d,
-                      long long n, double scale, void* out, void* stream) {
+static int products_launch(int precision, int mode, int nbatch, const void* a, const void* b, const void* c,
+                           const void* d, long n, double scale, void* out, hipStream_t st) {
     PS_REQUIRE(mode >= 0 && mode <= 3, "ps_pixel_products: mode must be 0..3");
     PS_REQUIRE(a && b && out, "ps_pixel_products: NULL argument");
     PS_REQUIRE(!(mode == 1 || mode == 3) || (c && d), "ps_pixel_products: c and d required for this mode");
     PS_REQUIRE(!(mode == 2) || c, "ps_pixel_products: c required for mode 2");
-    hipStream_t st = (hipStream_t)stream;
-    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 16384);
+    PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "ps_pixel_products: nbatch must be 1..65535");
+    const dim3 grid((unsigned)std::min<long>((n + 255) / 256, nbatch == 1 ? 16384 : 2048), (unsigned)nbatch);
     if (precision == 32)
-        hipLaunchKernelGGL(k_products<float>, dim3(nb), dim3(256), 0, st, mode, (const float*)a, (const float*)b,
-                           (const float*)c, (const float*)d, (long)n, (float)scale, (float*)out);
+        hipLaunchKernelGGL(k_products<float>, grid, dim3(256), 0, st, mode, (const float*)a, (const float*)b,
+                           (const float*)c, (const float*)d, n, (float)scale, (float*)out);
     else if (precision == 64)
-        hipLaunchKernelGGL(k_products<double>, dim3(nb), dim3(256), 0, st, mode, (const double*)a, (const double*)b,
-                           (const double*)c, (const double*)d, (long)n, scale, (double*)out);
+        hipLaunchKernelGGL(k_products<double>, grid, dim3(256), 0, st, mode, (const double*)a, (const double*)b,
+                           (const double*)c, (const double*)d, n, scale, (double*)out);
     else
         PS_REQUIRE(false, "ps_pixel_products: precision must be 32 or 64");
     PS_LAUNCH_CHECK();
     return 0;
+}
+
+int ps_pixel_products(int precision, int mode, const void* a, const void* b, const void* c, const void* d,
+                      long long n, double scale, void* out, void* stream) {
+    return products_launch(precision, mode, 1, a, b, c, d, (long)n, scale, out, (hipStream_t)stream);
+}
+
+int ps_pixel_products_batch(int precision, int mode, int nbatch, const void* a, const void* b, const void* c,
+                            const void* d, long long n, double scale, void* out, void* stream) {
+    return products_launch(precision, mode, nbatch, a, b, c, d, (long)n, scale, out, (hipStream_t)stream);
 }
 
 int ps_radial_sum(int precision, const void* lm_map, int nr, int lmax, int lmin, const void* f, int nslot, int npol,

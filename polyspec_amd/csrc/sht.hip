@@ -1680,6 +1680,141 @@ __global__ __launch_bounds__(64 * NW) void k_leg_adj_s0(ShtDev<T> d, int ncol, c
     }
 }
 
+// ------------------------------------- Legendre adjoint, spin 0, FP32 matrix cores
+// For one m, alm(l, col) = sum over ring pairs of lam_lm(ring) (P_north + P_south)(col) for
+// even l - m and lam_lm(ring) (P_north - P_south)(col) for odd l - m: a contraction over
+// rings, run on v_mfma_f32_32x32x2f32 with A = lam (32 values of l of one parity x 2 rings),
+// B = the summed / differenced phases (2 rings x 32 real columns), D(l, col) += A B (lane
+// layout as in the synthesis kernel).  Each lane runs the FP64 recursion of one ring pair
+// for a tile of 64 values of l and stores the values in its wave's LDS tile
+// [parity][l / 2][ring], rows XOR-swizzled by l / 2 so that the row-wise stores and the
+// transposed operand loads both hit 32 distinct banks per half-wave.  The B operands (32 ring
+// pairs x 2 per lane) stay in registers for the whole l loop.  One block = 4 waves = 256
+// ring pairs (a ring slice), one m, 16 complex columns; the waves' sums meet in LDS and
+// slices write partial sums that k_adj_sum_splits adds in a fixed order.
+constexpr int ADJM_PAIRS = 256;
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(256) void k_leg_adj_s0_mfma(ShtDev<float> d, int ncol,
+                                                         const cplx<float>* __restrict__ phase,
+                                                         cplx<float>* __restrict__ out, long out_stride, float osc,
+                                                         int nslice) {
+    __shared__ __align__(16) float T[4 * 2 * 32 * 64];  // lam tiles, then the waves' sums [w][par][i][33]
+    const int nchunk = (ncol + 15) / 16;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = lb % nchunk;
+    const int sl = (lb / nchunk) % nslice;
+    const int m = m_of_q(lb / (nchunk * nslice), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int wring = sl * ADJM_PAIRS + wave * 64;  // first ring pair of this wave
+    cplx<float>* dst = nslice > 1 ? out + (size_t)sl * ncol * out_stride : out;
+
+    // recursion of ring pair wring + lane
+    const int jr = wring + lane;
+    const int jrc = jr < d.npairs ? jr : d.npairs - 1;
+    const double x = d.cth[jrc];
+    Rec r;
+    rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jrc] + d.seedB[m] * d.l2ch[jrc], d.seedS[m]);
+
+    // B operands: ring pair wring + 2 s + h, real column li32 (complex column li32 / 2)
+    float bev[32], bod[32];
+    {
+        const int col = chunk * 16 + (li32 >> 1), part = li32 & 1;
+        const float* P = reinterpret_cast<const float*>(phase + (size_t)col * (d.mmax + 1) * d.nrings +
+                                                        (size_t)m * d.nrings);
+#pragma unroll
+        for (int s2 = 0; s2 < 32; s2++) {
+            const int jb = wring + 2 * s2 + h;
+            float pn = 0.f, ps = 0.f;
+            if (jb < d.npairs && col < ncol) {
+                const int rs = d.nrings - 1 - jb;
+                pn = P[2 * jb + part];
+                if (rs != jb) ps = P[2 * rs + part];
+            }
+            bev[s2] = pn + ps;
+            bod[s2] = pn - ps;
+        }
+    }
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const double* cf4 = d.coef + 4 * base;
+    // a block whose ring pairs are all dead at this m writes zeros without the recursion
+    const bool skip = __syncthreads_and(jr >= d.npairs || d.dead[(size_t)m * d.npairs + jrc]);
+    float* Tw = T + wave * 4096;
+    bool fast = false;
+    for (int l0 = m; l0 <= d.lmax; l0 += 64) {
+        const int n = min(64, d.lmax + 1 - l0);
+        v16f De = v16f{0}, Do = v16f{0};
+        if (!skip) {
+            // lane c holds (alpha, gamma) of l0 + c
+            double pal = 0.0, pga = 0.0;
+            if (lane < n) {
+                pal = cf4[4 * (l0 + lane)];
+                pga = cf4[4 * (l0 + lane) + 2];
+            }
+            if (!fast) fast = __all(r.k == 0);
+            if (fast && n == 64) {
+#pragma unroll 4
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 2048 + i * 64 + (lane ^ i)] = (float)r.f;
+                    rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                }
+            } else {
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    float v = 0.f;
+                    if (c < n) {
+                        v = fast ? (float)r.f : (float)(r.f * r.cf);
+                        rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                        if (!fast && (c & 3) == 3) rec_rescale(r);
+                    }
+                    Tw[(c & 1) * 2048 + i * 64 + (lane ^ i)] = v;
+                }
+            }
+            __syncthreads();  // the wave's tile is complete
+#pragma unroll
+            for (int s2 = 0; s2 < 32; s2++) {
+                const int rl = 2 * s2 + h;
+                const float aev = Tw[li32 * 64 + (rl ^ li32)];
+                const float aod = Tw[2048 + li32 * 64 + (rl ^ li32)];
+                De = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev[s2], De, 0, 0, 0);
+                Do = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod[s2], Do, 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave has read its tile: reuse the space for the sums
+        // D[i][j] at lane j + 32 ((i / 4) % 2), register 4 (i / 8) + i % 4; i = (l - l0) / 2
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int i = 8 * (v >> 2) + 4 * h + (v & 3);
+            T[((wave * 2) * 32 + i) * 33 + li32] = De[v];
+            T[((wave * 2 + 1) * 32 + i) * 33 + li32] = Do[v];
+        }
+        __syncthreads();
+        {
+            // thread: real column j, l = l0 + 8 (tid % 8) + u
+            const int j = tid >> 3, cc = chunk * 16 + (j >> 1), pp = j & 1;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int ll = (tid & 7) * 8 + u, par = ll & 1, i = ll >> 1;
+                float sum = T[((0 * 2 + par) * 32 + i) * 33 + j];
+                sum += T[((1 * 2 + par) * 32 + i) * 33 + j];
+                sum += T[((2 * 2 + par) * 32 + i) * 33 + j];
+                sum += T[((3 * 2 + par) * 32 + i) * 33 + j];
+                if (ll < n && cc < ncol)
+                    reinterpret_cast<float*>(dst + (size_t)cc * out_stride + base + l0 + ll)[pp] = sum * osc;
+            }
+        }
+        __syncthreads();  // the sums are read before the next tile overwrites them
+    }
+}
+
 // One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m + s.
 template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
 __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp)[NRT], Rec (&rm)[NRT],
@@ -1990,7 +2125,13 @@ int sht_adjoint_nsplit(const ShtPlan& p, int nmap) {
 size_t sht_adjoint_work_bytes(const ShtPlan& p, int nmap) {
     const size_t ph = (sht_phase_bytes(p, nmap) + 255) & ~(size_t)255;
     const int ns = sht_adjoint_nsplit(p, nmap);
-    return ns > 1 ? ph + (size_t)ns * nmap * p.ncomp * p.nlm * 2 * p.prec : ph;
+    // ring-slice partial sums: of the vector kernels' split, or of the matrix-core kernel
+    int np = ns > 1 ? ns : 0;
+    if (p.prec == 4 && p.spin == 0) {
+        const int nsm = (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS;
+        if (nsm > 1) np = std::max(np, nsm);
+    }
+    return ph + (size_t)np * nmap * p.ncomp * p.nlm * 2 * p.prec;
 }
 
 template <typename T, int NRT, int NW, int BC>
@@ -2027,8 +2168,13 @@ static constexpr AdjShape ADJ_SHAPES[] = {
     {4, 8, 2, true, true},  {1, 1, 1, false, true}, {2, 1, 1, false, true}, {4, 1, 1, false, true},
     {8, 1, 1, false, true}, {8, 2, 1, false, true}, {8, 4, 1, false, true}};
 static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES[0]));
+// shape id of the matrix-core adjoint (FP32, spin 0): 16 complex columns and 256 ring pairs per block
+static constexpr int ADJ_SHAPE_MFMA = N_ADJ_SHAPES;
+
+static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4 && p.spin == 0; }
 
 static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
+    if (id == ADJ_SHAPE_MFMA) return adj_mfma_ok(p);
     if (id < 0 || id >= N_ADJ_SHAPES) return false;
     const AdjShape& a = ADJ_SHAPES[id];
     return (p.spin == 0 ? a.s0 : a.sp) && 64 * a.nrt * a.nw >= pps;
@@ -2078,6 +2224,27 @@ static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phas
     return -1;
 }
 
+static int adj_mfma_nslice(const ShtPlan& p) { return (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS; }
+
+static int launch_adj_s0_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
+                              double scale, void* part, hipStream_t st) {
+    ShtDev<float> d = dev_view<float>(p);
+    const int nslice = adj_mfma_nslice(p), nchunk = (ncol + 15) / 16;
+    void* dst = nslice > 1 ? part : alm;
+    const long dstride = nslice > 1 ? p.nlm : alm_stride;
+    hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0, st, d,
+                       ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride,
+                       nslice > 1 ? 1.0f : (float)scale, nslice);
+    PS_LAUNCH_CHECK();
+    if (nslice == 1) return 0;
+    const long n = (long)ncol * p.nlm;
+    hipLaunchKernelGGL(k_adj_sum_splits<float>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0,
+                       st, (const cplx<float>*)part, nslice, ncol, p.nlm, (cplx<float>*)alm, alm_stride,
+                       (float)scale);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 template <typename T>
 static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride, double scale,
                       void* part, hipStream_t st, int shape) {
@@ -2092,6 +2259,13 @@ static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, 
     const long dstride = ns > 1 ? p.nlm : alm_stride;
     const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
     if (shape < 0) shape = adj_shape_rule(p, many, pps);
+    if (shape == ADJ_SHAPE_MFMA) {
+        if (!adj_mfma_ok(p)) {
+            set_error("adjoint: invalid launch shape");
+            return -1;
+        }
+        return launch_adj_s0_mfma(p, ncol, phase, alm, alm_stride, scale, part, st);
+    }
     int rc = launch_adj_shape<T>(p, shape, ncol, phase, dst, dstride, scale, ns, pps, st);
     if (rc || ns == 1) return rc;
     const long n = (long)ncol * p.nlm;
@@ -2330,6 +2504,7 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
                 const int cover = 64 * ADJ_SHAPES[id].nrt * ADJ_SHAPES[id].nw;
                 if (adj_shape_valid(p, id, pps) && (cover <= 4 * pps || cover == 64)) add(id);
             }
+            if (adj_mfma_ok(p)) add(ADJ_SHAPE_MFMA);
             break;
         }
         default: set_error("launch shapes: stage must be 0..3"); return -1;

@@ -217,6 +217,69 @@ def cg_update_p(num, den, z, p):
     check(load().ps_cg_update_p(_prec(p), p.numel(), ptr(num), ptr(den), ptr(z), ptr(p), stream_handle()))
 
 
-__all__ = ["lmat_apply", "mdot", "cg_update_xr", "cg_update_p", "convolve_F_lmajor", "weight_alm_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
+# ------------------------------------------------------------ batched CG (many maps in lock step)
+def multiplyRR_batch(a, b, out=None):
+    """multiplyRR of every stack of a (B, ...) with the shared b (a[0].shape)."""
+    _chk(a, b)
+    nb = a.shape[0]
+    assert b.numel() * nb == a.numel(), "b must have the shape of one stack of a"
+    if out is None:
+        out = torch.empty_like(a)
+    check(load().ps_pixel_products_batch(_prec(a), 0, nb, ptr(a), ptr(b), None, None, b.numel(), 1.0, ptr(out),
+                                         stream_handle()))
+    return out
+
+
+def lmat_apply_batch(x, lmax, mat=None, pre=None, post=None, out=None, alpha=1.0, accumulate=False):
+    """lmat_apply on every vector of x (B, nf, nlm)."""
+    _chk(x)
+    assert x.dim() == 3 and x.shape[2] == ((lmax + 1) * (lmax + 2)) // 2
+    dev = x.device
+    m = _dev_f64(mat, dev) if mat is not None else None
+    pr = _dev_f64(pre, dev) if pre is not None else None
+    po = _dev_f64(post, dev) if post is not None else None
+    if out is None:
+        out = torch.empty_like(x)
+    _chk(out)
+    check(load().ps_lmat_apply_batch(_prec(x), x.shape[0], x.shape[1], int(lmax), ptr(m) if m is not None else None,
+                                     ptr(pr) if pr is not None else None, ptr(po) if po is not None else None,
+                                     ptr(x), ptr(out), float(alpha), int(accumulate), stream_handle()))
+    return out
+
+
+_dot_scratch_b = {}
+
+
+def mdot_batch(x, y, lmax, out=None):
+    """CGD.dot of each system of a batch, x, y (B, nf, nlm) -> (B,) float64 device tensor."""
+    _chk(x, y)
+    nb, nf = x.shape[0], x.shape[1]
+    key = (x.device.index or 0, nb)
+    s = _dot_scratch_b.get(key)
+    if s is None:
+        s = torch.empty(int(load().ps_mdot_batch_scratch_bytes(nb)) // 8, dtype=torch.float64, device=x.device)
+        _dot_scratch_b[key] = s
+    if out is None:
+        out = torch.empty(nb, dtype=torch.float64, device=x.device)
+    check(load().ps_mdot_batch(_prec(x), nb, nf, int(lmax), ptr(x), ptr(y), ptr(s), ptr(out), stream_handle()))
+    return out
+
+
+def cg_update_xr_batch(rz, pAp, active, p, Ap, x, r):
+    _chk(rz, pAp, p, Ap, x, r)
+    nb = p.shape[0]
+    check(load().ps_cg_update_xr_batch(_prec(x), nb, x[0].numel(), ptr(rz), ptr(pAp), ptr(active), ptr(p), ptr(Ap),
+                                       ptr(x), ptr(r), stream_handle()))
+
+
+def cg_update_p_batch(num, den, active, z, p):
+    _chk(num, den, z, p)
+    nb = p.shape[0]
+    check(load().ps_cg_update_p_batch(_prec(p), nb, p[0].numel(), ptr(num), ptr(den), ptr(active), ptr(z), ptr(p),
+                                      stream_handle()))
+
+
+__all__ = ["lmat_apply", "mdot", "cg_update_xr", "cg_update_p", "lmat_apply_batch", "mdot_batch",
+           "cg_update_xr_batch", "cg_update_p_batch", "multiplyRR_batch", "convolve_F_lmajor", "weight_alm_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
            "multiplyRR", "multiplyRR_sym", "multiply", "multiply_asym", "radial_sum", "tau_sum_n0",
            "outer_product", "_abi"]

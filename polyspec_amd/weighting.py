@@ -190,6 +190,48 @@ class Weightings:
         return self.base._ret(out, npy)
 
 
+    # ------------------------------------------------ batched optimal S^-1
+    def applySinv_optimal_batch(self, input_maps, preconditioner='pseudo_inverse', nstep=25, thresh=1e-4,
+                                verb=False, lmax=None):
+        """applySinv_optimal of every map set of a stack (B, nf, Npix) -> (B, nf, nlm(lmax)).
+
+        The B systems run the reference's CG (weighting.py:148-184, 326-351) in lock step with
+        their own step lengths; each stops at the iteration where its own error falls below
+        thresh, as a separate call would, while the transforms of all B run batched."""
+        if not hasattr(self, 'noise_cov'):
+            raise Exception("The CGD solver requires a pixel-space noise covariance")
+        if preconditioner not in ('harmonic', 'pseudo_inverse'):
+            raise Exception("batched CG supports the 'harmonic' and 'pseudo_inverse' preconditioners")
+        if lmax is None:
+            lmax = self.base.lmax
+        npy = not isinstance(input_maps, torch.Tensor)
+        d = input_maps if not npy else torch.from_numpy(np.ascontiguousarray(input_maps))
+        d = d.to(self.base.device, self.base.rdtype).reshape(d.shape[0], self.nf, -1).contiguous()
+        cgd = BatchCGD(self, d, x0=self.wiener_start_batch(d), preconditioner=preconditioner, verb=verb)
+        cgd.run(nstep=nstep, thresh=thresh)
+        self.last_cg = cgd
+        out = cgd.get_ivar_lm()
+        if lmax != self.base.lmax:
+            out = truncate_lm(out, self.base.lmax, lmax)
+        return self.base._ret(out, npy)
+
+    def wiener_start_batch(self, d):
+        a = self.base.to_lm_batch(d)
+        t = K.lmat_apply_batch(a, self.base.lmax, mat=self.invCtot_d, post=self.beam_l_d)
+        return K.lmat_apply_batch(t, self.base.lmax, mat=self.Cl_l_d)
+
+    def apply_A_batch(self, x):
+        """apply_A on every system of a batch, x (B, nf, nlm(base_lmax))."""
+        L = self.base.lmax
+        bx = K.lmat_apply_batch(x, L, post=self.beam_l_d)
+        m = self.base.to_map_batch(bx)
+        m = K.multiplyRR_batch(m, self.wpix_d)
+        a = self.base.to_lm_batch(m)
+        out = K.lmat_apply_batch(x, L, mat=self.inv_Cl_l_d)
+        K.lmat_apply_batch(a, L, post=self.beam_l_d, out=out, accumulate=True)
+        return out
+
+
 _trunc_idx = {}
 
 
@@ -321,3 +363,96 @@ class CGD:
 
     def get_ivar_map(self):
         return self.base.to_map(self.get_ivar_lm())
+
+
+class BatchCGD:
+    """CGD (weighting.py:217-367) for B systems at once: vectors (B, nf, nlm), one step
+    length and one error per system, converged systems frozen (their x, r, p unchanged)."""
+
+    def __init__(self, weightings, d, x0=None, preconditioner='pseudo_inverse', verb=False):
+        self.preconditioner = preconditioner
+        self.base = weightings.base
+        self.weightings = weightings
+        self.verb = verb
+        self.L = self.base.lmax
+        W = weightings
+        b0 = self._get_b(d)
+        if x0 is not None:
+            self.x0 = x0
+            b = b0 - W.apply_A_batch(x0)
+        else:
+            self.x0 = torch.zeros_like(b0)
+            b = b0
+        self.nb = b0.shape[0]
+        self.x = torch.zeros_like(b)
+        self.r = b.clone()
+        z = self._apply_M(self.r)
+        self.rz = self.dot(self.r, z)
+        self.rz0 = self.dot(b0, self._apply_M(b0))
+        self.p = z
+        self.active = torch.ones(self.nb, dtype=torch.int32, device=b.device)
+        self.i = 0
+        self.iterations = np.zeros(self.nb, dtype=np.int64)  # steps each system took
+        self.err = (self.rz / self.rz0).cpu().numpy()
+        self.history = []
+
+    def dot(self, a, b):
+        return K.mdot_batch(a, b, self.L)
+
+    def _get_b(self, d):
+        W = self.weightings
+        m = K.multiplyRR_batch(d.contiguous(), W.Ninv_d)
+        a = self.base.to_lm_batch(m)
+        return K.lmat_apply_batch(a, self.L, post=W.beam_l_d)
+
+    def _apply_M(self, r):
+        W = self.weightings
+        if self.preconditioner == 'harmonic':
+            return K.lmat_apply_batch(r, self.L, mat=W.Sigma_l_d)
+        alm = K.lmat_apply_batch(r, self.L, mat=W.Sigma_l_d)
+        sig = K.lmat_apply_batch(alm, self.L, mat=W.inv_Cl_l_d)
+        t = K.lmat_apply_batch(alm, self.L, post=W.bn_d)
+        m = self.base.to_map_batch(t)
+        m = K.multiplyRR_batch(m, W.noise_cov_d)
+        a = self.base.to_lm_batch(m)
+        K.lmat_apply_batch(a, self.L, post=W.bnA_d, out=sig, accumulate=True)
+        return K.lmat_apply_batch(sig, self.L, mat=W.Sigma_l_d)
+
+    def step(self):
+        """One CG iteration of every active system (weighting.py:326-340)."""
+        Ap = self.weightings.apply_A_batch(self.p)
+        pAp = self.dot(self.p, Ap)
+        K.cg_update_xr_batch(self.rz, pAp, self.active, self.p, Ap, self.x, self.r)
+        del Ap
+        z = self._apply_M(self.r)
+        next_rz = self.dot(self.r, z)
+        K.cg_update_p_batch(next_rz, self.rz, self.active, z, self.p)
+        self.rz = torch.where(self.active.bool(), next_rz, self.rz)
+        self.i += 1
+        act = self.active.cpu().numpy().astype(bool)
+        self.iterations[act] += 1
+        self.err = (self.rz / self.rz0).cpu().numpy()
+        self.history.append((self.i, self.err.copy()))
+
+    def run(self, nstep=10, thresh=1e-6):
+        """Iterate until every system has rz/rz0 < thresh (weighting.py:342-351, per system)."""
+        for _ in range(nstep):
+            self.step()
+            done = self.err < thresh
+            if self.verb:
+                print("batched CGD step %d: max error = %.2e, %d of %d converged" %
+                      (self.i, self.err.max(), int(done.sum()), self.nb))
+            if done.any():
+                self.active.copy_(torch.from_numpy((~done).astype(np.int32)))
+            if done.all():
+                break
+        left = int((self.err >= thresh).sum())
+        if left:
+            print("batched CGD: %d of %d systems did not converge to %.2e after %d steps" % (left, self.nb, thresh,
+                                                                                          nstep))
+
+    def get_wiener_lm(self):
+        return self.x + self.x0
+
+    def get_ivar_lm(self):
+        return K.lmat_apply_batch((self.x + self.x0).contiguous(), self.L, mat=self.weightings.inv_Cl_l_d)

@@ -35,19 +35,18 @@ def _case(nside, lmax, spin, nmaps, seed):
 
 @pytest.mark.parametrize("nside,lmax,prec,spin,nmaps", [
     (32, 64, 32, 0, 1), (32, 64, 32, 0, 20), (32, 64, 64, 0, 3), (32, 64, 32, 2, 5), (32, 64, 64, 2, 1),
-    (16, 47, 32, 1, 3), (64, 128, 32, 0, 2), (64, 128, 64, 2, 2)])
+    (16, 47, 32, 1, 3), (64, 128, 32, 0, 2), (64, 128, 64, 2, 2), (256, 512, 32, 0, 17)])
 def test_every_candidate_shape_matches_oracle(gpu, nside, lmax, prec, spin, nmaps):
     from polyspec_amd._abi import ShtPlan
     a, m, ref_map, ref_alm = _case(nside, lmax, spin, nmaps, 4000 + nside + lmax + nmaps + 100 * spin)
     plan = ShtPlan(nside, lmax, spin, prec)
-    assert plan.shapes(nmaps) == [plan.shapes(nmaps)[i] for i in range(4)]
     ta = torch.from_numpy(a).to(gpu, plan.cdtype)
     tm = torch.from_numpy(m).to(gpu, plan.rdtype)
     tol = 1e-11 if prec == 64 else 2e-6
     for stage in range(4):
         cands = plan.shape_candidates(nmaps, stage)
         assert len(cands) >= 1
-        assert plan.shapes(nmaps)[stage] in cands or stage == 3
+        assert plan.shapes(nmaps)[stage] in cands  # the built-in choice is a candidate
         for c in cands:
             print(f"nside={nside} lmax={lmax} prec={prec} spin={spin} nmaps={nmaps} stage={stage} shape={c}",
                   flush=True)
