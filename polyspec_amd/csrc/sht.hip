@@ -1815,6 +1815,164 @@ __global__ __launch_bounds__(256) void k_leg_adj_s0_mfma(ShtDev<float> d, int nc
     }
 }
 
+// ------------------------------------- Legendre adjoint, spin s, FP32 matrix cores
+// Per map, with R, I the phases of the two real maps summed (+) and differenced (-) over
+// north / south rings, lam+- = (f+ +- f-) / 2 the two recursions' combinations, and for
+// l + m + s even (Rp, Rm, Ip, Im) = (R+, R-, I+, I-), for odd (R-, R+, I-, I+):
+//   G += -(lam+ Rp + i lam- Im),   C += -(lam+ Ip - i lam- Rm).
+// Written as f+ B+ + f- B- with B+- = (Bp +- Bn) / 2, where Bp holds the lam+ factors and
+// Bn the lam- factors of the four real outputs (G re, G im, C re, C im) of a map:
+//   even: Bp = -(R+re, R+im, I+re, I+im), Bn = (I-im, -I-re, -R-im, R-re)
+//   odd:  Bp = -(R-re, R-im, I-re, I-im), Bn = (I+im, -I+re, -R+im, R+re)
+// so both recursions feed the matrix cores directly: lanes 0-31 of a wave run f+ and lanes
+// 32-63 f- of the wave's 32 ring pairs, and per 64 values of l a wave issues 64
+// v_mfma_f32_32x32x2f32 (A = f+- of 32 values of l of one parity x 2 rings, B = B+- of 2 rings
+// x 32 real columns = 8 maps).  One block = 8 waves = 256 ring pairs, one m, 8 maps.
+constexpr int ADJM_SP_PAIRS = 256;
+
+__global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nmap,
+                                                         const cplx<float>* __restrict__ phase,
+                                                         cplx<float>* __restrict__ out, long out_stride, float osc,
+                                                         int nslice) {
+    // f tiles [wave][sign][parity][l / 2][ring ^ (l / 2)], then the waves' sums [w][par][i][33]
+    __shared__ __align__(16) float T[8 * 4 * 32 * 32];
+    const int nchunk = (nmap + 7) / 8;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = lb % nchunk;
+    const int sl = (lb / nchunk) % nslice;
+    const int m = m_of_q(lb / (nchunk * nslice), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int M1 = d.mmax + 1;
+    const int wring = sl * ADJM_SP_PAIRS + wave * 32;  // first ring pair of this wave
+    cplx<float>* dst = nslice > 1 ? out + (size_t)sl * 2 * nmap * out_stride : out;
+
+    // recursion f+ (h = 0) or f- (h = 1) of ring pair wring + li32
+    const int jr = wring + li32;
+    const int jrc = jr < d.npairs ? jr : d.npairs - 1;
+    const double x = d.cth[jrc];
+    const double sgb = h ? -1.0 : 1.0;
+    Rec r;
+    rec_seed(r, d.seedL[h * M1 + m] + d.seedA[h * M1 + m] * d.l2sh[jrc] + d.seedB[h * M1 + m] * d.l2ch[jrc],
+             d.seedS[h * M1 + m]);
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const int lstart = m > d.spin ? m : d.spin;
+    const int p0 = (lstart + m + d.spin) & 1;  // tile rows of parity p0 are the "odd" type
+
+    // B operands of ring pair wring + 2 s + h, real column li32 = 4 (map) + q:
+    // bp0 / bm0 for rows of even l - lstart, bp1 / bm1 for odd
+    float bp0[16], bm0[16], bp1[16], bm1[16];
+    {
+        const int b = chunk * 8 + (li32 >> 2), q = li32 & 3;
+        const size_t pstride = (size_t)M1 * d.nrings;
+        const cplx<float>* PR = phase + (size_t)(2 * b) * pstride + (size_t)m * d.nrings;
+        const cplx<float>* PI = PR + pstride;
+#pragma unroll
+        for (int s2 = 0; s2 < 16; s2++) {
+            const int jb = wring + 2 * s2 + h;
+            cplx<float> Rn{0.f, 0.f}, Rs{0.f, 0.f}, In{0.f, 0.f}, Is{0.f, 0.f};
+            if (jb < d.npairs && b < nmap) {
+                const int rs = d.nrings - 1 - jb;
+                Rn = PR[jb];
+                In = PI[jb];
+                if (rs != jb) {
+                    Rs = PR[rs];
+                    Is = PI[rs];
+                }
+            }
+            const cplx<float> Ra = cadd(Rn, Rs), Rd = csub(Rn, Rs), Ia = cadd(In, Is), Id = csub(In, Is);
+            float pe, ne, po, no;  // Bp, Bn of the even and the odd type
+            switch (q) {
+                case 0: pe = -Ra.re; ne = Id.im; po = -Rd.re; no = Ia.im; break;
+                case 1: pe = -Ra.im; ne = -Id.re; po = -Rd.im; no = -Ia.re; break;
+                case 2: pe = -Ia.re; ne = -Rd.im; po = -Id.re; no = -Ra.im; break;
+                default: pe = -Ia.im; ne = Rd.re; po = -Id.im; no = Ra.re; break;
+            }
+            // row parity 0 (l = lstart + even) is the even type when p0 == 0
+            const float p_0 = p0 ? po : pe, n_0 = p0 ? no : ne, p_1 = p0 ? pe : po, n_1 = p0 ? ne : no;
+            bp0[s2] = 0.5f * (p_0 + n_0);
+            bm0[s2] = 0.5f * (p_0 - n_0);
+            bp1[s2] = 0.5f * (p_1 + n_1);
+            bm1[s2] = 0.5f * (p_1 - n_1);
+        }
+    }
+
+    // coefficients with m <= l < spin vanish identically
+    for (int i = tid; i < (lstart - m) * 16; i += 512) {
+        const int l = m + i / 16, b = chunk * 8 + (i % 16) / 2, g = i % 2;
+        if (l <= d.lmax && b < nmap) dst[(size_t)(2 * b + g) * out_stride + base + l] = cplx<float>{0.f, 0.f};
+    }
+    const double* cf4 = d.coef + 4 * base;
+    const bool skip = __syncthreads_and(jr >= d.npairs || d.dead[(size_t)m * d.npairs + jrc]);
+    float* Tw = T + wave * 4096 + h * 2048;  // this lane's sign
+    const float* Ta = T + wave * 4096;
+    bool fast = false;
+    for (int l0 = lstart; l0 <= d.lmax; l0 += 64) {
+        const int n = min(64, d.lmax + 1 - l0);
+        v16f D0 = v16f{0}, D1 = v16f{0};
+        if (!skip) {
+            double pal = 0.0, pbe = 0.0, pga = 0.0;
+            if (lane < n) {
+                pal = cf4[4 * (l0 + lane)];
+                pbe = cf4[4 * (l0 + lane) + 1];
+                pga = cf4[4 * (l0 + lane) + 2];
+            }
+            if (!fast) fast = __all(r.k == 0);
+            if (fast && n == 64) {
+#pragma unroll 4
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 1024 + i * 32 + (li32 ^ i)] = (float)r.f;
+                    rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                }
+            } else {
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    float v = 0.f;
+                    if (c < n) {
+                        v = fast ? (float)r.f : (float)(r.f * r.cf);
+                        rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                        if (!fast && (c & 3) == 3) rec_rescale(r);
+                    }
+                    Tw[(c & 1) * 1024 + i * 32 + (li32 ^ i)] = v;
+                }
+            }
+            __syncthreads();  // the wave's tiles are complete
+#pragma unroll
+            for (int s2 = 0; s2 < 16; s2++) {
+                const int rl = 2 * s2 + h, sw = li32 * 32 + (rl ^ li32);
+                D0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[sw], bp0[s2], D0, 0, 0, 0);
+                D0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[2048 + sw], bm0[s2], D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[1024 + sw], bp1[s2], D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[2048 + 1024 + sw], bm1[s2], D1, 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave has read its tiles: reuse the space for the sums
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int i = 8 * (v >> 2) + 4 * h + (v & 3);
+            T[((wave * 2) * 32 + i) * 33 + li32] = D0[v];
+            T[((wave * 2 + 1) * 32 + i) * 33 + li32] = D1[v];
+        }
+        __syncthreads();
+        {
+            // thread: real column j = 4 (map) + q, l = l0 + 4 (tid % 16) + u
+            const int j = tid >> 4, b = chunk * 8 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int ll = (tid & 15) * 4 + u, par = ll & 1, i = ll >> 1;
+                float sum = 0.f;
+#pragma unroll
+                for (int w = 0; w < 8; w++) sum += T[((w * 2 + par) * 32 + i) * 33 + j];
+                if (ll < n && b < nmap)
+                    reinterpret_cast<float*>(dst + (size_t)(2 * b + g) * out_stride + base + l0 + ll)[pp] = sum * osc;
+            }
+        }
+        __syncthreads();  // the sums are read before the next tile overwrites them
+    }
+}
+
 // One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m + s.
 template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
 __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp)[NRT], Rec (&rm)[NRT],
@@ -2127,7 +2285,7 @@ size_t sht_adjoint_work_bytes(const ShtPlan& p, int nmap) {
     const int ns = sht_adjoint_nsplit(p, nmap);
     // ring-slice partial sums: of the vector kernels' split, or of the matrix-core kernel
     int np = ns > 1 ? ns : 0;
-    if (p.prec == 4 && p.spin == 0) {
+    if (p.prec == 4) {
         const int nsm = (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS;
         if (nsm > 1) np = std::max(np, nsm);
     }
@@ -2168,10 +2326,11 @@ static constexpr AdjShape ADJ_SHAPES[] = {
     {4, 8, 2, true, true},  {1, 1, 1, false, true}, {2, 1, 1, false, true}, {4, 1, 1, false, true},
     {8, 1, 1, false, true}, {8, 2, 1, false, true}, {8, 4, 1, false, true}};
 static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES[0]));
-// shape id of the matrix-core adjoint (FP32, spin 0): 16 complex columns and 256 ring pairs per block
+// shape id of the matrix-core adjoint (FP32): 256 ring pairs and 16 complex columns (spin 0) or
+// 8 maps (spin s) per block
 static constexpr int ADJ_SHAPE_MFMA = N_ADJ_SHAPES;
 
-static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4 && p.spin == 0; }
+static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4; }
 
 static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
     if (id == ADJ_SHAPE_MFMA) return adj_mfma_ok(p);
@@ -2183,7 +2342,9 @@ static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
 // Built-in rule.  Latency (few columns, ring slices): up to 8 ring pairs per thread, one column
 // chunk of 2 (spin 0) or 1 map (spin s) per block.  Throughput (many columns): 4 ring pairs per
 // thread and more waves per block.
-static int adj_shape_rule(const ShtPlan& p, bool many, int pps) {
+static int adj_shape_rule(const ShtPlan& p, int ncol, bool many, int pps) {
+    // FP32 batches: the matrix-core kernel (as for the synthesis)
+    if (p.prec == 4 && (p.spin == 0 ? ncol > 8 : ncol / 2 > 2)) return ADJ_SHAPE_MFMA;
     const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
     if (many) {
         static constexpr int thr[6] = {2, 2, 2, 6, 7, 8};
@@ -2224,17 +2385,25 @@ static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phas
     return -1;
 }
 
+static_assert(ADJM_PAIRS == ADJM_SP_PAIRS, "both matrix-core adjoints slice rings alike");
 static int adj_mfma_nslice(const ShtPlan& p) { return (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS; }
 
-static int launch_adj_s0_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
-                              double scale, void* part, hipStream_t st) {
+static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
+                           double scale, void* part, hipStream_t st) {
     ShtDev<float> d = dev_view<float>(p);
-    const int nslice = adj_mfma_nslice(p), nchunk = (ncol + 15) / 16;
+    const int nslice = adj_mfma_nslice(p);
     void* dst = nslice > 1 ? part : alm;
     const long dstride = nslice > 1 ? p.nlm : alm_stride;
-    hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0, st, d,
-                       ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride,
-                       nslice > 1 ? 1.0f : (float)scale, nslice);
+    const float osc = nslice > 1 ? 1.0f : (float)scale;
+    if (p.spin == 0) {
+        const int nchunk = (ncol + 15) / 16;
+        hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0, st,
+                           d, ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+    } else {
+        const int nmap = ncol / 2, nchunk = (nmap + 7) / 8;
+        hipLaunchKernelGGL(k_leg_adj_sp_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0, st,
+                           d, nmap, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+    }
     PS_LAUNCH_CHECK();
     if (nslice == 1) return 0;
     const long n = (long)ncol * p.nlm;
@@ -2258,13 +2427,13 @@ static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, 
     void* dst = ns > 1 ? part : alm;
     const long dstride = ns > 1 ? p.nlm : alm_stride;
     const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
-    if (shape < 0) shape = adj_shape_rule(p, many, pps);
+    if (shape < 0) shape = adj_shape_rule(p, ncol, many, pps);
     if (shape == ADJ_SHAPE_MFMA) {
         if (!adj_mfma_ok(p)) {
             set_error("adjoint: invalid launch shape");
             return -1;
         }
-        return launch_adj_s0_mfma(p, ncol, phase, alm, alm_stride, scale, part, st);
+        return launch_adj_mfma(p, ncol, phase, alm, alm_stride, scale, part, st);
     }
     int rc = launch_adj_shape<T>(p, shape, ncol, phase, dst, dstride, scale, ns, pps, st);
     if (rc || ns == 1) return rc;
@@ -2464,7 +2633,7 @@ int sht_default_shape(const ShtPlan& p, int nmap, int stage) {
         case SHT_STAGE_ADJ: {
             const int ns = sht_adjoint_nsplit(p, nmap);
             const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
-            return adj_shape_rule(p, many, (p.npairs + ns - 1) / ns);
+            return adj_shape_rule(p, ncol, many, (p.npairs + ns - 1) / ns);
         }
     }
     return -1;
