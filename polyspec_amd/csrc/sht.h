@@ -13,6 +13,7 @@ struct RingInfo {
     int M;       // FFT length (nphi if a power of two, else the Bluestein length)
     int logM;
     int voff;    // offset of the Bluestein kernel in the table, -1 for power-of-two rings
+    int coff;    // offset of the chirp exp(i pi t^2 / nphi), t < nphi, in the chirp table (Bluestein rings)
 };
 
 // Launch shapes for one batch width (ncol = nmaps * ncomp), chosen by timing in sht_autotune
@@ -35,7 +36,7 @@ struct ShtPlan {
     RingInfo* d_rings = nullptr;
     double *d_l2s = nullptr, *d_l2sh = nullptr, *d_l2ch = nullptr, *d_seedL = nullptr, *d_seedS = nullptr;
     int *d_seedA = nullptr, *d_seedB = nullptr;
-    void *d_cth = nullptr, *d_tw = nullptr, *d_blue = nullptr, *d_coef = nullptr;
+    void *d_cth = nullptr, *d_tw = nullptr, *d_blue = nullptr, *d_coef = nullptr, *d_chirp = nullptr;
     // dead[m * npairs + jp] = 1: |lambda_lm| (both spin signs) < 2^-200 at ring pair jp for every l,
     // so the pair contributes nothing at order m (the evanescent region of high m near the poles)
     unsigned char* d_dead = nullptr;

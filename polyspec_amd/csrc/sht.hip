@@ -85,7 +85,8 @@ int ShtPlan::build() {
     // ---- rings
     std::vector<RingInfo> rings(nrings);
     std::map<int, int> blue_off;  // nphi -> offset into the Bluestein table
-    long blue_total = 0;
+    std::map<int, int> chirp_off;  // nphi -> offset into the chirp table
+    long blue_total = 0, chirp_total = 0;
     maxM = 1;
     for (int i = 1; i <= nrings; i++) {
         int ir = (i > 2 * N) ? 4 * N - i : i;
@@ -109,14 +110,19 @@ int ShtPlan::build() {
         r.M = (int)M;
         r.logM = ilog2(M);
         r.voff = -1;
+        r.coff = -1;
         if (!pow2) {
             auto it = blue_off.find(r.nphi);
             if (it == blue_off.end()) {
                 blue_off[r.nphi] = (int)blue_total;
+                chirp_off[r.nphi] = (int)chirp_total;
                 r.voff = blue_total;
+                r.coff = chirp_total;
                 blue_total += M;
+                chirp_total += r.nphi;
             } else {
                 r.voff = it->second;
+                r.coff = chirp_off[r.nphi];
             }
         }
         maxM = std::max<long>(maxM, M);
@@ -166,6 +172,20 @@ int ShtPlan::build() {
                 if (k & (1L << b)) rk |= 1u << (lg - 1 - b);
             blue[2 * (kv.second + rk)] = v[k].real();
             blue[2 * (kv.second + rk) + 1] = v[k].imag();
+        }
+    }
+
+    // Bluestein chirps exp(i pi t^2 / n), t < n, per distinct ring length (exact argument
+    // reduction t^2 mod 2n, evaluated in FP64): the pre- and post-multiplications of the ring
+    // FFTs read them instead of evaluating sincospi per element
+    std::vector<double> chirp((size_t)std::max<long>(1, 2 * chirp_total));
+    for (auto& kv : chirp_off) {
+        const int n = kv.first;
+        for (long t = 0; t < n; t++) {
+            double s, c;
+            sincos(M_PI * (double)((t * t) % (2L * n)) / (double)n, &s, &c);
+            chirp[2 * (kv.second + t)] = c;
+            chirp[2 * (kv.second + t) + 1] = s;
         }
     }
 
@@ -269,16 +289,19 @@ int ShtPlan::build() {
         if (upload((float**)&d_tw, tf)) return -2;
         std::vector<float> bf(blue.begin(), blue.end());
         if (upload((float**)&d_blue, bf)) return -2;
+        std::vector<float> cf(chirp.begin(), chirp.end());
+        if (upload((float**)&d_chirp, cf)) return -2;
     } else {
         if (upload((double**)&d_tw, tw)) return -2;
         if (upload((double**)&d_blue, blue)) return -2;
+        if (upload((double**)&d_chirp, chirp)) return -2;
     }
     return 0;
 }
 
 ShtPlan::~ShtPlan() {
     void* ptrs[] = {d_rings, d_l2s, d_l2sh, d_l2ch, d_seedL, d_seedS, d_seedA,
-                    d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist, d_dead};
+                    d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist, d_dead, d_chirp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -295,6 +318,7 @@ struct ShtDev {
     const double* cth;  // recursion runs in FP64 for both precisions
     const cplx<T>* tw;
     const cplx<T>* blue;
+    const cplx<T>* chirp;  // exp(i pi t^2 / nphi) of the Bluestein rings, RingInfo::coff
     const double* coef;  // (alpha, beta, gamma, 0) per idx(l, m)
     const unsigned char* dead;  // (mmax+1) x npairs, see ShtPlan::d_dead
     long maxM;
@@ -323,6 +347,7 @@ static ShtDev<T> dev_view(const ShtPlan& p) {
     d.cth = (const double*)p.d_cth;
     d.tw = (const cplx<T>*)p.d_tw;
     d.blue = (const cplx<T>*)p.d_blue;
+    d.chirp = (const cplx<T>*)p.d_chirp;
     d.coef = (const double*)p.d_coef;
     d.maxM = p.maxM;
     return d;
@@ -1399,7 +1424,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
                 cplx<T> acc{a.re - b.im, a.im + b.re};  // A + i B
                 acc.re += Am[u].re + Bm[u].im;          // conj(A) + i conj(B) of the mirror
                 acc.im += Bm[u].re - Am[u].im;
-                if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));  // Bluestein pre-chirp
+                if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);  // Bluestein pre-chirp
                 buf[q * pitch + pidx(j)] = acc;
             }
         }
@@ -1426,7 +1451,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
             acc.im += B.re - A.im;
         }
         if (S == 1) {
-            if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));  // Bluestein pre-chirp
+            if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);  // Bluestein pre-chirp
             buf[q * pitch + pidx(j)] = acc;
         } else {
             part[i] = acc;
@@ -1438,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
             cplx<T> acc = part[r];
             for (int sl = 1; sl < S; sl++) acc = cadd(acc, part[sl * nitem + r]);
             const int j = r / NB, q = r - j * NB;
-            if (!pow2) acc = cmul(acc, expi_pi32<T>(j * j, n));
+            if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);
             buf[q * pitch + pidx(j)] = acc;
         }
     }
@@ -1465,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
         const T invM = T(1) / (T)M;
         for (int i = tid; i < NB * n; i += 256) {
             const int q = i / n, k = i - q * n;
-            const cplx<T> z = cmul(buf[q * pitch + pidx(k)], expi_pi32<T>(k * k, n));
+            const cplx<T> z = cmul(buf[q * pitch + pidx(k)], d.chirp[ri.coff + k]);
             if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
             if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
         }
@@ -1491,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
         cplx<T> v{T(0), T(0)};
         if (k < n) {
             v.re = in[k];
-            if (!pow2) v = cmul(v, expi_pi<T>((long)k * k, n));
+            if (!pow2) v = cmul(v, d.chirp[ri.coff + k]);
         }
         buf[k] = v;
     }
@@ -1514,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
         if (pow2) {
             C = buf[bitrev(j, logM)];
         } else {
-            cplx<T> g = cmul(buf[j], expi_pi<T>((long)j * j, n));
+            cplx<T> g = cmul(buf[j], d.chirp[ri.coff + j]);
             C = cplx<T>{g.re * invM, -g.im * invM};
         }
         P[(size_t)m * d.nrings] = cmul(C, expi_pi<T>(-(long)m * php, phq));
