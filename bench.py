@@ -223,13 +223,19 @@ def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=20
         tb = time.perf_counter() - t0
         its = W.last_cg.iterations.copy()
         t0 = time.perf_counter()
-        singles = [W.applySinv_optimal(ds[i], preconditioner='harmonic', nstep=nstep, thresh=thresh)
-                   for i in range(batch)]
+        singles, its_single = [], []
+        for i in range(batch):
+            singles.append(W.applySinv_optimal(ds[i], preconditioner='harmonic', nstep=nstep, thresh=thresh))
+            its_single.append(W.last_cg.i)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
-        diff = max(float((outb[i] - singles[i]).abs().max() / singles[i].abs().max()) for i in range(batch))
+        # FP32 solves whose statistic lands within rounding of the threshold can stop one iteration
+        # apart; solutions are compared where both stopped at the same iteration
+        same = [i for i in range(batch) if its_single[i] == its[i]]
+        diff = max(float((outb[i] - singles[i]).abs().max() / singles[i].abs().max()) for i in same) if same else None
         res["batched"] = {"maps": batch, "solves_per_s": batch / tb, "single_solves_per_s": batch / ts,
                           "speedup": ts / tb, "iterations_min_max": [int(its.min()), int(its.max())],
+                          "systems_stopping_at_another_iteration_than_single": batch - len(same),
                           "max_rel_diff_vs_single": diff,
                           "solver": "BatchCGD: harmonic preconditioner, every map its own step lengths and "
                                     "stopping iteration, transforms of all maps batched"}
