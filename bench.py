@@ -272,6 +272,114 @@ def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=20
     return res
 
 
+def bench_sht_roundtrip(nside=1024, lmax=2048, nsets=64, reps=2):
+    """Batched TEB round trip: nsets seeded a_lm sets (C^TT = 2 pi 1e3/(l(l+1)), C^EE = 0.05 C^TT,
+    TE correlation 0.3, B = 0), spin-0 synthesis of T and spin-2 synthesis of (E, B) -> (Q, U), then
+    the adjoints (scaled by 4 pi/Npix) back to a_lm, each direction one launch over every set.
+    complex64 kernels against the complex128 kernels on the same inputs (the FP64 transforms match
+    the FP64 CPU oracle to 1e-11 at Nside <= 256 and satisfy adjointness to 1e-10 at this size).
+    Reported: per-field max |a32 - a64| / max |a64| and rms / rms of the round-trip a_lm and of the
+    synthesised maps (B, zero on input, relative to E), the milliseconds of every batched launch in both precisions, and the
+    transforms' rates (4 R nlm flops per spin-0 map, 8 R nlm per spin-2 map; bytes 8 nlm + 4 Npix
+    per FP32 map and direction)."""
+    import torch
+    from polyspec_amd._abi import ShtPlan
+    from polyspec_amd.base import getlm
+    dev = torch.device("cuda", 0)
+    ls, ms = getlm(lmax)
+    nlm = len(ls)
+    lf = np.maximum(ls, 1).astype(np.float64)
+    tt = np.where(ls >= 2, 2 * np.pi * 1e3 / (lf * (lf + 1)), 0.)
+    ee = 0.05 * tt
+    te = 0.3 * np.sqrt(tt * ee)
+    c21 = np.where(tt > 0, te / np.sqrt(np.where(tt > 0, tt, 1.)), 0.)
+    c22 = np.sqrt(np.maximum(ee - c21 ** 2, 0.))
+    to_d = lambda x: torch.from_numpy(x).to(dev)
+    c11, c21, c22 = to_d(np.sqrt(tt)), to_d(c21), to_d(c22)
+    m0 = to_d(ms == 0)
+    g = torch.Generator(device=dev).manual_seed(1024)
+
+    def gauss():
+        z = torch.randn((nsets, nlm), dtype=torch.complex128, device=dev, generator=g)
+        z[:, m0] = (z[:, m0].real * np.sqrt(2.)).to(torch.complex128)
+        return z
+    g1, g2 = gauss(), gauss()
+    aT = (c11 * g1).unsqueeze(1).contiguous()
+    aEB = torch.zeros((nsets, 2, nlm), dtype=torch.complex128, device=dev)
+    aEB[:, 0] = c21 * g1 + c22 * g2
+    del g1, g2
+    npix = 12 * nside * nside
+    R = 2 * nside
+    res = {"workload": "Nside=%d, lmax=%d, %d TEB a_lm sets: spin-0 (T) and spin-2 (E,B) synthesis, then "
+                       "adjoint; complex64 kernels vs complex128 kernels" % (nside, lmax, nsets),
+           "n_maps": 3 * nsets}
+
+    def run(prec, tune):
+        plans = (ShtPlan(nside, lmax, 0, prec), ShtPlan(nside, lmax, 2, prec))
+        ins = [a.to(plans[0].cdtype) for a in (aT, aEB)]
+        if tune:
+            for p, a in zip(plans, ins):
+                p.autotune(a.shape[0])
+        times = {"synthesis_T": [], "synthesis_EB": [], "adjoint_T": [], "adjoint_EB": []}
+        for it in range(reps + 1):
+            evs = []
+            maps, alms = [], []
+            for p, a, name in zip(plans, ins, ("T", "EB")):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                maps.append(p.synthesis(a))
+                e1.record()
+                evs.append(("synthesis_" + name, e0, e1))
+            for p, m, name in zip(plans, maps, ("T", "EB")):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                alms.append(p.adjoint(m, 4 * np.pi / npix))
+                e1.record()
+                evs.append(("adjoint_" + name, e0, e1))
+            torch.cuda.synchronize()
+            if it > 0:  # the first pass is a warm-up
+                for name, e0, e1 in evs:
+                    times[name].append(e0.elapsed_time(e1))
+        del plans
+        return maps, alms, {k: float(np.mean(v)) for k, v in times.items()}
+
+    log("  SHT round trip fp64 (%d sets, Nside %d, lmax %d) ..." % (nsets, nside, lmax))
+    maps64, alms64, t64 = run(64, False)
+    log("  SHT round trip fp32 ...")
+    maps32, alms32, t32 = run(32, True)
+
+    def relerr(x32, x64, comp, norm_comp=None):
+        d = (x32[:, comp].to(x64.dtype) - x64[:, comp])
+        ref = x64[:, comp if norm_comp is None else norm_comp]
+        mx = float(ref.abs().max())
+        return {"max_abs_diff_over_max": float(d.abs().max()) / mx if mx > 0 else None,
+                "rms_diff_over_rms": float(d.abs().pow(2).mean().sqrt() / ref.abs().pow(2).mean().sqrt())
+                if mx > 0 else None}
+    res["alm_roundtrip_rel_err"] = {"T": relerr(alms32[0], alms64[0], 0), "E": relerr(alms32[1], alms64[1], 0),
+                                    "B_over_E": relerr(alms32[1], alms64[1], 1, 0)}
+    res["map_rel_err"] = {"T": relerr(maps32[0], maps64[0], 0), "Q": relerr(maps32[1], maps64[1], 0),
+                          "U": relerr(maps32[1], maps64[1], 1)}
+    res["ms_per_launch_fp32"] = t32
+    res["ms_per_launch_fp64"] = t64
+    fl0 = 4. * R * nlm * nsets
+    flops = {"synthesis_T": fl0, "synthesis_EB": 2 * fl0, "adjoint_T": fl0, "adjoint_EB": 2 * fl0}
+    bts = {"synthesis_T": (8. * nlm + 4. * npix) * nsets, "adjoint_T": (8. * nlm + 4. * npix) * nsets}
+    bts["synthesis_EB"] = bts["adjoint_EB"] = 2 * bts["synthesis_T"]
+    res["tflops_fp32"] = {k: flops[k] / (t32[k] * 1e-3) / 1e12 for k in flops}
+    res["tflops_fp64"] = {k: flops[k] / (t64[k] * 1e-3) / 1e12 for k in flops}
+    res["hbm_gbs_fp32_algorithmic"] = {k: bts[k] / (t32[k] * 1e-3) / 1e9 for k in bts}
+    tot32, tot64 = sum(t32.values()), sum(t64.values())
+    res["round_trip_ms"] = {"fp32": tot32, "fp64": tot64, "fp32_speedup": tot64 / tot32}
+    res["maps_per_s_fp32"] = {"synthesis": 3 * nsets / ((t32["synthesis_T"] + t32["synthesis_EB"]) * 1e-3),
+                              "adjoint": 3 * nsets / ((t32["adjoint_T"] + t32["adjoint_EB"]) * 1e-3)}
+    log("  round trip: fp32 %.1f ms, fp64 %.1f ms; max rel alm err T %.2e E %.2e" %
+        (tot32, tot64, res["alm_roundtrip_rel_err"]["T"]["max_abs_diff_over_max"],
+         res["alm_roundtrip_rel_err"]["E"]["max_abs_diff_over_max"]))
+    del maps64, alms64, maps32, alms32
+    torch.cuda.empty_cache()
+    return res
+
+
 def log_bins(lmax, nbins):
     """nbins log-spaced l-bin edges on [2, lmax + 1], strictly increasing integers."""
     e = np.geomspace(2, lmax + 1, nbins + 1)
@@ -476,6 +584,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the second timing of the batched numerator replayed from its captured HIP graph")
     ap.add_argument("--no-radial", action="store_true", help="skip the radial-node optimizer section")
+    ap.add_argument("--no-sht", action="store_true",
+                    help="skip the batched TEB SHT round trip (Nside 1024, lmax 2048, 64 sets, FP32 vs FP64)")
+    ap.add_argument("--sht-sets", type=int, default=64)
     ap.add_argument("--bspec-nside", type=str, default="256",
                     help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
     ap.add_argument("--bspec-sims", type=int, default=16)
@@ -582,6 +693,10 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del src, dst
+
+    if not args.no_sht:
+        log("batched SHT round trip ...")
+        result["sht_roundtrip"] = bench_sht_roundtrip(nsets=args.sht_sets)
 
     if not args.no_plain:
         log("plain per-simulation FP64 path ...")

@@ -2,7 +2,9 @@
 
 Every case runs twice: once after torch's caching allocator and the library's cached
 workspace have been refilled with NaN, once after they have been refilled with zeros.
-The kernels are deterministic, so the two results must be finite and bit-identical; a
+The kernels are deterministic for fixed launch shapes (the built-in ones here: two
+autotuning passes may keep different shapes, whose FP32 sums round differently), so the
+two results must be finite and bit-identical; a
 kernel that reads device memory it never wrote fails here even where fresh device memory
 happens to hold zeros (or the values a previous identical run left behind).
 """
@@ -76,7 +78,7 @@ def test_bspec_reads_only_written_memory(gpu, prec):
     maps = [oracle.to_map(grf_alm(rng, tot, L, True), nside, L) for _ in range(3)]
 
     def run():
-        base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=prec)
+        base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=prec, autotune=False)
         W = Weightings(base, np.ones(12 * nside ** 2), dict(raw))
         bs = BSpecBin(base, np.ones(12 * nside ** 2), W.applySinv_ideal, [2, 6, 12, 20, 30, 42],
                       fields=['TTT', 'TTE', 'TEE', 'EEE'], parity='both')
@@ -102,7 +104,7 @@ def test_template_reads_only_written_memory(gpu, pol, prec):
     r, w = radial_nodes(nr)
 
     def run():
-        base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=prec)
+        base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=prec, autotune=False)
         mask = np.ones(12 * nside ** 2)
         W = Weightings(base, mask, dict(raw))
         tm = TSpecTemplate(base, mask, W.applySinv_ideal, templates, k, Tl, 2, lmax, r_values=r,
