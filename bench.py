@@ -429,13 +429,24 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
     bs.load_sims(lambda i: maps[1 + i], nsims, preload=True)
     torch.cuda.synchronize()
     t_h = time.perf_counter() - t0
+    bs.fused_sums = True
+    bs.Bl_numerator(maps[0])  # warm-up of the fused launch with nsims simulations (autotunes its tiles)
+    torch.cuda.synchronize()
     bs.kernel_events = []
     t0 = time.perf_counter()
     b32 = bs.Bl_numerator(maps[0])
     torch.cuda.synchronize()
     t_num = time.perf_counter() - t0
     ev = bs.kernel_events
+    fused_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in ev if n == 'multi']))
+    # the same sums as one launch for the cubic term and one per simulation
+    bs.fused_sums = False
+    bs.kernel_events = []
+    b32_sep = bs.Bl_numerator(maps[0])
+    torch.cuda.synchronize()
+    ev = bs.kernel_events
     bs.kernel_events = None
+    bs.fused_sums = True
     cub_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in ev if n == 3]))
     lin_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in ev if n == 4]))
     npix = 12 * nside * nside
@@ -443,11 +454,16 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
     # linear 3 x (2 complex products + 3 complex multiply-adds) = 108
     cub_tf = 42. * out["n_triplets"] * npix / (cub_ms * 1e-3) / 1e12
     lin_tf = 108. * out["n_triplets"] * npix / (lin_ms * 1e-3) / 1e12
+    fused_tf = (42. + 108. * nsims) * out["n_triplets"] * npix / (fused_ms * 1e-3) / 1e12
     out["batched_fp32"] = {"sims_per_s": nsims / (t_h + t_num), "s_H_maps": t_h, "s_numerator": t_num,
-                           "cubic_sum_ms": cub_ms, "cubic_sum_tflops": cub_tf,
-                           "linear_sum_ms": lin_ms, "linear_sum_tflops": lin_tf}
-    log("bspec nside %d: batched fp32 %.2f sims/s (H maps %.2fs, numerator %.2fs; cubic sum %.3f ms = %.1f TFLOP/s, "
-        "linear sum %.3f ms = %.1f TFLOP/s)" % (nside, nsims / (t_h + t_num), t_h, t_num, cub_ms, cub_tf, lin_ms, lin_tf))
+                           "fused_sums_ms": fused_ms, "fused_sums_tflops": fused_tf,
+                           "fused_sums_shape": _bspec_shape_report(),
+                           "per_launch_cubic_sum_ms": cub_ms, "per_launch_cubic_sum_tflops": cub_tf,
+                           "per_launch_linear_sum_ms": lin_ms, "per_launch_linear_sum_tflops": lin_tf,
+                           "fused_vs_per_launch_max_rel_diff": float(np.abs(b32 - b32_sep).max() / np.abs(b32_sep).max())}
+    log("bspec nside %d: batched fp32 %.2f sims/s (H maps %.2fs, numerator %.2fs; fused sums %.3f ms = %.1f TFLOP/s; "
+        "per launch: cubic %.3f ms = %.1f TFLOP/s, linear %.3f ms = %.1f TFLOP/s)" %
+        (nside, nsims / (t_h + t_num), t_h, t_num, fused_ms, fused_tf, cub_ms, cub_tf, lin_ms, lin_tf))
     data32 = maps[0].double()
     sims64 = [m.double() for m in maps[1:]]
     del bs, maps
@@ -468,6 +484,74 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
             % (nside, nsims / t_plain, out["speedup_batched_over_plain"], out["max_abs_diff_over_max_abs"]))
         del bs64, base64
         torch.cuda.empty_cache()
+    return out
+
+
+def _bspec_shape_report():
+    """Launch shapes the binned-bispectrum sum autotuner kept: {segment kinds: (shape id, ms per shape)}."""
+    from polyspec_amd import bspec_bin as B
+    return {str(k[0]) + " x%d" % k[1]: {"shape": v[0], "ms_per_shape": {i: round(t, 4) for i, t in v[1].items()}}
+            for k, v in B._multi_shapes.items()}
+
+
+def bench_bspec_mixed(spec, nbins=20):
+    """Config-4 style mixed resolutions: one BSpecBin analysis per (Nside, simulations) of spec
+    (lmax = min(2 Nside, 2000), nbins log-spaced bins, TTT/TTE/TEE/EEE, both parities, the
+    synthetic mask), all simulations preloaded, and the pixel sums of every analysis (each cubic
+    term and every simulation's linear term) in ONE ps_bspec_sums_multi launch
+    (Bl_numerator_multi).  Compared with the analyses run one after another (one fused launch
+    each): time, and the largest difference relative to the largest numerator entry."""
+    import torch
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.bspec_bin import BSpecBin, Bl_numerator_multi
+    from polyspec_amd.weighting import Weightings
+    fields = ['TTT', 'TTE', 'TEE', 'EEE']
+    analyses, datas, info = [], [], []
+    t0 = time.perf_counter()
+    for nside, nsims in spec:
+        lmax = min(2 * nside, 2000)
+        raw, tot, beam = spectra(nside)
+        mask = synthetic_mask(nside)
+        base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=32)
+        base.grf_device = True
+        W = Weightings(base, mask, dict(raw))
+        bs = BSpecBin(base, mask, W.applySinv_ideal, log_bins(lmax, nbins), fields=fields, parity='both')
+        mask_d = torch.from_numpy(mask).to(base.device, base.rdtype)
+        maps = [(base.generate_data(seed=int(8e6) + 1000 * nside + i) * mask_d).contiguous() for i in range(1 + nsims)]
+        bs.load_sims(lambda i: maps[1 + i], nsims, preload=True)
+        analyses.append(bs)
+        datas.append(maps[0])
+        info.append({"nside": nside, "lmax": lmax, "n_sims": nsims, "n_triplets": int(bs._trip.shape[0])})
+        del maps
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    Bl_numerator_multi(analyses, datas)  # warm-up (tile autotuning)
+    torch.cuda.synchronize()
+    analyses[0].kernel_events = []
+    t0 = time.perf_counter()
+    multi = Bl_numerator_multi(analyses, datas)
+    torch.cuda.synchronize()
+    t_multi = time.perf_counter() - t0
+    launch_ms = float(np.mean([e0.elapsed_time(e1) for n, e0, e1 in analyses[0].kernel_events if n == 'multi']))
+    analyses[0].kernel_events = None
+    for bs, d in zip(analyses, datas):
+        bs.Bl_numerator(d)  # warm-up (tile autotuning of each analysis's own launch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sep = [bs.Bl_numerator(d) for bs, d in zip(analyses, datas)]
+    torch.cuda.synchronize()
+    t_sep = time.perf_counter() - t0
+    flops = sum((42. + 108. * a["n_sims"]) * a["n_triplets"] * 12 * a["nside"] ** 2 for a in info)
+    diff = max(float(np.abs(m - s_).max() / np.abs(s_).max()) for m, s_ in zip(multi, sep))
+    out = {"analyses": info, "n_sims_total": int(sum(a["n_sims"] for a in info)),
+           "s_setup_incl_H_maps_of_sims": t_setup, "s_numerators_one_launch": t_multi,
+           "one_launch_ms": launch_ms, "one_launch_tflops": flops / (launch_ms * 1e-3) / 1e12,
+           "s_numerators_one_after_another": t_sep, "max_rel_diff_one_launch_vs_separate": diff,
+           "shapes": _bspec_shape_report()}
+    log("bspec mixed %s: one launch %.1f ms (%.1f TFLOP/s), numerators %.3fs (separately %.3fs), max rel diff %.1e" %
+        (spec, launch_ms, out["one_launch_tflops"], t_multi, t_sep, diff))
+    del analyses, datas
+    torch.cuda.empty_cache()
     return out
 
 
@@ -590,6 +674,8 @@ def main():
     ap.add_argument("--bspec-nside", type=str, default="256",
                     help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
     ap.add_argument("--bspec-sims", type=int, default=16)
+    ap.add_argument("--bspec-mixed", type=str, default="256:8,512:4,1024:2",
+                    help="Nside:simulations list of the mixed-resolution binned-bispectrum section ('' = skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -738,6 +824,10 @@ def main():
         log("binned bispectrum ...")
         result["bspec"] = [bench_bspec(int(n), args.bspec_sims, plain=not args.no_plain)
                            for n in args.bspec_nside.split(",")]
+        if args.bspec_mixed:
+            log("binned bispectrum, mixed resolutions in one launch ...")
+            spec = [tuple(int(v) for v in x.split(":")) for x in args.bspec_mixed.split(",")]
+            result["bspec_mixed"] = bench_bspec_mixed(spec)
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")

@@ -239,6 +239,43 @@ int ps_bspec_linear_sums(int precision, const void *P, const void *Q, const void
                          long long npix, const int *triplets, int ntrip, double scale, void *scratch,
                          long long scratch_bytes, void *out, int accumulate, void *stream);
 
+/* Every pixel reduction of one or several BSpecBin numerators in one launch: the cubic term
+ * of each data set and the linear term of every simulation (bspec_bin.py:339-403, whose
+ * mp.Pool map products and per-simulation loop this replaces), at any mix of resolutions.
+ * A segment is one set of device stacks (nmap x npix complex, FP32 or FP64 as precision)
+ * with its device triplets (ntrip x 3 int32) and mode:
+ *   0 = ps_bspec_triple_sums on (X[0], X[1], X[2]); 1 = the same with A = B = X[0], C = X[1];
+ *   2 = ps_bspec_linear_sums on (X[0], X[1], X[2], X[3]) = (P, Q, M, N).
+ * Segments of one group (same ntrip) add into the group's out (ntrip complex FP64):
+ *   out (+)= scale * sum over its segments, summed in segment order, bit-reproducible.
+ * shape: launch-shape id in [0, ps_bspec_multi_shape_count(precision)) (pixel tile, triplets
+ * per thread, pixel ranges per segment), -1 = built-in; ps_bspec_multi_autotune times every
+ * candidate on the given segments (partials only; no group output is written) and returns the
+ * fastest in *best (the built-in one unless another is 3% faster).  scratch:
+ * ps_bspec_multi_scratch_bytes(...) bytes, enough for every shape.  The segment table is copied
+ * from host memory on the stream (not for graph capture). */
+typedef struct ps_bspec_segment {
+    int mode;
+    int nmap;
+    long long npix;
+    const void *X[4];
+    const int *triplets;
+    int ntrip;
+    int group;
+} ps_bspec_segment;
+typedef struct ps_bspec_group {
+    void *out;
+    double scale;
+    int accumulate;
+} ps_bspec_group;
+int ps_bspec_multi_shape_count(int precision);
+long long ps_bspec_multi_scratch_bytes(int precision, int nseg, const ps_bspec_segment *segs, int ngroup);
+int ps_bspec_sums_multi(int precision, int nseg, const ps_bspec_segment *segs, int ngroup,
+                        const ps_bspec_group *groups, int shape, void *scratch, long long scratch_bytes,
+                        void *stream);
+int ps_bspec_multi_autotune(int precision, int nseg, const ps_bspec_segment *segs, int ngroup, void *scratch,
+                            long long scratch_bytes, int reps, double *ms, int *best, void *stream);
+
 /* -------------------------------------------- radial-node optimizer
  * Exact gNL-loc Fisher-matrix derivative (fisher_deriv_gNL_loc, ideal_fisher.pyx:750-787,
  * called from _compute_fisher_derivatives, tspec_template.py:3858-3870), all FP64:

@@ -68,9 +68,26 @@ SIGNATURES = {
     "ps_bspec_scratch_bytes": (_LL, [_I]),
     "ps_bspec_triple_sums": (_I, [_I, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),
     "ps_bspec_linear_sums": (_I, [_I, _P, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),
+    "ps_bspec_multi_shape_count": (_I, [_I]),
+    "ps_bspec_multi_scratch_bytes": (_LL, [_I, _I, _P, _I]),
+    "ps_bspec_sums_multi": (_I, [_I, _I, _P, _I, _P, _I, _P, _LL, _P]),
+    "ps_bspec_multi_autotune": (_I, [_I, _I, _P, _I, _P, _LL, _I, ctypes.POINTER(_D), ctypes.POINTER(_I), _P]),
     "ps_fisher_deriv_work_bytes": (_LL, [_I, _I, _I, _I]),
     "ps_fisher_deriv_gnl_loc": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _I, _P, _I, _P, _LL, _P, _P]),
 }
+
+
+
+class BspecSegment(ctypes.Structure):
+    """ps_bspec_segment: one set of stacks of the multi-segment binned-bispectrum launch."""
+    _fields_ = [("mode", _I), ("nmap", _I), ("npix", _LL), ("X", _P * 4), ("triplets", _P), ("ntrip", _I),
+                ("group", _I)]
+
+
+class BspecGroup(ctypes.Structure):
+    """ps_bspec_group: the output of the segments of one group."""
+    _fields_ = [("out", _P), ("scale", _D), ("accumulate", _I)]
+
 
 _lib = None
 

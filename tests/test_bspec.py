@@ -154,3 +154,94 @@ def test_gpu_triple_and_linear_sums_match_numpy(gpu, prec, nmap, npix, ntrip):
     wantl = cyc(P, Q, N) + cyc(Q, P, N) + cyc(Q, Q, M)
     gotl = run(lib.ps_bspec_linear_sums, tuple(Xd), 0.5) / 0.5
     assert np.abs(gotl - wantl).max() < tol * np.abs(wantl).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [32, 64])
+def test_gpu_multi_segment_sums_match_numpy(gpu, prec):
+    """ps_bspec_sums_multi: segments of every mode, different stack sizes, pixel counts and
+    triplet counts, two segments adding into one group, every launch shape, against direct FP64
+    sums; the autotuner returns a valid shape."""
+    import torch
+    from polyspec_amd import bspec_bin as B
+    from polyspec_amd._abi import load
+    print(f"bspec multi prec={prec}", flush=True)
+    rng = np.random.default_rng(23)
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+
+    def stack(nmap, npix):
+        x = rng.standard_normal((nmap, npix)) + 1j * rng.standard_normal((nmap, npix))
+        xd = torch.from_numpy(x).to(gpu, cdt).contiguous()
+        return xd, xd.cpu().to(torch.complex128).numpy()
+
+    def cyc(A, Bm, C, trip):
+        a, b, c = trip[:, 0], trip[:, 1], trip[:, 2]
+        return (A[a] * Bm[b] * C[c] + A[b] * Bm[c] * C[a] + A[c] * Bm[a] * C[b]).sum(1)
+
+    # group 0: cubic (7 maps, 5001 px) + triple; group 1: two linear segments (60 maps, 3000 px);
+    # group 2: one linear segment of another size (12 maps, 777 px, 2100 triplets)
+    specs = [(0, 1, 7, 5001, 300), (0, 0, 7, 5001, 300), (1, 2, 60, 3000, 4500), (1, 2, 60, 3000, 4500),
+             (2, 2, 12, 777, 2100)]
+    segs, want = [], {}
+    trips = {}
+    for g, mode, nmap, npix, ntrip in specs:
+        if g not in trips:
+            trips[g] = rng.integers(0, nmap, (ntrip, 3)).astype(np.int32)
+        t = trips[g]
+        td = torch.from_numpy(t).to(gpu)
+        ns = {0: 3, 1: 2, 2: 4}[mode]
+        st = [stack(nmap, npix) for _ in range(ns)]
+        X = [x[1] for x in st]
+        if mode == 0:
+            w = cyc(X[0], X[1], X[2], t)
+        elif mode == 1:
+            w = cyc(X[0], X[0], X[1], t)
+        else:
+            P, Q, M, N = X
+            w = cyc(P, Q, N, t) + cyc(Q, P, N, t) + cyc(Q, Q, M, t)
+        want[g] = want.get(g, 0) + w
+        segs.append((tuple(x[0] for x in st), td, g))
+    tol = 1e-11 if prec == 64 else 2e-5
+    nshape = load().ps_bspec_multi_shape_count(prec)
+    assert nshape >= 4
+    for shape in list(range(nshape)) + [-1]:
+        outs = [torch.full((len(trips[g]), 2), np.nan, dtype=torch.float64, device=gpu) for g in range(3)]
+        B.multi_sums(segs, outs, shape)
+        for g in range(3):
+            o = outs[g].cpu().numpy()
+            got = o[:, 0] + 1j * o[:, 1]
+            err = np.abs(got - want[g]).max() / np.abs(want[g]).max()
+            assert err < tol, (shape, g, err)
+    best, ms = B.multi_autotune(segs, 3)
+    assert 0 <= best < nshape and ms[0] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [32, 64])
+def test_gpu_bspec_fused_and_multi_resolution(gpu, prec):
+    """Bl_numerator's fused launch equals the per-simulation launches, and Bl_numerator_multi
+    over two resolutions equals each analysis's own numerator."""
+    import torch
+    from polyspec_amd.bspec_bin import Bl_numerator_multi
+    print(f"bspec fused/multi prec={prec}", flush=True)
+    runs = []
+    for nside in (16, 32):
+        base, W, bs, bins, fields, datas = _setup(nside, True, prec, 3)
+        bs.load_sims(lambda i: datas[1 + i], 3, preload=True)
+        bs.fused_sums = False
+        sep = bs.Bl_numerator(datas[0])
+        bs.fused_sums = True
+        fused = bs.Bl_numerator(datas[0])
+        tol = 1e-12 if prec == 64 else 1e-6
+        assert np.abs(fused - sep).max() < tol * np.abs(sep).max()
+        runs.append((bs, datas[0], fused))
+    # one launch over both resolutions: other pixel ranges per segment, so FP32 tile sums round
+    # differently
+    tol = 1e-12 if prec == 64 else 1e-6
+    multi = Bl_numerator_multi([r[0] for r in runs], [r[1] for r in runs])
+    for (bs, d, fused), m in zip(runs, multi):
+        assert np.abs(m - fused).max() < tol * np.abs(fused).max()
+    cub = Bl_numerator_multi([r[0] for r in runs], [r[1] for r in runs], include_linear_term=False)
+    for (bs, d, fused), c in zip(runs, cub):
+        ref = bs.Bl_numerator(d, include_linear_term=False)
+        assert np.abs(c - ref).max() < tol * np.abs(ref).max()
