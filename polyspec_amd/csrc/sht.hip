@@ -1163,42 +1163,6 @@ __device__ __forceinline__ int bitrev(int k, int logM) {
     return logM ? (int)(__brev((unsigned)k) >> (32 - logM)) : 0;
 }
 
-// Decimation in frequency: natural order in, bit-reversed out.
-template <typename T>
-__device__ void fft_dif(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
-    for (int h = M >> 1; h >= 1; h >>= 1) {
-        const long ts = maxM / (2L * h);
-        for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
-            const int pos = t & (h - 1);
-            const int i = ((t - pos) << 1) + pos, j = i + h;
-            cplx<T> w = tw[pos * ts];
-            if (inv) w.im = -w.im;
-            const cplx<T> u = a[i], v = a[j];
-            a[i] = cadd(u, v);
-            a[j] = cmul(csub(u, v), w);
-        }
-        __syncthreads();
-    }
-}
-
-// Decimation in time: bit-reversed order in, natural out.
-template <typename T>
-__device__ void fft_dit(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
-    for (int h = 1; h < M; h <<= 1) {
-        const long ts = maxM / (2L * h);
-        for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
-            const int pos = t & (h - 1);
-            const int i = ((t - pos) << 1) + pos, j = i + h;
-            cplx<T> w = tw[pos * ts];
-            if (inv) w.im = -w.im;
-            const cplx<T> u = a[i], v = cmul(a[j], w);
-            a[i] = cadd(u, v);
-            a[j] = csub(u, v);
-        }
-        __syncthreads();
-    }
-}
-
 // LDS element index of entry i of a transform buffer: bits 2..5 of i are XORed into bits
 // 0..3 (a permutation inside each aligned group of 16 entries, so no padding).  With
 // radix-4 butterflies a quarter-wave of 16 lanes then touches 32 distinct 4-byte banks in
@@ -1207,7 +1171,9 @@ __device__ void fft_dit(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long 
 __device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 2) & 15); }
 __host__ __device__ __forceinline__ int fft_pitch(int M) { return M + 1; }
 
-// Batched transforms over NB padded buffers of length M <= 256 * 4 * FFT_MAXQ.  Two
+// Batched transforms over NB padded buffers of length M <= 256 * 4 * Q (Q <= FFT_MAXQ: the
+// butterfly-group slots each thread holds, a template parameter so that the twiddle registers
+// of a launch are sized for its longest transform).  Two
 // radix-2 stages are fused into one radix-2^2 pass in registers (half the LDS traffic,
 // same bit-reversed ordering); with log2 M odd one plain radix-2 stage remains.  The
 // twiddles of a pass are loaded for all of a thread's butterflies before any of them
@@ -1222,14 +1188,14 @@ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a, bool inv) {  // a * (-i), o
 }
 
 // one radix-2 stage of half-size h over all buffers (DIF when !dit)
-template <typename T, int NB>
+template <typename T, int NB, int Q>
 __device__ __forceinline__ void fft_r2_stage(cplx<T>* a, int M, int h, const cplx<T>* __restrict__ tw, long maxM,
                                              bool inv, bool dit) {
     const int half = M >> 1, pitch = fft_pitch(M);
     const long ts = maxM / (2L * h);
-    cplx<T> w[2 * FFT_MAXQ];
+    cplx<T> w[2 * Q];
 #pragma unroll
-    for (int k = 0; k < 2 * FFT_MAXQ; k++) {
+    for (int k = 0; k < 2 * Q; k++) {
         const int u = threadIdx.x + 256 * k;
         if (u < half) {
             w[k] = tw[(u & (h - 1)) * ts];
@@ -1237,7 +1203,7 @@ __device__ __forceinline__ void fft_r2_stage(cplx<T>* a, int M, int h, const cpl
         }
     }
 #pragma unroll
-    for (int k = 0; k < 2 * FFT_MAXQ; k++) {
+    for (int k = 0; k < 2 * Q; k++) {
         const int u = threadIdx.x + 256 * k;
         if (u < half) {
             const int pos = u & (h - 1);
@@ -1260,25 +1226,38 @@ __device__ __forceinline__ void fft_r2_stage(cplx<T>* a, int M, int h, const cpl
     __syncthreads();
 }
 
-// Decimation in frequency: natural order in, bit-reversed out.  Fused stages (h, h/2).
-template <typename T, int NB>
+// Twiddles of the fused DIF pass with stage half-size h: wA[k] for butterfly group
+// v = threadIdx.x + 256 k (stage h; stage h/2 derives its own from them).
+template <typename T, int Q>
+__device__ __forceinline__ void dif_twiddles(cplx<T> (&wA)[Q], int h, int quarter,
+                                             const cplx<T>* __restrict__ tw, long maxM, bool inv) {
+    const int hh = h >> 1;
+    const long ts = maxM / (2L * h);
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int v = threadIdx.x + 256 * k;
+        if (v < quarter) {
+            wA[k] = tw[(v & (hh - 1)) * ts];
+            if (inv) wA[k].im = -wA[k].im;
+        }
+    }
+}
+
+// Decimation in frequency: natural order in, bit-reversed out.  Fused stages (h, h/2).  The
+// next pass's twiddles are loaded before the current pass's butterflies, so their global-memory
+// latency overlaps the LDS work and the barrier.
+template <typename T, int NB, int Q>
 __device__ void fft_dif_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
     const int quarter = M >> 2, pitch = fft_pitch(M);
     int h = M >> 1;
+    cplx<T> wA[Q];
+    if (h >= 2) dif_twiddles<T, Q>(wA, h, quarter, tw, maxM, inv);
     for (; h >= 2; h >>= 2) {
         const int hh = h >> 1;
-        const long ts = maxM / (2L * h);  // stage h; stage h/2 uses 2 ts
-        cplx<T> wA[FFT_MAXQ];
+        cplx<T> wN[Q];
+        if ((h >> 2) >= 2) dif_twiddles<T, Q>(wN, h >> 2, quarter, tw, maxM, inv);
 #pragma unroll
-        for (int k = 0; k < FFT_MAXQ; k++) {
-            const int v = threadIdx.x + 256 * k;
-            if (v < quarter) {
-                wA[k] = tw[(v & (hh - 1)) * ts];
-                if (inv) wA[k].im = -wA[k].im;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < FFT_MAXQ; k++) {
+        for (int k = 0; k < Q; k++) {
             const int v = threadIdx.x + 256 * k;
             if (v < quarter) {
                 const int pos = v & (hh - 1);
@@ -1299,37 +1278,50 @@ __device__ void fft_dif_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, lo
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < Q; k++) wA[k] = wN[k];
     }
-    if (h == 1) fft_r2_stage<T, NB>(a, M, 1, tw, maxM, inv, false);
+    if (h == 1) fft_r2_stage<T, NB, Q>(a, M, 1, tw, maxM, inv, false);
 }
 
-// Decimation in time: bit-reversed order in, natural out.  Fused stages (h, 2h).
-template <typename T, int NB>
+// Twiddles of the fused DIT pass with stage half-size h: wA (stage h) and wB (stage 2h) for
+// butterfly group v = threadIdx.x + 256 k.
+template <typename T, int Q>
+__device__ __forceinline__ void dit_twiddles(cplx<T> (&wA)[Q], cplx<T> (&wB)[Q], int h, int quarter,
+                                             const cplx<T>* __restrict__ tw, long maxM, bool inv) {
+    const long ts = maxM / (2L * h);  // stage h; stage 2h uses ts / 2
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int v = threadIdx.x + 256 * k;
+        if (v < quarter) {
+            const int pos = v & (h - 1);
+            wA[k] = tw[pos * ts];
+            wB[k] = tw[pos * (ts >> 1)];
+            if (inv) {
+                wA[k].im = -wA[k].im;
+                wB[k].im = -wB[k].im;
+            }
+        }
+    }
+}
+
+// Decimation in time: bit-reversed order in, natural out.  Fused stages (h, 2h); the next
+// pass's twiddles are loaded before the current pass's butterflies.
+template <typename T, int NB, int Q>
 __device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, long maxM, bool inv) {
     const int quarter = M >> 2, pitch = fft_pitch(M);
     int h = 1;
     if (ilog2_dev(M) & 1) {
-        fft_r2_stage<T, NB>(a, M, 1, tw, maxM, inv, true);
+        fft_r2_stage<T, NB, Q>(a, M, 1, tw, maxM, inv, true);
         h = 2;
     }
+    cplx<T> wA[Q], wB[Q];
+    if (2 * h < M) dit_twiddles<T, Q>(wA, wB, h, quarter, tw, maxM, inv);
     for (; 2 * h < M; h <<= 2) {
-        const long ts = maxM / (2L * h);  // stage h; stage 2h uses ts / 2
-        cplx<T> wA[FFT_MAXQ], wB[FFT_MAXQ];
+        cplx<T> nA[Q], nB[Q];
+        if (8 * h < M) dit_twiddles<T, Q>(nA, nB, 4 * h, quarter, tw, maxM, inv);
 #pragma unroll
-        for (int k = 0; k < FFT_MAXQ; k++) {
-            const int v = threadIdx.x + 256 * k;
-            if (v < quarter) {
-                const int pos = v & (h - 1);
-                wA[k] = tw[pos * ts];
-                wB[k] = tw[pos * (ts >> 1)];
-                if (inv) {
-                    wA[k].im = -wA[k].im;
-                    wB[k].im = -wB[k].im;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < FFT_MAXQ; k++) {
+        for (int k = 0; k < Q; k++) {
             const int v = threadIdx.x + 256 * k;
             if (v < quarter) {
                 const int pos = v & (h - 1);
@@ -1350,6 +1342,11 @@ __device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, lo
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            wA[k] = nA[k];
+            wB[k] = nB[k];
+        }
     }
 }
 
@@ -1361,7 +1358,7 @@ __device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, lo
 // rounds of nphi consecutive values (each round hits every j once, so no two threads
 // add to one element), direct and mirrored terms in separate passes.  Then an LDS
 // radix-2 FFT (nphi a power of two) or a Bluestein chirp-z transform.
-template <typename T, int NB>
+template <typename T, int NB, int Q>
 __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __restrict__ ring_list, int ncol,
                                                     const cplx<T>* __restrict__ phase, T* __restrict__ map,
                                                     long map_stride) {
@@ -1397,9 +1394,9 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
         // no aliasing (n > mmax): item (j, q) takes m = j and the mirror m = n - j at most.
         // Batches of FB items issue all their loads before any use, so the global-memory
         // latency is paid once per batch rather than once per item.
-        constexpr int FB = 8;
+        constexpr int FB = 4;
         for (int i0 = tid; i0 < nitem; i0 += 256 * FB) {
-            cplx<T> A[FB], B[FB], Am[FB], Bm[FB];
+            cplx<T> A[FB], B[FB], Am[FB], Bm[FB], ch[FB];
 #pragma unroll
             for (int u = 0; u < FB; u++) {
                 const int i = i0 + 256 * u;
@@ -1413,18 +1410,19 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
                 B[u] = (hasB && dir) ? src[(size_t)j * mstride + 1] : z;
                 Am[u] = (hasA && mir) ? src[(size_t)(n - j) * mstride] : z;
                 Bm[u] = (hasB && mir) ? src[(size_t)(n - j) * mstride + 1] : z;
+                if (!pow2 && in) ch[u] = d.chirp[ri.coff + j];
             }
 #pragma unroll
             for (int u = 0; u < FB; u++) {
                 const int i = i0 + 256 * u;
-                if (i >= nitem) break;
+                if (i >= nitem) continue;
                 const int j = i / NB, q = i - (i / NB) * NB;
                 cplx<T> a = A[u], b = B[u];
                 if (j == 0) a.im = b.im = T(0);
                 cplx<T> acc{a.re - b.im, a.im + b.re};  // A + i B
                 acc.re += Am[u].re + Bm[u].im;          // conj(A) + i conj(B) of the mirror
                 acc.im += Bm[u].re - Am[u].im;
-                if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);  // Bluestein pre-chirp
+                if (!pow2) acc = cmul(acc, ch[u]);  // Bluestein pre-chirp
                 buf[q * pitch + pidx(j)] = acc;
             }
         }
@@ -1470,7 +1468,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
     __syncthreads();
     const int nc = min(G, ncol - c0);
     if (pow2) {
-        fft_dif_nb<T, NB>(buf, M, d.tw, d.maxM, true);
+        fft_dif_nb<T, NB, Q>(buf, M, d.tw, d.maxM, true);
         for (int i = tid; i < NB * n; i += 256) {
             const int q = i / n, k = i - q * n;
             const cplx<T> z = buf[q * pitch + pidx(bitrev(k, logM))];
@@ -1478,27 +1476,55 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
             if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im;
         }
     } else {
-        fft_dif_nb<T, NB>(buf, M, d.tw, d.maxM, false);
+        fft_dif_nb<T, NB, Q>(buf, M, d.tw, d.maxM, false);
+        // pointwise product with the chirp kernel's transform; loads issued in batches of 8
         const cplx<T>* V = d.blue + ri.voff;
-        for (int i = tid; i < NB * M; i += 256) {
-            const int q = i >> logM, k = i & (M - 1);
-            cplx<T>& e = buf[q * pitch + pidx(k)];
-            e = cmul(e, V[k]);
+        for (int i0 = tid; i0 < NB * M; i0 += 256 * 8) {
+            cplx<T> v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + 256 * u;
+                if (i < NB * M) v[u] = V[i & (M - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + 256 * u;
+                if (i < NB * M) {
+                    cplx<T>& e = buf[(i >> logM) * pitch + pidx(i & (M - 1))];
+                    e = cmul(e, v[u]);
+                }
+            }
         }
         __syncthreads();
-        fft_dit_nb<T, NB>(buf, M, d.tw, d.maxM, true);
+        fft_dit_nb<T, NB, Q>(buf, M, d.tw, d.maxM, true);
         const T invM = T(1) / (T)M;
-        for (int i = tid; i < NB * n; i += 256) {
-            const int q = i / n, k = i - q * n;
-            const cplx<T> z = cmul(buf[q * pitch + pidx(k)], d.chirp[ri.coff + k]);
-            if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
-            if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
+        for (int i0 = tid; i0 < NB * n; i0 += 256 * 8) {
+            cplx<T> ch[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + 256 * u;
+                if (i < NB * n) ch[u] = d.chirp[ri.coff + (i - (i / n) * n)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + 256 * u;
+                if (i < NB * n) {
+                    const int q = i / n, k = i - q * n;
+                    const cplx<T> z = cmul(buf[q * pitch + pidx(k)], ch[u]);
+                    if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
+                    if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
+                }
+            }
         }
     }
 }
 
-// maps [col][npix] -> phases [col][mmax+1][nrings]: sum_k f_k exp(-i m phi_k)
-template <typename T>
+// maps [col][npix] -> phases [col][mmax+1][nrings]: sum_k f_k exp(-i m phi_k).  One block per
+// (ring, column): the ring's pixels into one LDS buffer (loads issued in batches of 8), the
+// radix-2^2 LDS FFT of the synthesis kernels (power-of-two rings) or a Bluestein chirp-z
+// transform, then the m rows with the phi0 rotation.  Q: butterfly slots per thread for the
+// plan's longest transform (M <= 1024 Q).
+template <typename T, int Q>
 __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __restrict__ map,
                                                       long map_stride, cplx<T>* __restrict__ phase) {
     extern __shared__ unsigned char smem_raw[];
@@ -1509,40 +1535,74 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
     const RingInfo ri = d.rings[ring];
     const int n = ri.nphi, M = ri.M, logM = ri.logM;
     const bool pow2 = ri.voff < 0;
+    const int tid = threadIdx.x;
     const T* in = map + (size_t)col * map_stride + ri.start;
     const long phq = ri.phi_q, php = ri.phi_p;
 
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
-        cplx<T> v{T(0), T(0)};
-        if (k < n) {
-            v.re = in[k];
-            if (!pow2) v = cmul(v, d.chirp[ri.coff + k]);
+    for (int k0 = tid; k0 < M; k0 += 256 * 8) {
+        T x[8];
+        cplx<T> ch[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + 256 * u;
+            x[u] = k < n ? in[k] : T(0);
+            if (!pow2 && k < n) ch[u] = d.chirp[ri.coff + k];
         }
-        buf[k] = v;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + 256 * u;
+            if (k < M) {
+                cplx<T> v{x[u], T(0)};
+                if (!pow2 && k < n) v = cmul(v, ch[u]);
+                buf[pidx(k)] = v;
+            }
+        }
     }
     __syncthreads();
-    if (pow2) {
-        fft_dif(buf, M, d.tw, d.maxM, false);
-    } else {
+    fft_dif_nb<T, 1, Q>(buf, M, d.tw, d.maxM, false);
+    if (!pow2) {
         // g_j = sum_k f_k exp(+2 pi i jk/n) by Bluestein; C_j = conj(g_j)
-        fft_dif(buf, M, d.tw, d.maxM, false);
         const cplx<T>* V = d.blue + ri.voff;
-        for (int k = threadIdx.x; k < M; k += blockDim.x) buf[k] = cmul(buf[k], V[k]);
+        for (int k0 = tid; k0 < M; k0 += 256 * 8) {
+            cplx<T> v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = k0 + 256 * u;
+                if (k < M) v[u] = V[k];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = k0 + 256 * u;
+                if (k < M) buf[pidx(k)] = cmul(buf[pidx(k)], v[u]);
+            }
+        }
         __syncthreads();
-        fft_dit(buf, M, d.tw, d.maxM, true);
+        fft_dit_nb<T, 1, Q>(buf, M, d.tw, d.maxM, true);
     }
     cplx<T>* P = phase + ((size_t)col * (d.mmax + 1)) * d.nrings + ring;
     const T invM = T(1) / (T)M;
-    for (int m = threadIdx.x; m <= d.mmax; m += blockDim.x) {
-        const int j = m % n;
-        cplx<T> C;
-        if (pow2) {
-            C = buf[bitrev(j, logM)];
-        } else {
-            cplx<T> g = cmul(buf[j], d.chirp[ri.coff + j]);
-            C = cplx<T>{g.re * invM, -g.im * invM};
+    for (int m0 = tid; m0 <= d.mmax; m0 += 256 * 8) {
+        cplx<T> ch[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int m = m0 + 256 * u;
+            if (!pow2 && m <= d.mmax) ch[u] = d.chirp[ri.coff + m % n];
         }
-        P[(size_t)m * d.nrings] = cmul(C, expi_pi<T>(-(long)m * php, phq));
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int m = m0 + 256 * u;
+            if (m <= d.mmax) {
+                const int j = m % n;
+                cplx<T> C;
+                if (pow2) {
+                    C = buf[pidx(bitrev(j, logM))];
+                } else {
+                    const cplx<T> g = cmul(buf[pidx(j)], ch[u]);
+                    C = cplx<T>{g.re * invM, -g.im * invM};
+                }
+                P[(size_t)m * d.nrings] = cmul(C, expi_pi<T>(-(long)m * php, phq));
+            }
+        }
     }
 }
 
@@ -2221,20 +2281,38 @@ static int launch_synth_sp(const ShtPlan& p, int nmap, int ld, const void* alm_t
     return 0;
 }
 
-// Column pairs per block: as many FFT buffers as fit 64 KB of LDS for the ring class.
-template <typename T, int NB>
-static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
-                                const void* phase, void* map, long map_stride, hipStream_t st) {
+// Column pairs per block: as many FFT buffers as fit 64 KB of LDS for the ring class.  Q: the
+// butterfly slots per thread for the class's longest transform (maxM <= 1024 Q).
+template <typename T, int NB, int Q>
+static int launch_ring_synth_nbq(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
+                                 const void* phase, void* map, long map_stride, hipStream_t st) {
     ShtDev<T> d = dev_view<T>(p);
     const size_t sh = (size_t)NB * fft_pitch((int)maxM) * sizeof(cplx<T>);
     if (sh + 256 * sizeof(cplx<T>) > 65536)  // dynamic buffers plus the static slice partials
-        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T, NB>,
+        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T, NB, Q>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     const int ngrp = (ncol + 2 * NB - 1) / (2 * NB);
-    hipLaunchKernelGGL((k_ring_synth<T, NB>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d, list, ncol,
+    hipLaunchKernelGGL((k_ring_synth<T, NB, Q>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d, list, ncol,
                        (const cplx<T>*)phase, (T*)map, map_stride);
     PS_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename T, int NB>
+static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
+                                const void* phase, void* map, long map_stride, hipStream_t st) {
+    if (maxM <= 1024)
+        return launch_ring_synth_nbq<T, NB, 1>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    if (maxM <= 2048)
+        return launch_ring_synth_nbq<T, NB, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    if constexpr (NB <= 2) {
+        if (maxM <= 4096)
+            return launch_ring_synth_nbq<T, NB, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    }
+    if constexpr (NB == 1)
+        return launch_ring_synth_nbq<T, NB, FFT_MAXQ>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    set_error("ring synthesis: transform too long for this many buffers per block");
+    return -1;
 }
 
 // Ring-synthesis shape: rings per block (NB FFT buffers of two real columns each).  The
@@ -2280,18 +2358,31 @@ static int launch_ring_synth(const ShtPlan& p, int ncol, const void* phase, void
                                      map_stride, st, nb_blue);
 }
 
-template <typename T>
-static int launch_ring_adjoint(const ShtPlan& p, int ncol, const void* map, long map_stride, void* phase,
-                               hipStream_t st) {
+template <typename T, int Q>
+static int launch_ring_adjoint_q(const ShtPlan& p, int ncol, const void* map, long map_stride, void* phase,
+                                 hipStream_t st) {
     ShtDev<T> d = dev_view<T>(p);
     size_t sh = lds_bytes<T>(p);
     if (sh > 65536)
-        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_adjoint<T>,
+        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_adjoint<T, Q>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(k_ring_adjoint<T>, dim3((unsigned)(p.nrings * ncol)), dim3(256), sh, st, d,
+    hipLaunchKernelGGL((k_ring_adjoint<T, Q>), dim3((unsigned)(p.nrings * ncol)), dim3(256), sh, st, d,
                        (const T*)map, map_stride, (cplx<T>*)phase);
     PS_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename T>
+static int launch_ring_adjoint(const ShtPlan& p, int ncol, const void* map, long map_stride, void* phase,
+                               hipStream_t st) {
+    if (p.maxM > 4L * 256 * FFT_MAXQ) {
+        set_error("ring adjoint: FFT length above 8192 is not supported");
+        return -1;
+    }
+    if (p.maxM <= 1024) return launch_ring_adjoint_q<T, 1>(p, ncol, map, map_stride, phase, st);
+    if (p.maxM <= 2048) return launch_ring_adjoint_q<T, 2>(p, ncol, map, map_stride, phase, st);
+    if (p.maxM <= 4096) return launch_ring_adjoint_q<T, 4>(p, ncol, map, map_stride, phase, st);
+    return launch_ring_adjoint_q<T, FFT_MAXQ>(p, ncol, map, map_stride, phase, st);
 }
 
 // Ring slices of the adjoint: with few batch columns the blocks of small m (the longest
