@@ -1470,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
     if (pow2) {
         fft_dif_nb<T, NB, Q>(buf, M, d.tw, d.maxM, true);
         for (int i = tid; i < NB * n; i += 256) {
-            const int q = i / n, k = i - q * n;
+            const int q = i >> logM, k = i & (M - 1);  // n = M for power-of-two rings
             const cplx<T> z = buf[q * pitch + pidx(bitrev(k, logM))];
             if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re;
             if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im;
@@ -1498,21 +1498,26 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
         __syncthreads();
         fft_dit_nb<T, NB, Q>(buf, M, d.tw, d.maxM, true);
         const T invM = T(1) / (T)M;
-        for (int i0 = tid; i0 < NB * n; i0 += 256 * 8) {
+        // post-chirp and output: each chirp value is loaded once and serves the NB buffers
+        for (int k0 = tid; k0 < n; k0 += 256 * 8) {
             cplx<T> ch[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const int i = i0 + 256 * u;
-                if (i < NB * n) ch[u] = d.chirp[ri.coff + (i - (i / n) * n)];
+                const int k = k0 + 256 * u;
+                if (k < n) ch[u] = d.chirp[ri.coff + k];
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int i = i0 + 256 * u;
-                if (i < NB * n) {
-                    const int q = i / n, k = i - q * n;
-                    const cplx<T> z = cmul(buf[q * pitch + pidx(k)], ch[u]);
-                    if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
-                    if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
+            for (int q = 0; q < NB; q++) {
+                T* outA = map + (size_t)(c0 + 2 * q) * map_stride + ri.start;
+                T* outB = outA + map_stride;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int k = k0 + 256 * u;
+                    if (k < n) {
+                        const cplx<T> z = cmul(buf[q * pitch + pidx(k)], ch[u]);
+                        if (2 * q < nc) outA[k] = z.re * invM;
+                        if (2 * q + 1 < nc) outB[k] = z.im * invM;
+                    }
                 }
             }
         }
