@@ -102,12 +102,20 @@ def synthetic_mask(nside, nholes=200, seed=1, cut_deg=20., rhole_deg=0.5, apo_de
     zc = rng.uniform(-1, 1, nholes)
     pc = rng.uniform(0, 2 * np.pi, nholes)
     st = np.sqrt(1 - z * z)
+    rad = np.radians(rhole_deg + apo_deg)
+    negz = -z  # RING order: z decreases with the pixel index
     for zi, pi in zip(zc, pc):
-        cosd = z * zi + st * np.sqrt(1 - zi * zi) * np.cos(ph - pi)
-        near = cosd > np.cos(np.radians(rhole_deg + apo_deg))
+        # only the pixels whose colatitude is within the hole's reach
+        th = np.arccos(zi)
+        lo = np.searchsorted(negz, -np.cos(max(th - rad, 0.)), side='left')
+        hi = np.searchsorted(negz, -np.cos(min(th + rad, np.pi)), side='right')
+        sl = slice(lo, hi)
+        cosd = z[sl] * zi + st[sl] * np.sqrt(1 - zi * zi) * np.cos(ph[sl] - pi)
+        near = cosd > np.cos(rad)
         dd = np.degrees(np.arccos(np.clip(cosd[near], -1, 1))) - rhole_deg
         taper = np.where(dd <= 0, 0., 0.5 * (1 - np.cos(np.pi * np.clip(dd / apo_deg, 0, 1))))
-        m[near] *= taper
+        idx = np.arange(lo, hi)[near]
+        m[idx] *= taper
     return m
 
 
