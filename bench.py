@@ -376,6 +376,9 @@ def bench_sht_roundtrip(nside=1024, lmax=2048, nsets=64, reps=2):
     res["tflops_fp32"] = {k: flops[k] / (t32[k] * 1e-3) / 1e12 for k in flops}
     res["tflops_fp64"] = {k: flops[k] / (t64[k] * 1e-3) / 1e12 for k in flops}
     res["hbm_gbs_fp32_algorithmic"] = {k: bts[k] / (t32[k] * 1e-3) / 1e9 for k in bts}
+    # the transforms are matrix-core / VALU bound: their algorithmic HBM rate is a small fraction of peak
+    res["hbm_frac_of_peak_fp32"] = {k: bts[k] / (t32[k] * 1e-3) / PEAK_HBM for k in bts}
+    res["mfma_frac_of_peak_fp32"] = {k: flops[k] / (t32[k] * 1e-3) / PEAK_FP32 for k in flops}
     tot32, tot64 = sum(t32.values()), sum(t64.values())
     res["round_trip_ms"] = {"fp32": tot32, "fp64": tot64, "fp32_speedup": tot64 / tot32}
     res["maps_per_s_fp32"] = {"synthesis": 3 * nsets / ((t32["synthesis_T"] + t32["synthesis_EB"]) * 1e-3),

@@ -19,7 +19,7 @@ from estimator_setup import grf_alm, radial_nodes, spectra, transfer
 pytestmark = pytest.mark.gpu
 
 
-def build(nside, lmax, pol, nr, templates, precision, batch_sims):
+def build(nside, lmax, pol, nr, templates, precision, batch_sims, nk=60):
     from polyspec_amd.base import PolySpec
     from polyspec_amd.tspec_template import TSpecTemplate
     from polyspec_amd.weighting import Weightings
@@ -27,7 +27,7 @@ def build(nside, lmax, pol, nr, templates, precision, batch_sims):
     base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=precision)
     mask = np.ones(12 * nside ** 2)
     W = Weightings(base, mask, dict(raw))
-    k, Tl = transfer(lmax, pol)
+    k, Tl = transfer(lmax, pol, nk)
     r, w = radial_nodes(nr)
     tm = TSpecTemplate(base, mask, W.applySinv_ideal, templates, k, Tl, 2, lmax, r_values=r,
                        r_weights={t: w for t in templates}, batch_sims=batch_sims)
@@ -65,6 +65,26 @@ def test_numerator_matches_oracle(gpu, nside, lmax, pol, nr, templates, precisio
     # a second numerator reuses the cached t0 (tspec_template.py:1361-1365)
     got2 = tm.Tl_numerator(data)
     assert np.allclose(got2, got, rtol=1e-12 if precision == 64 else 1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision,batch", [(64, False), (32, True)])
+def test_numerator_at_the_reference_cpu_workload_size(gpu, precision, batch):
+    """The reference's CPU-runnable gNL-loc workload at its own size: T only, unit mask,
+    Nside 64, lmin 2, lmax 128, 10 radial nodes, transfer on 200 log-spaced k; the FP64
+    per-simulation loop and the batched FP32 path against the FP64 oracle."""
+    print(f"numerator nside=64 lmax=128 nr=10 prec={precision} batch={batch}", flush=True)
+    nside, lmax, nr, templates = 64, 128, 10, ['gNL-loc']
+    base, tm, orc, tot = build(nside, lmax, False, nr, templates, precision, batch, nk=200)
+    rng = np.random.default_rng(64)
+    L = 3 * nside - 1
+    data = oracle.to_map(grf_alm(rng, tot, L, False), nside, L)
+    sims = [(grf_alm(rng, tot, lmax, False), grf_alm(rng, tot, lmax, False)) for _ in range(2)]
+    tm.load_sims(lambda ii: sims[ii], len(sims), input_type='harmonic')
+    got = tm.Tl_numerator(data)
+    ref = orc.numerator(data, sims, input_type='map', sim_type='harmonic')
+    ref4 = orc.numerator(data, sims, input_type='map', sim_type='harmonic', include_disconnected_term=False)
+    tol = 1e-8 if precision == 64 else 2e-4
+    assert abs(got[0] - ref[0]) <= tol * abs(ref4[0]), (got[0], ref[0], ref4[0])
 
 
 @pytest.mark.parametrize("nside,lmax,pol,nr,templates,precision", [
