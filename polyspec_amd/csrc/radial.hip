@@ -102,6 +102,49 @@ __global__ __launch_bounds__(256) void k_gnl_fused(const T* __restrict__ P, cons
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// The same sum with four consecutive pixels per thread and step (128-bit loads of P and Q,
+// two of U and V) and the radial loop unrolled by two, for rows whose starts are aligned
+// (npix a multiple of 4, 16-byte aligned P and Q, 32-byte aligned U and V).  The pixel loop of
+// every thread visits groups g, g + G, ... (G threads) in order; the last npix % 4 pixels do
+// not occur (npix is a multiple of 4).
+template <typename T> struct vec4_of;
+template <> struct vec4_of<float> { typedef float4 type; };
+template <> struct vec4_of<double> { typedef double4 type; };
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gnl_fused4(const T* __restrict__ P, const T* __restrict__ Q,
+                                                    const double* __restrict__ U, const double* __restrict__ V,
+                                                    const double* __restrict__ w, int nr, long npix, double c4,
+                                                    double c2, double* __restrict__ part) {
+    typedef typename vec4_of<T>::type V4;
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long ng = npix >> 2;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += stride) {
+#pragma unroll 2
+        for (int r = 0; r < nr; r++) {
+            const long i = (long)r * ng + g;
+            const V4 p4 = reinterpret_cast<const V4*>(P)[i], q4 = reinterpret_cast<const V4*>(Q)[i];
+            const double p[4] = {(double)p4.x, (double)p4.y, (double)p4.z, (double)p4.w};
+            const double q[4] = {(double)q4.x, (double)q4.y, (double)q4.z, (double)q4.w};
+            double t = 0.0;
+            if (U) {
+                const double4 u4 = reinterpret_cast<const double4*>(U)[i], v4 = reinterpret_cast<const double4*>(V)[i];
+                const double u[4] = {u4.x, u4.y, u4.z, u4.w}, v[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) t += c4 * p[k] * p[k] * p[k] * q[k] + c2 * (p[k] * p[k] * u[k] + p[k] * q[k] * v[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) t += c4 * p[k] * p[k] * p[k] * q[k];
+            }
+            acc += w[r] * t;
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 // tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285):
 // sum_r w_r sum_p (P1^2 P2 Q2 + P2^2 P1 Q1)
 template <typename T>
@@ -322,8 +365,14 @@ template <typename T>
 static int reduce_launch_fused(const void* P, const void* Q, const void* U, const void* V, const double* w, int nr,
                                long long npix, double c4, double c2, double* part, double* out, int accumulate,
                                hipStream_t st) {
-    hipLaunchKernelGGL(k_gnl_fused<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const T*)P, (const T*)Q,
-                       (const double*)U, (const double*)V, w, nr, (long)npix, c4, c2, part);
+    const bool vec = npix % 4 == 0 && ((uintptr_t)P | (uintptr_t)Q) % (4 * sizeof(T)) == 0 &&
+                     (!U || (((uintptr_t)U | (uintptr_t)V) % 32 == 0));
+    if (vec)
+        hipLaunchKernelGGL(k_gnl_fused4<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const T*)P, (const T*)Q,
+                           (const double*)U, (const double*)V, w, nr, (long)npix, c4, c2, part);
+    else
+        hipLaunchKernelGGL(k_gnl_fused<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, (const T*)P, (const T*)Q,
+                           (const double*)U, (const double*)V, w, nr, (long)npix, c4, c2, part);
     PS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part, RED_BLOCKS, out, accumulate);
     PS_LAUNCH_CHECK();

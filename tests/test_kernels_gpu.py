@@ -119,3 +119,7 @@ def test_pixel_kernels(gpu, prec):
                                                       oracle.gnl_loc_disc_sum(w, a, c, b, a))
     assert abs(f - ref) < 10 * tol * np.sum(w[:, None] * np.abs(a) ** 2 * (np.abs(a * b) + np.abs(U.cpu().numpy())
                                                                            + np.abs(V.cpu().numpy())))
+    # a pixel count that is not a multiple of 4 takes the one-pixel-per-step kernel
+    a3, b3 = a[:, :2999].copy(), b[:, :2999].copy()
+    s = float(K.gnl_loc_sum(w, T(a3, rdt), T(b3, rdt)))
+    assert abs(s - oracle.gnl_loc_sum(w, a3, b3)) < tol * np.sum(w[:, None] * np.abs(a3) ** 3 * np.abs(b3))
