@@ -288,7 +288,8 @@ def bench_sht_roundtrip(nside=1024, lmax=2048, nsets=64, reps=2):
     the FP64 CPU oracle to 1e-11 at Nside <= 256 and satisfy adjointness to 1e-10 at this size).
     Reported: per-field max |a32 - a64| / max |a64| and rms / rms of the round-trip a_lm and of the
     synthesised maps (B, zero on input, relative to E), the milliseconds of every batched launch in both precisions, and the
-    transforms' rates (4 R nlm flops per spin-0 map, 8 R nlm per spin-2 map; bytes 8 nlm + 4 Npix
+    transforms' rates (4 R nlm flops per spin-0 map, 16 R nlm per spin-2 map: G and C each against
+    lambda+ and lambda-, four complex-by-real multiply-adds per (l, m, ring pair); bytes 8 nlm + 4 Npix
     per FP32 map and direction)."""
     import torch
     from polyspec_amd._abi import ShtPlan
@@ -370,7 +371,7 @@ def bench_sht_roundtrip(nside=1024, lmax=2048, nsets=64, reps=2):
     res["ms_per_launch_fp32"] = t32
     res["ms_per_launch_fp64"] = t64
     fl0 = 4. * R * nlm * nsets
-    flops = {"synthesis_T": fl0, "synthesis_EB": 2 * fl0, "adjoint_T": fl0, "adjoint_EB": 2 * fl0}
+    flops = {"synthesis_T": fl0, "synthesis_EB": 4 * fl0, "adjoint_T": fl0, "adjoint_EB": 4 * fl0}
     bts = {"synthesis_T": (8. * nlm + 4. * npix) * nsets, "adjoint_T": (8. * nlm + 4. * npix) * nsets}
     bts["synthesis_EB"] = bts["adjoint_EB"] = 2 * bts["synthesis_T"]
     res["tflops_fp32"] = {k: flops[k] / (t32[k] * 1e-3) / 1e12 for k in flops}
