@@ -24,9 +24,17 @@ FLAGS = [
 ]
 
 
+# Per-file flags.  sht.hip: matrix-core accumulators in VGPRs (gfx950 MFMA instructions take
+# VGPR sources and destinations), so the Legendre kernels do not copy their accumulator tiles
+# between VGPRs and AGPRs around every staged chunk of l.
+FILE_FLAGS = {
+    "sht.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o").replace(".cpp", ".o"))
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
