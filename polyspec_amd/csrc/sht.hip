@@ -760,6 +760,31 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
             }
             continue;
         }
+        if (!fast && n == LCH) {
+            // full chunk while some lane still carries a scale exponent: straight-line groups,
+            // one exponent update per group (the general loop below, without its bounds tests)
+#pragma unroll 2
+            for (int g = 0; g < LCH; g += 4) {
+                double v[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    v[c] = r.f * r.cf;
+                    rec_step(r, sco[2 * (g + c)] * x, sco[2 * (g + c) + 1]);
+                }
+                rec_rescale(r);
+                if (!__any(r.k >= scale_consts<float>::KMIN - 1)) continue;  // all values below FP32 range
+                const float aev = (float)(h ? v[2] : v[0]);
+                const float aod = (float)(h ? v[3] : v[1]);
+#pragma unroll
+                for (int c = 0; c < NT; c++) {
+                    const float bev = sa[(g + 2 * h) * RW + 32 * c + li32];
+                    const float bod = sa[(g + 1 + 2 * h) * RW + 32 * c + li32];
+                    Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
+                    Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
+                }
+            }
+            continue;
+        }
         for (int g = 0; g < n; g += 4) {
             // recursion values at l0+g .. l0+g+3 (zero past the end of the chunk)
             double v[4];
@@ -909,6 +934,34 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int 
                     rec_fast(rp, al * (x + be), ga);
                     rec_fast(rm, al * (x - be), ga);
                 }
+                const float bev = sa[(g + 2 * h) * 2 * BCM + li32];
+                const float bod = sa[(g + 1 + 2 * h) * 2 * BCM + li32];
+                Pev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[2] : vp[0]), bev, Pev, 0, 0, 0);
+                Pod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[3] : vp[1]), bod, Pod, 0, 0, 0);
+                Mev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[2] : vm[0]), bev, Mev, 0, 0, 0);
+                Mod = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vm[3] : vm[1]), bod, Mod, 0, 0, 0);
+            }
+            continue;
+        }
+        if (!fast && n == LCH) {
+            // full chunk while some lane still carries a scale exponent: straight-line groups,
+            // one exponent update per group (the general loop below, without its bounds tests)
+#pragma unroll 2
+            for (int g = 0; g < LCH; g += 4) {
+                double vp[4], vm[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                    vp[c] = 0.5 * (fp + fm);
+                    vm[c] = 0.5 * (fp - fm);
+                    const double al = sco[3 * (g + c)], be = sco[3 * (g + c) + 1], ga = sco[3 * (g + c) + 2];
+                    rec_step(rp, al * (x + be), ga);
+                    rec_step(rm, al * (x - be), ga);
+                }
+                rec_rescale(rp);
+                rec_rescale(rm);
+                if (!__any((rp.k >= scale_consts<float>::KMIN - 1) || (rm.k >= scale_consts<float>::KMIN - 1)))
+                    continue;  // all values below FP32 range
                 const float bev = sa[(g + 2 * h) * 2 * BCM + li32];
                 const float bod = sa[(g + 1 + 2 * h) * 2 * BCM + li32];
                 Pev = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? vp[2] : vp[0]), bev, Pev, 0, 0, 0);
