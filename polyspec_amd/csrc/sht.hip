@@ -1909,6 +1909,16 @@ __global__ __launch_bounds__(256) void k_leg_adj_s0_mfma(ShtDev<float> d, int nc
                     Tw[(c & 1) * 2048 + i * 64 + (lane ^ i)] = (float)r.f;
                     rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
                 }
+            } else if (n == 64) {
+                // full tile while some lane still carries a scale exponent: the loop below
+                // without its bounds tests, one exponent update per 4 values of l
+#pragma unroll 4
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 2048 + i * 64 + (lane ^ i)] = (float)(r.f * r.cf);
+                    rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                    if ((c & 3) == 3) rec_rescale(r);
+                }
             } else {
                 for (int c = 0; c < 64; c++) {
                     const int i = c >> 1;
@@ -2068,6 +2078,16 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
                     const int i = c >> 1;
                     Tw[(c & 1) * 1024 + i * 32 + (li32 ^ i)] = (float)r.f;
                     rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                }
+            } else if (n == 64) {
+                // full tile while some lane still carries a scale exponent: the loop below
+                // without its bounds tests, one exponent update per 4 values of l
+#pragma unroll 4
+                for (int c = 0; c < 64; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 1024 + i * 32 + (li32 ^ i)] = (float)(r.f * r.cf);
+                    rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                    if ((c & 3) == 3) rec_rescale(r);
                 }
             } else {
                 for (int c = 0; c < 64; c++) {
