@@ -393,8 +393,8 @@ __device__ __forceinline__ void rec_rescale(Rec& r) {
     r.f = ldexp(r.f, e);
     r.fm1 = ldexp(r.fm1, e);
     r.k += up ? 1 : 0;
-    const double DN = scale_consts<double>::DN;
-    r.cf = (r.k == 0) ? 1.0 : (r.k == -1) ? DN : (r.k == -2) ? DN * DN : 0.0;
+    // 2^(64 k) for k >= -2 (exact), 0 below: one ldexp and one select, no branches
+    r.cf = r.k >= -2 ? ldexp(1.0, 64 * r.k) : 0.0;
 }
 
 __device__ __forceinline__ int m_of_q(int q, int mmax) {
@@ -722,22 +722,89 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
     };
     // a block whose ring pairs are all dead at this m writes zero phases without the l loop
     const bool skip = __syncthreads_and(d.dead[(size_t)m * d.npairs + jpc]);
-    if (!skip) fetch(m);
-    bool fast = false;
-    for (int l0 = m; l0 <= d.lmax && !skip; l0 += LCH) {
-        const int n = min(LCH, d.lmax + 1 - l0);
+    // stage the chunk at l0s (alm rows and coefficients into LDS) and fetch the next one
+    auto stage = [&](int l0s, int ns) {
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < NT; e++) *reinterpret_cast<float4*>(sa + 4 * (tid + 256 * e)) = pa[e];
-        if (tid < n) {
+        if (tid < ns) {
             sco[2 * tid] = pal;
             sco[2 * tid + 1] = pga;
         }
         __syncthreads();
-        if (l0 + LCH <= d.lmax) fetch(l0 + LCH);
-        if (!fast) fast = __all(r.k == 0);
-        if (fast && n == LCH) {
-            // every scale exponent is zero and the chunk is full: straight-line groups
+        if (l0s + LCH <= d.lmax) fetch(l0s + LCH);
+    };
+    // one group of 4 values of l: two MFMAs per column tile (even and odd l - m)
+    auto mma = [&](int g, float aev, float aod) {
+#pragma unroll
+        for (int c = 0; c < NT; c++) {
+            const float bev = sa[(g + 2 * h) * RW + 32 * c + li32];
+            const float bod = sa[(g + 1 + 2 * h) * RW + 32 * c + li32];
+            Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
+            Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
+        }
+    };
+    // The l loop runs in three phases, so that each inner loop is a loop of its own (the
+    // accumulators then stay in the same registers from chunk to chunk): chunks while some
+    // lane still carries a scale exponent, full chunks once every lane is scaled in, and the
+    // last partial chunk.  Every wave stages every chunk once, so the block's barriers match
+    // even when its waves switch phase at different chunks.
+    int l0 = m;
+    bool staged = false;
+    if (!skip) {
+        fetch(m);
+        for (; l0 <= d.lmax; l0 += LCH) {
+            const int n = min(LCH, d.lmax + 1 - l0);
+            stage(l0, n);
+            if (__all(r.k == 0)) {
+                staged = true;  // this chunk and the rest run scaled in
+                break;
+            }
+            if (n == LCH) {
+                // straight-line groups, one exponent update per group
+#pragma unroll 2
+                for (int g = 0; g < LCH; g += 4) {
+                    double v[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        v[c] = r.f * r.cf;
+                        rec_step(r, sco[2 * (g + c)] * x, sco[2 * (g + c) + 1]);
+                    }
+                    rec_rescale(r);
+                    if (!__any(r.k >= scale_consts<float>::KMIN - 1)) continue;  // all values below FP32 range
+                    mma(g, (float)(h ? v[2] : v[0]), (float)(h ? v[3] : v[1]));
+                }
+            } else {
+                for (int g = 0; g < n; g += 4) {
+                    // recursion values at l0+g .. l0+g+3 (zero past the end of the chunk)
+                    double v[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int li = g + c;
+                        if (li < n) {
+                            v[c] = r.f * r.cf;
+                            rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
+                        } else {
+                            v[c] = 0.0;
+                        }
+                    }
+                    rec_rescale(r);
+                    if (!__any(r.k >= scale_consts<float>::KMIN - 1)) continue;
+                    const int le = g + 2 * h, lo = g + 1 + 2 * h;
+#pragma unroll
+                    for (int c = 0; c < NT; c++) {
+                        const float bev = le < n ? sa[le * RW + 32 * c + li32] : 0.f;
+                        const float bod = lo < n ? sa[lo * RW + 32 * c + li32] : 0.f;
+                        Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? v[2] : v[0]), bev, Dev[c], 0, 0, 0);
+                        Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? v[3] : v[1]), bod, Dod[c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // full chunks, every scale exponent zero: straight-line groups
+        for (; l0 + LCH <= d.lmax + 1; l0 += LCH) {
+            if (!staged) stage(l0, LCH);
+            staged = false;
 #pragma unroll 2
             for (int g = 0; g < LCH; g += 4) {
                 const double v0 = r.f;
@@ -748,70 +815,33 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
                 rec_fast(r, sco[2 * g + 4] * x, sco[2 * g + 5]);
                 const double v3 = r.f;
                 rec_fast(r, sco[2 * g + 6] * x, sco[2 * g + 7]);
-                const float aev = (float)(h ? v2 : v0);
-                const float aod = (float)(h ? v3 : v1);
-#pragma unroll
-                for (int c = 0; c < NT; c++) {
-                    const float bev = sa[(g + 2 * h) * RW + 32 * c + li32];
-                    const float bod = sa[(g + 1 + 2 * h) * RW + 32 * c + li32];
-                    Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
-                    Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
-                }
+                mma(g, (float)(h ? v2 : v0), (float)(h ? v3 : v1));
             }
-            continue;
         }
-        if (!fast && n == LCH) {
-            // full chunk while some lane still carries a scale exponent: straight-line groups,
-            // one exponent update per group (the general loop below, without its bounds tests)
-#pragma unroll 2
-            for (int g = 0; g < LCH; g += 4) {
+        // the last partial chunk, scaled in
+        if (l0 <= d.lmax) {
+            const int n = d.lmax + 1 - l0;
+            if (!staged) stage(l0, n);
+            for (int g = 0; g < n; g += 4) {
                 double v[4];
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    v[c] = r.f * r.cf;
-                    rec_step(r, sco[2 * (g + c)] * x, sco[2 * (g + c) + 1]);
+                    const int li = g + c;
+                    if (li < n) {
+                        v[c] = r.f;
+                        rec_fast(r, sco[2 * li] * x, sco[2 * li + 1]);
+                    } else {
+                        v[c] = 0.0;
+                    }
                 }
-                rec_rescale(r);
-                if (!__any(r.k >= scale_consts<float>::KMIN - 1)) continue;  // all values below FP32 range
-                const float aev = (float)(h ? v[2] : v[0]);
-                const float aod = (float)(h ? v[3] : v[1]);
+                const int le = g + 2 * h, lo = g + 1 + 2 * h;
 #pragma unroll
                 for (int c = 0; c < NT; c++) {
-                    const float bev = sa[(g + 2 * h) * RW + 32 * c + li32];
-                    const float bod = sa[(g + 1 + 2 * h) * RW + 32 * c + li32];
-                    Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
-                    Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
+                    const float bev = le < n ? sa[le * RW + 32 * c + li32] : 0.f;
+                    const float bod = lo < n ? sa[lo * RW + 32 * c + li32] : 0.f;
+                    Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? v[2] : v[0]), bev, Dev[c], 0, 0, 0);
+                    Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32((float)(h ? v[3] : v[1]), bod, Dod[c], 0, 0, 0);
                 }
-            }
-            continue;
-        }
-        for (int g = 0; g < n; g += 4) {
-            // recursion values at l0+g .. l0+g+3 (zero past the end of the chunk)
-            double v[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int li = g + c;
-                if (li < n) {
-                    v[c] = fast ? r.f : r.f * r.cf;
-                    if (fast)
-                        rec_fast(r, sco[2 * li] * x, sco[2 * li + 1]);
-                    else
-                        rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
-                } else {
-                    v[c] = 0.0;
-                }
-            }
-            if (!fast) rec_rescale(r);
-            if (!fast && !__any(r.k >= scale_consts<float>::KMIN - 1)) continue;  // all values below FP32 range
-            const float aev = (float)(h ? v[2] : v[0]);
-            const float aod = (float)(h ? v[3] : v[1]);
-            const int le = g + 2 * h, lo = g + 1 + 2 * h;
-#pragma unroll
-            for (int c = 0; c < NT; c++) {
-                const float bev = le < n ? sa[le * RW + 32 * c + li32] : 0.f;
-                const float bod = lo < n ? sa[lo * RW + 32 * c + li32] : 0.f;
-                Dev[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[c], 0, 0, 0);
-                Dod[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[c], 0, 0, 0);
             }
         }
     }
