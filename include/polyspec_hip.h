@@ -120,6 +120,14 @@ long long ps_reduce_scratch_bytes(void);
  * h: (npol, nlm(lmax)) complex. */
 int ps_weight_alm_lmajor(int precision, const void *f, int nslot, int nr, const void *h, int npol, int lmax,
                          int lmin, double weight, int ld, int col_offset, void *out_t, void *stream);
+/* The two weightings of one filtered field in one launch (TSpecTemplate._compute_weighted_maps,
+ * tspec_template.py:799-855, for the 'p00' and 'q' maps of _apply_all_filters): columns
+ * col_offset + r as ps_weight_alm_lmajor with (fP, wP), columns col_offset + nr + r with
+ * (fQ, wQ); fP and fQ share nslot and nr (rows l <= lmax are read).  Each row of out_t is written by
+ * one launch, not half by each of two. */
+int ps_weight_alm_pq_lmajor(int precision, const void *fP, const void *fQ, int nslot, int nr, const void *h,
+                            int npol, int lmax, int lmin, double wP, double wQ, int ld, int col_offset,
+                            void *out_t, void *stream);
 
 /* c4 * tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270) plus, when U and V are
  * given, c2 * sum_r w_r sum_p (P^2 U + P Q V): with U = sum_s P_s Q_s and

@@ -37,6 +37,7 @@ SIGNATURES = {
     "ps_sht_ring_synthesis": (_I, [_P, _I, _P, _P, _P]),
     "ps_reduce_scratch_bytes": (_LL, []),
     "ps_weight_alm_lmajor": (_I, [_I, _P, _I, _I, _P, _I, _I, _I, _D, _I, _I, _P, _P]),
+    "ps_weight_alm_pq_lmajor": (_I, [_I, _P, _P, _I, _I, _P, _I, _I, _I, _D, _D, _I, _I, _P, _P]),
     "ps_gnl_loc_fused_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _D, _D, _P, _P, _I, _P]),
     "ps_gnl_loc_disc_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _P, _P, _I, _P]),
     "ps_mc_accumulate": (_I, [_I, _P, _P, _I, _LL, _P, _P, _P]),

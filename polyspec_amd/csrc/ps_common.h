@@ -90,9 +90,12 @@ __device__ __forceinline__ T scale_factor(int k) {
 
 // healpy m-major index lm = m(2 lmax + 1 - m)/2 + l  ->  (l, m).  The m block
 // starts at s(m) = m(2 lmax + 3 - m)/2; m is the largest value with s(m) <= lm.
+// The first guess comes from an FP32 square root (within a few units of m at any lmax below
+// 2^14); the two loops then move it to the exact m.
 __device__ __forceinline__ void lm_decode(long lm, int lmax, int* l, int* m) {
-    const double B = 2.0 * lmax + 3.0;
-    int mm = (int)floor((B - sqrt(B * B - 8.0 * (double)lm)) * 0.5);
+    const float B = 2.0f * lmax + 3.0f;
+    int mm = (int)floorf((B - sqrtf(fmaxf(B * B - 8.0f * (float)lm, 0.0f))) * 0.5f);
+    mm = min(max(mm, 0), lmax);
     while (mm > 0 && (long)mm * (2 * lmax + 3 - mm) / 2 > lm) mm--;
     while (mm < lmax && (long)(mm + 1) * (2 * lmax + 2 - mm) / 2 <= lm) mm++;
     *m = mm;

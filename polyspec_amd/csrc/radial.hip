@@ -55,8 +55,16 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nlm * nr) return;
-    const long lm = i / nr;
-    const int c = (int)(i % nr);
+    long lm;
+    int c;
+    if (nlm * nr < (1L << 31)) {  // 32-bit index arithmetic
+        const unsigned iu = (unsigned)i;
+        lm = (long)(iu / (unsigned)nr);
+        c = (int)(iu - (unsigned)lm * (unsigned)nr);
+    } else {
+        lm = i / nr;
+        c = (int)(i % nr);
+    }
     const int l = l_of_lm(lm, lmax);
     cplx<T> v{T(0), T(0)};
     if (l >= lmin) {
@@ -74,6 +82,50 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
         v.im *= w;
     }
     out[lm * ld + col_offset + c] = v;
+}
+
+// The P and Q weightings of one field in one pass: columns col_offset + r take the P table
+// fP with weight wP, columns col_offset + nr + r the Q table fQ with weight wQ (both
+// (lmax_f+1, nslot, nr)), so each l-major row is written whole by one launch.
+template <typename T>
+__global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __restrict__ fQ, int nslot, int nr,
+                                       const cplx<T>* __restrict__ h, int npol, int lmax, int lmin, T wP, T wQ,
+                                       int ld, cplx<T>* __restrict__ out, int col_offset) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const int nc = 2 * nr;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlm * nc) return;
+    long lm;
+    int c2;
+    if (nlm * nc < (1L << 31)) {  // 32-bit index arithmetic
+        const unsigned iu = (unsigned)i;
+        lm = (long)(iu / (unsigned)nc);
+        c2 = (int)(iu - (unsigned)lm * (unsigned)nc);
+    } else {
+        lm = i / nc;
+        c2 = (int)(i % nc);
+    }
+    const bool isQ = c2 >= nr;
+    const int c = isQ ? c2 - nr : c2;
+    const T* f = isQ ? fQ : fP;
+    const int l = l_of_lm(lm, lmax);
+    cplx<T> v{T(0), T(0)};
+    if (l >= lmin) {
+        const cplx<T> h0 = h[lm];
+        const T f0 = f[((long)l * nslot + 0) * nr + c];
+        v.re = f0 * h0.re;
+        v.im = f0 * h0.im;
+        if (npol > 1) {
+            const cplx<T> h1 = h[nlm + lm];
+            const T f1 = f[((long)l * nslot + 1) * nr + c];
+            v.re += f1 * h1.re;
+            v.im += f1 * h1.im;
+        }
+        const T w = isQ ? wQ : wP;
+        v.re *= w;
+        v.im *= w;
+    }
+    out[lm * ld + col_offset + c2] = v;
 }
 
 // ------------------------------------------------------------- gNL-loc sums
@@ -362,6 +414,18 @@ static int weight_alm_impl(const void* f, int nslot, int nr, const void* h, int 
 }
 
 template <typename T>
+static int weight_alm_pq_impl(const void* fP, const void* fQ, int nslot, int nr, const void* h, int npol, int lmax,
+                              int lmin, double wP, double wQ, int ld, void* out, int col_offset, void* stream) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const long tot = nlm * 2 * nr;
+    hipLaunchKernelGGL(k_weight_alm_pq_lmajor<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax,
+                       lmin, (T)wP, (T)wQ, ld, (cplx<T>*)out, col_offset);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
 static int reduce_launch_fused(const void* P, const void* Q, const void* U, const void* V, const double* w, int nr,
                                long long npix, double c4, double c2, double* part, double* out, int accumulate,
                                hipStream_t st) {
@@ -392,6 +456,19 @@ int ps_weight_alm_lmajor(int precision, const void* f, int nslot, int nr, const 
     if (precision == 32) return weight_alm_impl<float>(f, nslot, nr, h, npol, lmax, lmin, weight, ld, out_t, col_offset, stream);
     PS_REQUIRE(precision == 64, "ps_weight_alm_lmajor: precision must be 32 or 64");
     return weight_alm_impl<double>(f, nslot, nr, h, npol, lmax, lmin, weight, ld, out_t, col_offset, stream);
+}
+
+int ps_weight_alm_pq_lmajor(int precision, const void* fP, const void* fQ, int nslot, int nr, const void* h,
+                            int npol, int lmax, int lmin, double wP, double wQ, int ld, int col_offset, void* out_t,
+                            void* stream) {
+    PS_REQUIRE(fP && fQ && h && out_t, "ps_weight_alm_pq_lmajor: NULL argument");
+    PS_REQUIRE(nr >= 1 && col_offset >= 0 && col_offset + 2 * nr <= ld, "ps_weight_alm_pq_lmajor: bad nr/ld/col_offset");
+    PS_REQUIRE(npol >= 1 && npol <= 2 && nslot >= npol, "ps_weight_alm_pq_lmajor: npol must be 1 or 2 and <= nslot");
+    PS_REQUIRE(lmin >= 0 && lmax >= lmin, "ps_weight_alm_pq_lmajor: bad l range");
+    if (precision == 32)
+        return weight_alm_pq_impl<float>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset, stream);
+    PS_REQUIRE(precision == 64, "ps_weight_alm_pq_lmajor: precision must be 32 or 64");
+    return weight_alm_pq_impl<double>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset, stream);
 }
 
 int ps_gnl_loc_fused_sum(int precision, const void* P, const void* Q, const void* U, const void* V,

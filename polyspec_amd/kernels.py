@@ -56,6 +56,21 @@ def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
     return out_t
 
 
+def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npol=None):
+    """The P and Q weightings of one field (apply_fl_weights with fP, wP and with fQ, wQ) into the
+    l-major columns col_offset + [0, nr) and col_offset + [nr, 2 nr), one launch.  fP and fQ:
+    (>= lmax+1, nslot, nr) with the same nslot and nr (their l ranges may differ)."""
+    _chk(fP, fQ, h, out_t)
+    assert fP.shape[1:] == fQ.shape[1:] and fP.dtype == fQ.dtype, "P and Q tables must share their layout"
+    assert min(fP.shape[0], fQ.shape[0]) > lmax, "the tables must cover l <= lmax"
+    nr = fP.shape[2]
+    npol = min(fP.shape[1], h.shape[0], 2) if npol is None else npol
+    check(load().ps_weight_alm_pq_lmajor(_prec(h), ptr(fP), ptr(fQ), fP.shape[1], nr, ptr(h), npol, int(lmax),
+                                         int(lmin), float(wP), float(wQ), out_t.shape[1], int(col_offset),
+                                         ptr(out_t), stream_handle()))
+    return out_t
+
+
 def gnl_loc_fused_sum(P, Q, w, c4=1.0, U=None, V=None, c2=0.0, out=None, accumulate=False):
     """c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] as a device scalar (U, V FP64)."""
     _chk(P, Q, U, V)
@@ -280,6 +295,6 @@ def cg_update_p_batch(num, den, active, z, p):
 
 
 __all__ = ["lmat_apply", "mdot", "cg_update_xr", "cg_update_p", "lmat_apply_batch", "mdot_batch",
-           "cg_update_xr_batch", "cg_update_p_batch", "multiplyRR_batch", "convolve_F_lmajor", "weight_alm_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
+           "cg_update_xr_batch", "cg_update_p_batch", "multiplyRR_batch", "convolve_F_lmajor", "weight_alm_lmajor", "weight_alm_pq_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
            "multiplyRR", "multiplyRR_sym", "multiply", "multiply_asym", "radial_sum", "tau_sum_n0",
            "outer_product", "_abi"]

@@ -399,8 +399,8 @@ class TSpecTemplate:
         ncol = 2 * nr * len(hs)
         buf, plan = self._lmajor_buffer(ncol, self.lmax)
         for i, h in enumerate(hs):
-            K.weight_alm_lmajor(self.fP_d, h, self.lmax, self.lmin, P_weight, buf, col_offset=2 * nr * i)
-            K.weight_alm_lmajor(self.fQ_d, h, self.lmax, self.lmin, 1., buf, col_offset=2 * nr * i + nr)
+            K.weight_alm_pq_lmajor(self.fP_d, self.fQ_d, h, self.lmax, self.lmin, P_weight, 1., buf,
+                                   col_offset=2 * nr * i)
         if self.kernel_events is not None:
             # split stages with device events around the Legendre stage (kernel timing)
             key = ('phase', ncol)

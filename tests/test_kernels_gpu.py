@@ -59,6 +59,15 @@ def test_weight_radial_tau_outer(gpu, prec):
     got = out.cpu().numpy()
     assert rel(got[:, 3:3 + nr].T, ref) < tol
     assert np.all(got[:, :3] == 0) and np.all(got[:, 3 + nr:] == 0)
+    # both weightings of one field in one launch equal the two single launches bit for bit
+    f2 = rng.standard_normal((lmax + 2, 3, nr))
+    two = torch.full((len(ls), 2 * nr + 4), float("nan"), dtype=cdt, device='cuda')
+    K.weight_alm_lmajor(T(f, rdt), T(h, cdt), lmax, lmin, 0.7, two, col_offset=2)
+    K.weight_alm_lmajor(T(f2, rdt), T(h, cdt), lmax, lmin, -1.3, two, col_offset=2 + nr)
+    one = torch.full_like(two, float("nan"))
+    K.weight_alm_pq_lmajor(T(f, rdt), T(f2, rdt), T(h, cdt), lmax, lmin, 0.7, -1.3, one, col_offset=2)
+    assert torch.equal(one[:, 2:2 + 2 * nr], two[:, 2:2 + 2 * nr])
+    assert torch.isnan(one[:, :2].real).all() and torch.isnan(one[:, 2 + 2 * nr:].real).all()
 
     lm = rng.standard_normal((nr, len(ls))) + 1j * rng.standard_normal((nr, len(ls)))
     w = rng.random(nr)
