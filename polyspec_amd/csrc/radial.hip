@@ -8,6 +8,8 @@
 #include <math.h>
 
 #include "polyspec_hip.h"
+#include <algorithm>
+
 #include "ps_common.h"
 
 namespace ps {
@@ -93,39 +95,42 @@ __global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __rest
                                        int ld, cplx<T>* __restrict__ out, int col_offset) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const int nc = 2 * nr;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlm * nc) return;
-    long lm;
-    int c2;
-    if (nlm * nc < (1L << 31)) {  // 32-bit index arithmetic
-        const unsigned iu = (unsigned)i;
-        lm = (long)(iu / (unsigned)nc);
-        c2 = (int)(iu - (unsigned)lm * (unsigned)nc);
-    } else {
-        lm = i / nc;
-        c2 = (int)(i % nc);
-    }
-    const bool isQ = c2 >= nr;
-    const int c = isQ ? c2 - nr : c2;
-    const T* f = isQ ? fQ : fP;
-    const int l = l_of_lm(lm, lmax);
-    cplx<T> v{T(0), T(0)};
-    if (l >= lmin) {
-        const cplx<T> h0 = h[lm];
-        const T f0 = f[((long)l * nslot + 0) * nr + c];
-        v.re = f0 * h0.re;
-        v.im = f0 * h0.im;
-        if (npol > 1) {
-            const cplx<T> h1 = h[nlm + lm];
-            const T f1 = f[((long)l * nslot + 1) * nr + c];
-            v.re += f1 * h1.re;
-            v.im += f1 * h1.im;
+    const long tot = nlm * nc;
+    // a fixed grid strides over the (lm, column) elements: blocks of single stores would be
+    // bound by the workgroup dispatch rate
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        long lm;
+        int c2;
+        if (tot < (1L << 31)) {  // 32-bit index arithmetic
+            const unsigned iu = (unsigned)i;
+            lm = (long)(iu / (unsigned)nc);
+            c2 = (int)(iu - (unsigned)lm * (unsigned)nc);
+        } else {
+            lm = i / nc;
+            c2 = (int)(i % nc);
         }
-        const T w = isQ ? wQ : wP;
-        v.re *= w;
-        v.im *= w;
+        const bool isQ = c2 >= nr;
+        const int c = isQ ? c2 - nr : c2;
+        const T* f = isQ ? fQ : fP;
+        const int l = l_of_lm(lm, lmax);
+        cplx<T> v{T(0), T(0)};
+        if (l >= lmin) {
+            const cplx<T> h0 = h[lm];
+            const T f0 = f[((long)l * nslot + 0) * nr + c];
+            v.re = f0 * h0.re;
+            v.im = f0 * h0.im;
+            if (npol > 1) {
+                const cplx<T> h1 = h[nlm + lm];
+                const T f1 = f[((long)l * nslot + 1) * nr + c];
+                v.re += f1 * h1.re;
+                v.im += f1 * h1.im;
+            }
+            const T w = isQ ? wQ : wP;
+            v.re *= w;
+            v.im *= w;
+        }
+        out[lm * ld + col_offset + c2] = v;
     }
-    out[lm * ld + col_offset + c2] = v;
 }
 
 // ------------------------------------------------------------- gNL-loc sums
@@ -418,7 +423,8 @@ static int weight_alm_pq_impl(const void* fP, const void* fQ, int nslot, int nr,
                               int lmin, double wP, double wQ, int ld, void* out, int col_offset, void* stream) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
     const long tot = nlm * 2 * nr;
-    hipLaunchKernelGGL(k_weight_alm_pq_lmajor<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+    const long nblk = std::min<long>((tot + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_weight_alm_pq_lmajor<T>, dim3((unsigned)nblk), dim3(256), 0,
                        (hipStream_t)stream, (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax,
                        lmin, (T)wP, (T)wQ, ld, (cplx<T>*)out, col_offset);
     PS_LAUNCH_CHECK();
