@@ -2349,15 +2349,33 @@ __global__ void k_adj_sum_splits(const cplx<T>* __restrict__ part, int nsplit, i
 }
 
 // --------------------------------------------------------- layout helpers
-// alm [col][nlm] (row stride alm_stride) -> alm_t [nlm][ld]; padding columns zeroed
+// alm [col][nlm] (row stride alm_stride) -> alm_t [nlm][ld]; padding columns zeroed.  A block
+// transposes a tile of 64 values of lm x 16 columns through LDS: the loads run along lm (one
+// 64-element run per column), the stores along the columns of each lm row.
+constexpr int TRL = 64, TRC = 16;
+
 template <typename T>
-__global__ void k_alm_to_lmajor(const cplx<T>* __restrict__ alm, long alm_stride, int ncol, long nlm,
-                                int ld, cplx<T>* __restrict__ out) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlm * ld) return;
-    long lm = i / ld;
-    int col = (int)(i % ld);
-    out[i] = (col < ncol) ? alm[(size_t)col * alm_stride + lm] : cplx<T>{T(0), T(0)};
+__global__ __launch_bounds__(256) void k_alm_to_lmajor(const cplx<T>* __restrict__ alm, long alm_stride, int ncol,
+                                                       long nlm, int ld, cplx<T>* __restrict__ out) {
+    __shared__ cplx<T> tile[TRC][TRL + 1];
+    const long lm0 = (long)blockIdx.x * TRL;
+    const int c0 = blockIdx.y * TRC;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TRL * TRC / 256; k++) {
+        const int e = tid + 256 * k, c = e / TRL, t = e % TRL;  // t fastest: coalesced along lm
+        const long lm = lm0 + t;
+        const int col = c0 + c;
+        tile[c][t] = (col < ncol && lm < nlm) ? alm[(size_t)col * alm_stride + lm] : cplx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TRL * TRC / 256; k++) {
+        const int e = tid + 256 * k, t = e / TRC, c = e % TRC;  // c fastest: coalesced along a row
+        const long lm = lm0 + t;
+        const int col = c0 + c;
+        if (lm < nlm && col < ld) out[lm * ld + col] = tile[c][t];
+    }
 }
 
 // ---------------------------------------------------------------- drivers
@@ -2822,13 +2840,12 @@ int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map,
 
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
                       hipStream_t st) {
-    long tot = p.nlm * (long)ld;
-    unsigned nb = (unsigned)((tot + 255) / 256);
+    const dim3 grid((unsigned)((p.nlm + TRL - 1) / TRL), (unsigned)((ld + TRC - 1) / TRC));
     if (p.prec == 4)
-        hipLaunchKernelGGL(k_alm_to_lmajor<float>, dim3(nb), dim3(256), 0, st, (const cplx<float>*)alm,
+        hipLaunchKernelGGL(k_alm_to_lmajor<float>, grid, dim3(256), 0, st, (const cplx<float>*)alm,
                            alm_stride, ncol, p.nlm, ld, (cplx<float>*)alm_t);
     else
-        hipLaunchKernelGGL(k_alm_to_lmajor<double>, dim3(nb), dim3(256), 0, st, (const cplx<double>*)alm,
+        hipLaunchKernelGGL(k_alm_to_lmajor<double>, grid, dim3(256), 0, st, (const cplx<double>*)alm,
                            alm_stride, ncol, p.nlm, ld, (cplx<double>*)alm_t);
     PS_LAUNCH_CHECK();
     return 0;
