@@ -565,27 +565,27 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0(ShtDev<T> d, int ncol, int
                     }
                 }
             } else {
-                for (int li = 0; li < n; li++) {
-                    if (__any(r.k >= scale_consts<T>::KMIN - 1)) {
-                        const T v = (T)(r.f * r.cf);
-                        if (li & 1) {
-    #pragma unroll
-                            for (int j = 0; j < BC; j++) {
-                                const cplx<T> al = sa[li * BC + j];
-                                od[j].re += v * al.re;
-                                od[j].im += v * al.im;
+                // groups of 4 values of l: the parity of l and the exponent update are fixed per
+                // unrolled step (no per-l branches); a partial last group is not rescaled
+                for (int li0 = 0; li0 < n; li0 += 4) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int li = li0 + c;
+                        if (li < n) {
+                            if (__any(r.k >= scale_consts<T>::KMIN - 1)) {
+                                const T v = (T)(r.f * r.cf);
+#pragma unroll
+                                for (int j = 0; j < BC; j++) {
+                                    const cplx<T> al = sa[li * BC + j];
+                                    cplx<T>& acc = (c & 1) ? od[j] : ev[j];
+                                    acc.re += v * al.re;
+                                    acc.im += v * al.im;
+                                }
                             }
-                        } else {
-    #pragma unroll
-                            for (int j = 0; j < BC; j++) {
-                                const cplx<T> al = sa[li * BC + j];
-                                ev[j].re += v * al.re;
-                                ev[j].im += v * al.im;
-                            }
+                            rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
                         }
                     }
-                    rec_step(r, sco[2 * li] * x, sco[2 * li + 1]);
-                    if ((li & 3) == 3) rec_rescale(r);
+                    if (li0 + 3 < n) rec_rescale(r);
                 }
             }
         }
@@ -1181,19 +1181,24 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp(ShtDev<T> d, int nmap, int
                 if (li < n)
                     synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, (T)(0.5 * (rp.f + rm.f)), (T)(0.5 * (rp.f - rm.f)));
             } else {
-                for (int li = 0; li < n; li++) {
-                    if (__any((rp.k >= scale_consts<T>::KMIN - 1) || (rm.k >= scale_consts<T>::KMIN - 1))) {
-                        const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
-                        const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
-                        if (li & 1)
-                            synth_sp_acc<T, BC>(acc1, sa + li * 2 * BC, lp, ln);
-                        else
-                            synth_sp_acc<T, BC>(acc0, sa + li * 2 * BC, lp, ln);
+                // groups of 4 values of l: the accumulator and the exponent update are fixed per
+                // unrolled step (no per-l branches); a partial last group is not rescaled
+                for (int li0 = 0; li0 < n; li0 += 4) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int li = li0 + c;
+                        if (li < n) {
+                            if (__any((rp.k >= scale_consts<T>::KMIN - 1) || (rm.k >= scale_consts<T>::KMIN - 1))) {
+                                const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                                const T lp = (T)(0.5 * (fp + fm)), ln = (T)(0.5 * (fp - fm));
+                                synth_sp_acc<T, BC>((c & 1) ? acc1 : acc0, sa + li * 2 * BC, lp, ln);
+                            }
+                            const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
+                            rec_step(rp, al * (x + be), ga);
+                            rec_step(rm, al * (x - be), ga);
+                        }
                     }
-                    const double al = sco[3 * li], be = sco[3 * li + 1], ga = sco[3 * li + 2];
-                    rec_step(rp, al * (x + be), ga);
-                    rec_step(rm, al * (x - be), ga);
-                    if ((li & 3) == 3) {
+                    if (li0 + 3 < n) {
                         rec_rescale(rp);
                         rec_rescale(rm);
                     }
