@@ -88,6 +88,8 @@ int ps_sht_adjoint(const ps_sht_plan *plan, int nmaps, const void *map, void *al
  * with the built-in choice to rounding.  Shapes are stored in the plan: set or tune them
  * before, not during, transforms on other threads.
  *
+ * Ring-stage ids: FFT buffers (of two real columns each) per block, 1, 2 or 4, plus 8 for the
+ * self-sorting radix-16 transforms (9, 10) instead of the radix-2^2 ones (the built-in choice).
  * candidates: the shape ids of `stage` (count returned; at most max_ids written).
  * get_shapes: shapes[4] in use for nmaps (tuned or set: returns 1; built-in rule: 0).
  * set_shapes: shapes[4], -1 keeps the built-in rule for a stage.

@@ -30,6 +30,7 @@ struct ShtPlan {
     int nrings = 0, npairs = 0, ncomp = 1;
     long npix = 0, nlm = 0, maxM = 1;
     long maxM_pow2 = 1, maxM_blue = 1;  // largest FFT length of the power-of-two / Bluestein rings
+    long blue_total = 0;                // Bluestein kernel entries per order (d_blue holds two orders)
     int n_pow2 = 0;                     // power-of-two rings, listed first in d_rlist
     int* d_rlist = nullptr;             // ring indices: power-of-two rings, then Bluestein rings
     int device = 0;

@@ -86,7 +86,8 @@ int ShtPlan::build() {
     std::vector<RingInfo> rings(nrings);
     std::map<int, int> blue_off;  // nphi -> offset into the Bluestein table
     std::map<int, int> chirp_off;  // nphi -> offset into the chirp table
-    long blue_total = 0, chirp_total = 0;
+    long chirp_total = 0;
+    blue_total = 0;
     maxM = 1;
     for (int i = 1; i <= nrings; i++) {
         int ir = (i > 2 * N) ? 4 * N - i : i;
@@ -149,10 +150,11 @@ int ShtPlan::build() {
         tw[2 * k] = c;
         tw[2 * k + 1] = s;
     }
-    // Bluestein kernels V = FFT_M(v), v_t = exp(-i pi t^2 / n) wrapped; stored
-    // in bit-reversed order (the pointwise product is taken between a DIF
-    // forward transform and a DIT inverse transform).
-    std::vector<double> blue((size_t)std::max<long>(1, 2 * blue_total));
+    // Bluestein kernels V = FFT_M(v), v_t = exp(-i pi t^2 / n) wrapped; stored in
+    // bit-reversed order (for the product taken between a DIF forward transform and a DIT
+    // inverse transform), then again in natural order at offset blue_total (for the
+    // self-sorting transforms).
+    std::vector<double> blue((size_t)std::max<long>(1, 4 * blue_total));
     for (auto& kv : blue_off) {
         int n = kv.first;
         long M = 1L << ilog2(2L * n - 1);
@@ -172,6 +174,8 @@ int ShtPlan::build() {
                 if (k & (1L << b)) rk |= 1u << (lg - 1 - b);
             blue[2 * (kv.second + rk)] = v[k].real();
             blue[2 * (kv.second + rk) + 1] = v[k].imag();
+            blue[2 * (blue_total + kv.second + k)] = v[k].real();
+            blue[2 * (blue_total + kv.second + k) + 1] = v[k].imag();
         }
     }
 
@@ -317,7 +321,8 @@ struct ShtDev {
     const int *seedA, *seedB;
     const double* cth;  // recursion runs in FP64 for both precisions
     const cplx<T>* tw;
-    const cplx<T>* blue;
+    const cplx<T>* blue;   // bit-reversed order; natural order from blue + blue_nat
+    long blue_nat;
     const cplx<T>* chirp;  // exp(i pi t^2 / nphi) of the Bluestein rings, RingInfo::coff
     const double* coef;  // (alpha, beta, gamma, 0) per idx(l, m)
     const unsigned char* dead;  // (mmax+1) x npairs, see ShtPlan::d_dead
@@ -347,6 +352,7 @@ static ShtDev<T> dev_view(const ShtPlan& p) {
     d.cth = (const double*)p.d_cth;
     d.tw = (const cplx<T>*)p.d_tw;
     d.blue = (const cplx<T>*)p.d_blue;
+    d.blue_nat = p.blue_total;
     d.chirp = (const cplx<T>*)p.d_chirp;
     d.coef = (const double*)p.d_coef;
     d.maxM = p.maxM;
@@ -1438,6 +1444,141 @@ __device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, lo
     }
 }
 
+// ------------------------------------------- self-sorting (Stockham) ring FFTs (LDS)
+
+// LDS element index of entry i of a self-sorting transform buffer: bits 4..7 of i XORed into
+// bits 0..3.  The first pass writes entries 16 apart across lanes; the XOR puts 16 such lanes
+// on 16 distinct 8-byte slots (all 32 4-byte banks).  Every other access touches runs of 16
+// consecutive aligned entries, which the XOR only permutes.
+__device__ __forceinline__ int sidx(int i) { return i ^ ((i >> 4) & 15); }
+
+// cos(2 pi k / 16), sin(2 pi k / 16)
+__host__ __device__ constexpr double ur16_cos(int k) {
+    constexpr double c[5] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0};
+    k &= 15;
+    return k <= 4 ? c[k] : k <= 8 ? -c[8 - k] : k <= 12 ? -c[k - 8] : c[16 - k];
+}
+__host__ __device__ constexpr double ur16_sin(int k) { return ur16_cos(k + 12); }
+__host__ __device__ constexpr int brev_c(int r, int lr) {
+    int o = 0;
+    for (int b = 0; b < lr; b++)
+        if (r & (1 << b)) o |= 1 << (lr - 1 - b);
+    return o;
+}
+
+// In-register DFT of R = 2, 4, 8 or 16 points (kernel exp(-2 pi i jk / R), or exp(+...) for
+// inv): radix-2 decimation in frequency with compile-time twiddles; v[p] ends holding output
+// bitrev_R(p).
+template <typename T, int R>
+__device__ __forceinline__ void dft_reg(cplx<T> (&v)[R], bool inv) {
+#pragma unroll
+    for (int h = R / 2; h >= 1; h >>= 1) {
+#pragma unroll
+        for (int b = 0; b < R; b += 2 * h) {
+#pragma unroll
+            for (int p = 0; p < h; p++) {
+                const cplx<T> x = v[b + p], y = v[b + p + h];
+                v[b + p] = cadd(x, y);
+                cplx<T> dl = csub(x, y);
+                if (p > 0) {
+                    if (2 * p == h) {
+                        dl = mul_mi(dl, inv);
+                    } else {
+                        const int k = 8 * p / h;
+                        const T c = (T)ur16_cos(k), sn = (T)ur16_sin(k);
+                        dl = cmul(dl, cplx<T>{c, inv ? sn : -sn});
+                    }
+                }
+                v[b + p + h] = dl;
+            }
+        }
+    }
+}
+
+// One self-sorting pass of radix R over NB buffers of length M, after sub-transforms of length
+// Ns: butterfly j < M/R reads entries j + r M/R, multiplies entry r by w^(r (j mod Ns)) with
+// w = exp(-/+ 2 pi i / (Ns R)), transforms the R values in registers and hands output r to
+// out(q, (j - j mod Ns) R + j mod Ns + r Ns, value).  All reads come before the barrier and all
+// writes after it, so the pass runs in place.  Powers w^(2^e) are table entries (indices below
+// maxM / 2); the others are products of at most three of them.  U: butterflies per thread
+// (M / R <= 256 U).
+template <typename T, int NB, int R, int U, typename Out>
+__device__ __forceinline__ void stk_pass(const cplx<T>* a, int M, int Ns, const cplx<T>* __restrict__ tw, long maxM,
+                                         bool inv, Out&& out) {
+    constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+    const int nb = M / R, pitch = fft_pitch(M);
+    cplx<T> v[U][NB][R];
+    cplx<T> w[U][LR];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = threadIdx.x + 256 * u;
+        if (j < nb) {
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+#pragma unroll
+                for (int r = 0; r < R; r++) v[u][q][r] = a[q * pitch + sidx(j + r * nb)];
+            if (Ns > 1) {
+                const long ts = maxM / ((long)Ns * R);
+                const long k = j & (Ns - 1);
+#pragma unroll
+                for (int e = 0; e < LR; e++) {
+                    cplx<T> t = tw[(k << e) * ts];
+                    if (inv) t.im = -t.im;
+                    w[u][e] = t;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = threadIdx.x + 256 * u;
+        if (j < nb) {
+            const int k = j & (Ns - 1);
+            if (Ns > 1) {
+                cplx<T> pw[R];
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const int hb = 1 << (31 - __builtin_clz((unsigned)r));
+                    pw[r] = hb == r ? w[u][31 - __builtin_clz((unsigned)r)] : cmul(pw[hb], pw[r - hb]);
+                }
+#pragma unroll
+                for (int q = 0; q < NB; q++)
+#pragma unroll
+                    for (int r = 1; r < R; r++) v[u][q][r] = cmul(v[u][q][r], pw[r]);
+            }
+#pragma unroll
+            for (int q = 0; q < NB; q++) dft_reg<T, R>(v[u][q], inv);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+#pragma unroll
+                for (int r = 0; r < R; r++) out(q, base + r * Ns, v[u][q][brev_c(r, LR)]);
+        }
+    }
+}
+
+// Self-sorting LDS FFT over NB buffers of length M = 2^logM, natural order in and out (entry i
+// at sidx(i)): radix-16 passes, the last of radix 2^(logM - 4 p) in 2..16.  Every pass but the
+// last writes back to the buffers; the last hands its outputs to last(q, index, value) and the
+// caller synchronises after it.  M <= 1024 Q.
+template <typename T, int NB, int Q, typename Last>
+__device__ __forceinline__ void fft_stk(cplx<T>* a, int M, int logM, const cplx<T>* __restrict__ tw, long maxM,
+                                        bool inv, Last&& last) {
+    constexpr int U16 = Q >= 4 ? Q / 4 : 1, U8 = Q >= 2 ? Q / 2 : 1, U4 = Q, U2 = 2 * Q;
+    const int pitch = fft_pitch(M);
+    auto back = [&](int q, int i, cplx<T> z) { a[q * pitch + sidx(i)] = z; };
+    int Ns = 1, rem = logM;
+    for (; rem > 4; rem -= 4, Ns <<= 4) {
+        stk_pass<T, NB, 16, U16>(a, M, Ns, tw, maxM, inv, back);
+        __syncthreads();
+    }
+    if (rem == 4) stk_pass<T, NB, 16, U16>(a, M, Ns, tw, maxM, inv, last);
+    else if (rem == 3) stk_pass<T, NB, 8, U8>(a, M, Ns, tw, maxM, inv, last);
+    else if (rem == 2) stk_pass<T, NB, 4, U4>(a, M, Ns, tw, maxM, inv, last);
+    else stk_pass<T, NB, 2, U2>(a, M, Ns, tw, maxM, inv, last);
+}
+
 // phases [ring][mmax+1][ncol], already rotated by exp(i m phi0) -> maps [col][npix].
 // One block per (ring of ring_list, group of 2 NB columns).  Columns 2q and 2q+1 of the
 // group share one complex transform: with A_j, B_j their Hermitian folds
@@ -1445,8 +1586,10 @@ __device__ void fft_dit_nb(cplx<T>* a, int M, const cplx<T>* __restrict__ tw, lo
 // the inverse DFT of A + i B is f_2q + i f_2q+1, both real.  m is folded modulo nphi in
 // rounds of nphi consecutive values (each round hits every j once, so no two threads
 // add to one element), direct and mirrored terms in separate passes.  Then an LDS
-// radix-2 FFT (nphi a power of two) or a Bluestein chirp-z transform.
-template <typename T, int NB, int Q>
+// radix-2^2 FFT (nphi a power of two) or a Bluestein chirp-z transform; with STK the
+// self-sorting radix-16 FFT instead, whose last pass writes the maps (or, forward in Bluestein,
+// multiplies by the chirp kernel's transform).
+template <typename T, int NB, int Q, bool STK>
 __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __restrict__ ring_list, int ncol,
                                                     const cplx<T>* __restrict__ phase, T* __restrict__ map,
                                                     long map_stride) {
@@ -1511,7 +1654,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
                 acc.re += Am[u].re + Bm[u].im;          // conj(A) + i conj(B) of the mirror
                 acc.im += Bm[u].re - Am[u].im;
                 if (!pow2) acc = cmul(acc, ch[u]);  // Bluestein pre-chirp
-                buf[q * pitch + pidx(j)] = acc;
+                buf[q * pitch + (STK ? sidx(j) : pidx(j))] = acc;
             }
         }
     } else
@@ -1538,7 +1681,7 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
         }
         if (S == 1) {
             if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);  // Bluestein pre-chirp
-            buf[q * pitch + pidx(j)] = acc;
+            buf[q * pitch + (STK ? sidx(j) : pidx(j))] = acc;
         } else {
             part[i] = acc;
         }
@@ -1550,11 +1693,33 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
             for (int sl = 1; sl < S; sl++) acc = cadd(acc, part[sl * nitem + r]);
             const int j = r / NB, q = r - j * NB;
             if (!pow2) acc = cmul(acc, d.chirp[ri.coff + j]);
-            buf[q * pitch + pidx(j)] = acc;
+            buf[q * pitch + (STK ? sidx(j) : pidx(j))] = acc;
         }
     }
     __syncthreads();
     const int nc = min(G, ncol - c0);
+    if constexpr (STK) {
+        if (pow2) {
+            fft_stk<T, NB, Q>(buf, M, logM, d.tw, d.maxM, true, [&](int q, int k, cplx<T> z) {
+                if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re;
+                if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im;
+            });
+        } else {
+            const cplx<T>* V = d.blue + d.blue_nat + ri.voff;
+            fft_stk<T, NB, Q>(buf, M, logM, d.tw, d.maxM, false,
+                              [&](int q, int i, cplx<T> z) { buf[q * pitch + sidx(i)] = cmul(z, V[i]); });
+            __syncthreads();
+            const T invM = T(1) / (T)M;
+            fft_stk<T, NB, Q>(buf, M, logM, d.tw, d.maxM, true, [&](int q, int k, cplx<T> z) {
+                if (k < n) {
+                    z = cmul(z, d.chirp[ri.coff + k]);
+                    if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
+                    if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
+                }
+            });
+        }
+        return;
+    }
     if (pow2) {
         fft_dif_nb<T, NB, Q>(buf, M, d.tw, d.maxM, true);
         for (int i = tid; i < NB * n; i += 256) {
@@ -1614,8 +1779,8 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
 
 // maps [col][npix] -> phases [col][mmax+1][nrings]: sum_k f_k exp(-i m phi_k).  One block per
 // (ring, column): the ring's pixels into one LDS buffer (loads issued in batches of 8), the
-// radix-2^2 LDS FFT of the synthesis kernels (power-of-two rings) or a Bluestein chirp-z
-// transform, then the m rows with the phi0 rotation.  Q: butterfly slots per thread for the
+// self-sorting radix-16 LDS FFT (power-of-two rings) or a Bluestein chirp-z transform built on
+// it, then the m rows with the phi0 rotation.  Q: butterfly slots per thread for the
 // plan's longest transform (M <= 1024 Q).
 template <typename T, int Q>
 __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __restrict__ map,
@@ -1647,31 +1812,24 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
             if (k < M) {
                 cplx<T> v{x[u], T(0)};
                 if (!pow2 && k < n) v = cmul(v, ch[u]);
-                buf[pidx(k)] = v;
+                buf[sidx(k)] = v;
             }
         }
     }
     __syncthreads();
-    fft_dif_nb<T, 1, Q>(buf, M, d.tw, d.maxM, false);
-    if (!pow2) {
-        // g_j = sum_k f_k exp(+2 pi i jk/n) by Bluestein; C_j = conj(g_j)
-        const cplx<T>* V = d.blue + ri.voff;
-        for (int k0 = tid; k0 < M; k0 += 256 * 8) {
-            cplx<T> v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int k = k0 + 256 * u;
-                if (k < M) v[u] = V[k];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int k = k0 + 256 * u;
-                if (k < M) buf[pidx(k)] = cmul(buf[pidx(k)], v[u]);
-            }
-        }
+    auto back = [&](int, int i, cplx<T> z) { buf[sidx(i)] = z; };
+    if (pow2) {
+        fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, false, back);
+    } else {
+        // g_j = sum_k f_k exp(+2 pi i jk/n) by Bluestein; C_j = conj(g_j).  The forward
+        // transform's last pass multiplies by the chirp kernel's transform.
+        const cplx<T>* V = d.blue + d.blue_nat + ri.voff;
+        fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, false,
+                         [&](int, int i, cplx<T> z) { buf[sidx(i)] = cmul(z, V[i]); });
         __syncthreads();
-        fft_dit_nb<T, 1, Q>(buf, M, d.tw, d.maxM, true);
+        fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, true, back);
     }
+    __syncthreads();
     cplx<T>* P = phase + ((size_t)col * (d.mmax + 1)) * d.nrings + ring;
     const T invM = T(1) / (T)M;
     for (int m0 = tid; m0 <= d.mmax; m0 += 256 * 8) {
@@ -1688,9 +1846,9 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
                 const int j = m % n;
                 cplx<T> C;
                 if (pow2) {
-                    C = buf[pidx(bitrev(j, logM))];
+                    C = buf[sidx(j)];
                 } else {
-                    const cplx<T> g = cmul(buf[pidx(j)], ch[u]);
+                    const cplx<T> g = cmul(buf[sidx(j)], ch[u]);
                     C = cplx<T>{g.re * invM, -g.im * invM};
                 }
                 P[(size_t)m * d.nrings] = cmul(C, expi_pi<T>(-(long)m * php, phq));
@@ -2413,49 +2571,57 @@ static int launch_synth_sp(const ShtPlan& p, int nmap, int ld, const void* alm_t
 }
 
 // Column pairs per block: as many FFT buffers as fit 64 KB of LDS for the ring class.  Q: the
-// butterfly slots per thread for the class's longest transform (maxM <= 1024 Q).
-template <typename T, int NB, int Q>
+// butterfly slots per thread for the class's longest transform (maxM <= 1024 Q).  STK: the
+// self-sorting radix-16 transforms instead of the radix-2^2 ones.
+template <typename T, int NB, int Q, bool STK>
 static int launch_ring_synth_nbq(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
                                  const void* phase, void* map, long map_stride, hipStream_t st) {
     ShtDev<T> d = dev_view<T>(p);
     const size_t sh = (size_t)NB * fft_pitch((int)maxM) * sizeof(cplx<T>);
     if (sh + 256 * sizeof(cplx<T>) > 65536)  // dynamic buffers plus the static slice partials
-        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T, NB, Q>,
+        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth<T, NB, Q, STK>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     const int ngrp = (ncol + 2 * NB - 1) / (2 * NB);
-    hipLaunchKernelGGL((k_ring_synth<T, NB, Q>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d, list, ncol,
-                       (const cplx<T>*)phase, (T*)map, map_stride);
+    hipLaunchKernelGGL((k_ring_synth<T, NB, Q, STK>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d, list,
+                       ncol, (const cplx<T>*)phase, (T*)map, map_stride);
     PS_LAUNCH_CHECK();
     return 0;
 }
 
-template <typename T, int NB>
+template <typename T, int NB, bool STK>
 static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
                                 const void* phase, void* map, long map_stride, hipStream_t st) {
     if (maxM <= 1024)
-        return launch_ring_synth_nbq<T, NB, 1>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        return launch_ring_synth_nbq<T, NB, 1, STK>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
     if (maxM <= 2048)
-        return launch_ring_synth_nbq<T, NB, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        return launch_ring_synth_nbq<T, NB, 2, STK>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
     if constexpr (NB <= 2) {
         if (maxM <= 4096)
-            return launch_ring_synth_nbq<T, NB, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+            return launch_ring_synth_nbq<T, NB, 4, STK>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
     }
     if constexpr (NB == 1)
-        return launch_ring_synth_nbq<T, NB, FFT_MAXQ>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        return launch_ring_synth_nbq<T, NB, FFT_MAXQ, STK>(p, list, nlist, maxM, ncol, phase, map, map_stride,
+                                                           st);
     set_error("ring synthesis: transform too long for this many buffers per block");
     return -1;
 }
 
-// Ring-synthesis shape: rings per block (NB FFT buffers of two real columns each).  The
-// built-in rule fills 64 KB of LDS with as many buffers as the batch can use.
-static int ring_nb_rule(long maxM, int ncol, int prec) {
+// Ring-synthesis shape id: NB FFT buffers of two real columns each per block (1, 2 or 4), plus
+// RING_STK for the self-sorting radix-16 transforms (NB 1 or 2) instead of the radix-2^2 ones.
+// The built-in rule takes the self-sorting transforms: two buffers per block for power-of-two
+// rings when they fit 64 KB of LDS and the batch can use them, one for Bluestein rings (two
+// transforms per block already; measured on MI355X at Nside 512 and 1024, both precisions).
+constexpr int RING_STK = 8;
+
+static int ring_nb_rule(long maxM, int ncol, int prec, bool blue) {
     const long per = maxM * 2L * prec;
-    if (ncol >= 8 && 4 * per <= 65536) return 4;
-    if (ncol >= 4 && 2 * per <= 65536) return 2;
-    return 1;
+    if (!blue && ncol >= 4 && 2 * per <= 65536) return RING_STK | 2;
+    return RING_STK | 1;
 }
 
-static bool ring_nb_valid(long maxM, int ncol, int prec, int nb) {
+static bool ring_nb_valid(long maxM, int ncol, int prec, int id) {
+    const int nb = id & ~RING_STK;
+    if (id < 0 || ((id & RING_STK) && nb > 2)) return false;
     const long per = maxM * 2L * prec;
     // one buffer always (beyond 64 KB it takes the dynamic-LDS attribute); more only within 64 KB
     return nb == 1 || ((nb == 2 || nb == 4) && nb * per <= 65536 && ncol > nb);
@@ -2469,14 +2635,20 @@ static int launch_ring_synth_list(const ShtPlan& p, const int* list, int nlist, 
         set_error("ring synthesis: FFT length above 8192 is not supported");
         return -1;
     }
-    if (nb < 0) nb = ring_nb_rule(maxM, ncol, (int)sizeof(T));
+    if (nb < 0) nb = ring_nb_rule(maxM, ncol, (int)sizeof(T), list != p.d_rlist);
     if (!ring_nb_valid(maxM, ncol, (int)sizeof(T), nb)) {
         set_error("ring synthesis: invalid launch shape");
         return -1;
     }
-    if (nb == 4) return launch_ring_synth_nb<T, 4>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
-    if (nb == 2) return launch_ring_synth_nb<T, 2>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
-    return launch_ring_synth_nb<T, 1>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    switch (nb) {
+        case 4: return launch_ring_synth_nb<T, 4, false>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        case 2: return launch_ring_synth_nb<T, 2, false>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        case RING_STK | 1:
+            return launch_ring_synth_nb<T, 1, true>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        case RING_STK | 2:
+            return launch_ring_synth_nb<T, 2, true>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+        default: return launch_ring_synth_nb<T, 1, false>(p, list, nlist, maxM, ncol, phase, map, map_stride, st);
+    }
 }
 
 template <typename T>
@@ -2874,8 +3046,8 @@ int sht_default_shape(const ShtPlan& p, int nmap, int stage) {
     switch (stage) {
         case SHT_STAGE_SYNTH:
             return p.prec == 4 ? synth_shape_rule<float>(p, ncol) : synth_shape_rule<double>(p, ncol);
-        case SHT_STAGE_RING_POW2: return ring_nb_rule(p.maxM_pow2, ncol, p.prec);
-        case SHT_STAGE_RING_BLUE: return ring_nb_rule(p.maxM_blue, ncol, p.prec);
+        case SHT_STAGE_RING_POW2: return ring_nb_rule(p.maxM_pow2, ncol, p.prec, false);
+        case SHT_STAGE_RING_BLUE: return ring_nb_rule(p.maxM_blue, ncol, p.prec, true);
         case SHT_STAGE_ADJ: {
             const int ns = sht_adjoint_nsplit(p, nmap);
             const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
@@ -2909,6 +3081,8 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
             const long maxM = stage == SHT_STAGE_RING_POW2 ? p.maxM_pow2 : p.maxM_blue;
             for (int nb = 1; nb <= 4; nb *= 2)
                 if (ring_nb_valid(maxM, ncol, p.prec, nb)) add(nb);
+            for (int nb = 1; nb <= 2; nb *= 2)
+                if (ring_nb_valid(maxM, ncol, p.prec, RING_STK | nb)) add(RING_STK | nb);
             break;
         }
         case SHT_STAGE_ADJ: {
