@@ -1778,39 +1778,48 @@ __global__ __launch_bounds__(256) void k_ring_synth(ShtDev<T> d, const int* __re
 }
 
 // maps [col][npix] -> phases [col][mmax+1][nrings]: sum_k f_k exp(-i m phi_k).  One block per
-// (ring, column): the ring's pixels into one LDS buffer (loads issued in batches of 8), the
-// self-sorting radix-16 LDS FFT (power-of-two rings) or a Bluestein chirp-z transform built on
-// it, then the m rows with the phi0 rotation.  Q: butterfly slots per thread for the
-// plan's longest transform (M <= 1024 Q).
+// (ring, pair of columns a, b): the two real rings share one complex transform of
+// z = f_a + i f_b (loads issued in batches of 8), by the self-sorting radix-16 LDS FFT
+// (power-of-two rings) or a Bluestein chirp-z transform built on it, giving D_j = A_j + i B_j
+// with A, B the DFTs of f_a, f_b.  Since f_a, f_b are real, A_j = (D_j + conj D_{n-j}) / 2 and
+// B_j = (D_j - conj D_{n-j}) / 2i; then the m rows with the phi0 rotation.  An odd last column
+// runs with f_b = 0.  Q: butterfly slots per thread for the plan's longest transform
+// (M <= 1024 Q).
 template <typename T, int Q>
-__global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __restrict__ map,
+__global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, int ncol, const T* __restrict__ map,
                                                       long map_stride, cplx<T>* __restrict__ phase) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int ring = lb % d.nrings;
-    const int col = lb / d.nrings;
+    const int ca = 2 * (int)(lb / d.nrings);
+    const bool hasB = ca + 1 < ncol;
     const RingInfo ri = d.rings[ring];
     const int n = ri.nphi, M = ri.M, logM = ri.logM;
     const bool pow2 = ri.voff < 0;
     const int tid = threadIdx.x;
-    const T* in = map + (size_t)col * map_stride + ri.start;
+    const T* inA = map + (size_t)ca * map_stride + ri.start;
+    const T* inB = inA + map_stride;
     const long phq = ri.phi_q, php = ri.phi_p;
 
+    // Bluestein evaluates g_j = sum_k w_k exp(+2 pi i jk/n) and takes conj(g_j); with
+    // w = conj(z) = f_a - i f_b that is D_j.
+    const T sgnB = pow2 ? T(1) : T(-1);
     for (int k0 = tid; k0 < M; k0 += 256 * 8) {
-        T x[8];
+        T x[8], y[8];
         cplx<T> ch[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int k = k0 + 256 * u;
-            x[u] = k < n ? in[k] : T(0);
+            x[u] = k < n ? inA[k] : T(0);
+            y[u] = (hasB && k < n) ? inB[k] : T(0);
             if (!pow2 && k < n) ch[u] = d.chirp[ri.coff + k];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int k = k0 + 256 * u;
             if (k < M) {
-                cplx<T> v{x[u], T(0)};
+                cplx<T> v{x[u], sgnB * y[u]};
                 if (!pow2 && k < n) v = cmul(v, ch[u]);
                 buf[sidx(k)] = v;
             }
@@ -1821,8 +1830,7 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
     if (pow2) {
         fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, false, back);
     } else {
-        // g_j = sum_k f_k exp(+2 pi i jk/n) by Bluestein; C_j = conj(g_j).  The forward
-        // transform's last pass multiplies by the chirp kernel's transform.
+        // the forward transform's last pass multiplies by the chirp kernel's transform
         const cplx<T>* V = d.blue + d.blue_nat + ri.voff;
         fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, false,
                          [&](int, int i, cplx<T> z) { buf[sidx(i)] = cmul(z, V[i]); });
@@ -1830,28 +1838,38 @@ __global__ __launch_bounds__(256) void k_ring_adjoint(ShtDev<T> d, const T* __re
         fft_stk<T, 1, Q>(buf, M, logM, d.tw, d.maxM, true, back);
     }
     __syncthreads();
-    cplx<T>* P = phase + ((size_t)col * (d.mmax + 1)) * d.nrings + ring;
+    cplx<T>* PA = phase + ((size_t)ca * (d.mmax + 1)) * d.nrings + ring;
+    cplx<T>* PB = PA + (size_t)(d.mmax + 1) * d.nrings;
     const T invM = T(1) / (T)M;
     for (int m0 = tid; m0 <= d.mmax; m0 += 256 * 8) {
-        cplx<T> ch[8];
+        cplx<T> c1[8], c2[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int m = m0 + 256 * u;
-            if (!pow2 && m <= d.mmax) ch[u] = d.chirp[ri.coff + m % n];
+            if (!pow2 && m <= d.mmax) {
+                const int j = m % n;
+                c1[u] = d.chirp[ri.coff + j];
+                c2[u] = d.chirp[ri.coff + (j ? n - j : 0)];
+            }
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int m = m0 + 256 * u;
             if (m <= d.mmax) {
-                const int j = m % n;
-                cplx<T> C;
-                if (pow2) {
-                    C = buf[sidx(j)];
-                } else {
-                    const cplx<T> g = cmul(buf[sidx(j)], ch[u]);
-                    C = cplx<T>{g.re * invM, -g.im * invM};
+                const int j = m % n, j2 = j ? n - j : 0;
+                cplx<T> D1 = buf[sidx(j)], D2 = buf[sidx(j2)];
+                if (!pow2) {
+                    const cplx<T> g1 = cmul(D1, c1[u]), g2 = cmul(D2, c2[u]);
+                    D1 = cplx<T>{g1.re * invM, -g1.im * invM};
+                    D2 = cplx<T>{g2.re * invM, -g2.im * invM};
                 }
-                P[(size_t)m * d.nrings] = cmul(C, expi_pi<T>(-(long)m * php, phq));
+                const cplx<T> rot = expi_pi<T>(-(long)m * php, phq);
+                const cplx<T> A{T(0.5) * (D1.re + D2.re), T(0.5) * (D1.im - D2.im)};
+                PA[(size_t)m * d.nrings] = cmul(A, rot);
+                if (hasB) {
+                    const cplx<T> B{T(0.5) * (D1.im + D2.im), T(0.5) * (D2.re - D1.re)};
+                    PB[(size_t)m * d.nrings] = cmul(B, rot);
+                }
             }
         }
     }
@@ -2669,8 +2687,8 @@ static int launch_ring_adjoint_q(const ShtPlan& p, int ncol, const void* map, lo
     if (sh > 65536)
         PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_adjoint<T, Q>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL((k_ring_adjoint<T, Q>), dim3((unsigned)(p.nrings * ncol)), dim3(256), sh, st, d,
-                       (const T*)map, map_stride, (cplx<T>*)phase);
+    hipLaunchKernelGGL((k_ring_adjoint<T, Q>), dim3((unsigned)(p.nrings * ((ncol + 1) / 2))), dim3(256), sh, st,
+                       d, ncol, (const T*)map, map_stride, (cplx<T>*)phase);
     PS_LAUNCH_CHECK();
     return 0;
 }
