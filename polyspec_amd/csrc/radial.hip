@@ -88,48 +88,68 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
 
 // The P and Q weightings of one field in one pass: columns col_offset + r take the P table
 // fP with weight wP, columns col_offset + nr + r the Q table fQ with weight wQ (both
-// (lmax_f+1, nslot, nr)), so each l-major row is written whole by one launch.
-template <typename T>
+// (lmax_f+1, nslot, nr)), so each l-major row is written whole by one launch.  Each work item
+// is V consecutive columns of one row (V = 4 when nr, ld and col_offset are multiples of 4 and
+// the output is 32-byte aligned: one index decode and one pair of h loads per four columns, and
+// 16-byte stores); every column is computed as in the one-column form, so both agree bit for bit.
+template <typename T, int V>
 __global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __restrict__ fQ, int nslot, int nr,
                                        const cplx<T>* __restrict__ h, int npol, int lmax, int lmin, T wP, T wQ,
                                        int ld, cplx<T>* __restrict__ out, int col_offset) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    const int nc = 2 * nr;
-    const long tot = nlm * nc;
-    // a fixed grid strides over the (lm, column) elements: blocks of single stores would be
+    const int ng = 2 * nr / V;
+    const long tot = nlm * ng;
+    // a fixed grid strides over the (lm, column group) items: blocks of single stores would be
     // bound by the workgroup dispatch rate
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
         long lm;
-        int c2;
+        int g;
         if (tot < (1L << 31)) {  // 32-bit index arithmetic
             const unsigned iu = (unsigned)i;
-            lm = (long)(iu / (unsigned)nc);
-            c2 = (int)(iu - (unsigned)lm * (unsigned)nc);
+            lm = (long)(iu / (unsigned)ng);
+            g = (int)(iu - (unsigned)lm * (unsigned)ng);
         } else {
-            lm = i / nc;
-            c2 = (int)(i % nc);
+            lm = i / ng;
+            g = (int)(i % ng);
         }
-        const bool isQ = c2 >= nr;
+        const int c2 = g * V;
+        const bool isQ = c2 >= nr;  // a group never straddles P and Q (nr % V == 0)
         const int c = isQ ? c2 - nr : c2;
         const T* f = isQ ? fQ : fP;
         const int l = l_of_lm(lm, lmax);
-        cplx<T> v{T(0), T(0)};
+        cplx<T> v[V];
+#pragma unroll
+        for (int k = 0; k < V; k++) v[k] = cplx<T>{T(0), T(0)};
         if (l >= lmin) {
             const cplx<T> h0 = h[lm];
-            const T f0 = f[((long)l * nslot + 0) * nr + c];
-            v.re = f0 * h0.re;
-            v.im = f0 * h0.im;
-            if (npol > 1) {
-                const cplx<T> h1 = h[nlm + lm];
-                const T f1 = f[((long)l * nslot + 1) * nr + c];
-                v.re += f1 * h1.re;
-                v.im += f1 * h1.im;
-            }
+            const cplx<T> h1 = npol > 1 ? h[nlm + lm] : cplx<T>{T(0), T(0)};
             const T w = isQ ? wQ : wP;
-            v.re *= w;
-            v.im *= w;
+            const T* f0 = f + ((long)l * nslot + 0) * nr + c;
+            const T* f1 = f0 + nr;
+#pragma unroll
+            for (int k = 0; k < V; k++) {
+                v[k].re = f0[k] * h0.re;
+                v[k].im = f0[k] * h0.im;
+                if (npol > 1) {
+                    v[k].re += f1[k] * h1.re;
+                    v[k].im += f1[k] * h1.im;
+                }
+                v[k].re *= w;
+                v[k].im *= w;
+            }
         }
-        out[lm * ld + col_offset + c2] = v;
+        cplx<T>* o = out + lm * ld + col_offset + c2;
+        if constexpr (V == 4 && sizeof(T) == 4) {
+            float4* o4 = reinterpret_cast<float4*>(o);
+            o4[0] = make_float4(v[0].re, v[0].im, v[1].re, v[1].im);
+            o4[1] = make_float4(v[2].re, v[2].im, v[3].re, v[3].im);
+        } else if constexpr (V == 4) {
+            double2* o2 = reinterpret_cast<double2*>(o);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o2[k] = make_double2(v[k].re, v[k].im);
+        } else {
+            o[0] = v[0];
+        }
     }
 }
 
@@ -422,11 +442,17 @@ template <typename T>
 static int weight_alm_pq_impl(const void* fP, const void* fQ, int nslot, int nr, const void* h, int npol, int lmax,
                               int lmin, double wP, double wQ, int ld, void* out, int col_offset, void* stream) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    const long tot = nlm * 2 * nr;
+    const bool vec = nr % 4 == 0 && ld % 4 == 0 && col_offset % 4 == 0 && (uintptr_t)out % 32 == 0;
+    const long tot = nlm * 2 * nr / (vec ? 4 : 1);
     const long nblk = std::min<long>((tot + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_weight_alm_pq_lmajor<T>, dim3((unsigned)nblk), dim3(256), 0,
-                       (hipStream_t)stream, (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax,
-                       lmin, (T)wP, (T)wQ, ld, (cplx<T>*)out, col_offset);
+    if (vec)
+        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 4>), dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream,
+                           (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)wP, (T)wQ,
+                           ld, (cplx<T>*)out, col_offset);
+    else
+        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 1>), dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream,
+                           (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)wP, (T)wQ,
+                           ld, (cplx<T>*)out, col_offset);
     PS_LAUNCH_CHECK();
     return 0;
 }

@@ -68,6 +68,16 @@ def test_weight_radial_tau_outer(gpu, prec):
     K.weight_alm_pq_lmajor(T(f, rdt), T(f2, rdt), T(h, cdt), lmax, lmin, 0.7, -1.3, one, col_offset=2)
     assert torch.equal(one[:, 2:2 + 2 * nr], two[:, 2:2 + 2 * nr])
     assert torch.isnan(one[:, :2].real).all() and torch.isnan(one[:, 2 + 2 * nr:].real).all()
+    # the four-column form (nr, ld and col_offset multiples of 4) agrees with the one-column form
+    n4 = 8
+    g1, g2 = rng.standard_normal((2, lmax + 2, 3, n4))
+    two = torch.full((len(ls), 2 * n4 + 8), float("nan"), dtype=cdt, device='cuda')
+    K.weight_alm_lmajor(T(g1, rdt), T(h, cdt), lmax, lmin, 0.7, two, col_offset=4)
+    K.weight_alm_lmajor(T(g2, rdt), T(h, cdt), lmax, lmin, -1.3, two, col_offset=4 + n4)
+    one = torch.full_like(two, float("nan"))
+    K.weight_alm_pq_lmajor(T(g1, rdt), T(g2, rdt), T(h, cdt), lmax, lmin, 0.7, -1.3, one, col_offset=4)
+    assert torch.equal(one[:, 4:4 + 2 * n4], two[:, 4:4 + 2 * n4])
+    assert torch.isnan(one[:, :4].real).all() and torch.isnan(one[:, 4 + 2 * n4:].real).all()
 
     lm = rng.standard_normal((nr, len(ls))) + 1j * rng.standard_normal((nr, len(ls)))
     w = rng.random(nr)
