@@ -44,6 +44,10 @@ struct ShtPlan {
     // per-batch-width launch shapes (not thread-safe against concurrent transforms on this plan)
     ShtShapes shapes[SHT_MAX_SHAPES];
     int nshapes = 0;
+    // side stream and events of the overlapped spin-0 synthesis (created on first use, on the
+    // plan's device; like the shapes, not for concurrent transforms on this plan)
+    mutable hipStream_t side = nullptr;
+    mutable hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     const ShtShapes* shapes_for(int ncol) const {
         for (int i = 0; i < nshapes; i++)
             if (shapes[i].ncol == ncol) return &shapes[i];

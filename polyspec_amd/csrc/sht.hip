@@ -304,6 +304,9 @@ int ShtPlan::build() {
 }
 
 ShtPlan::~ShtPlan() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     void* ptrs[] = {d_rings, d_l2s, d_l2sh, d_l2ch, d_seedL, d_seedS, d_seedA,
                     d_seedB, d_cth, d_tw, d_blue, d_coef, d_rlist, d_dead, d_chirp};
     for (void* p : ptrs)
@@ -675,18 +678,18 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 template <int NT>
 __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int ncol, int ld,
                                                            const cplx<float>* __restrict__ alm_t,
-                                                           cplx<float>* __restrict__ phase) {
+                                                           cplx<float>* __restrict__ phase, int rb0, int nrbs) {
     constexpr int BCM = 16 * NT;   // complex columns per block
     constexpr int RW = 2 * BCM;    // floats per staged row
     __shared__ __align__(16) float sa[LCH * RW];
     __shared__ double sco[LCH * 2];
     __shared__ cplx<float> rot[128];
-    const int nrb = (d.npairs + 127) / 128;
+    // ring-pair blocks rb0 .. rb0 + nrbs - 1 of 128 pairs each
     const int nchunk = (ncol + BCM - 1) / BCM;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int rb = lb % nrb;
-    const int chunk = (lb / nrb) % nchunk;
-    const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
+    const int rb = rb0 + (int)(lb % nrbs);
+    const int chunk = (lb / nrbs) % nchunk;
+    const int m = m_of_q(lb / (nrbs * nchunk), d.mmax);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, li32 = lane & 31;
     const int jp = rb * 128 + wave * 32 + li32;
@@ -2898,7 +2901,7 @@ size_t sht_phase_bytes(const ShtPlan& p, int nmap) {
 // Column-chunk size of the synthesis kernels: the smallest instantiated chunk covering
 // the batch (a single map uses BC = 1 instead of wasting the padding columns).
 static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
-                                hipStream_t st) {
+                                hipStream_t st, int rb0 = 0, int rb1 = -1) {
     // One tile of 16 complex columns per block.  Sharing the recursion over up to 5 tiles
     // (NT = 5, 418 registers, one wave per SIMD) measured slower at Nside 512 / 80 maps
     // (3.69 ms against 3.57 ms), and so did v_mfma_f32_32x32x1f32 with one ring per lane
@@ -2907,9 +2910,10 @@ static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     ShtDev<float> d = dev_view<float>(p);
     constexpr int NT = 1;
     const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * NT - 1) / (16 * NT);
-    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    if (rb1 < 0) rb1 = nrb;  // ring-pair blocks [rb0, rb1) of 128 pairs
+    const unsigned nb = (unsigned)((rb1 - rb0) * nchunk * (p.mmax + 1));
     hipLaunchKernelGGL(k_leg_synth_s0_mfma<NT>, dim3(nb), dim3(256), 0, st, d, ncol, ld,
-                       (const cplx<float>*)alm_t, (cplx<float>*)phase);
+                       (const cplx<float>*)alm_t, (cplx<float>*)phase, rb0, rb1 - rb0);
     PS_LAUNCH_CHECK();
     return 0;
 }
@@ -3012,8 +3016,46 @@ static int legendre_adjoint(const ShtPlan& p, int ncol, const void* phase, void*
     return launch_adj<double>(p, ncol, phase, alm, alm_stride, scale, part, st, shape);
 }
 
+// Matrix-core spin-0 synthesis with the Bluestein ring FFTs overlapped: with Nside a power of
+// two every Bluestein ring is a polar ring (nphi = 4 i, i < Nside), whose phases come from ring
+// pairs below Nside - 1 only.  The Legendre stage runs in two launches, the ring-pair blocks
+// holding the polar pairs first; the Bluestein rings start on the plan's side stream as soon as
+// those are done and run beside the second launch (the matrix-core kernel is bound by its VALU
+// stream, the ring FFTs by LDS latency), and the power-of-two rings follow on `st`, which then
+// waits for the side stream.  The two launches write disjoint phase entries.  Returns 1 when
+// the plan or batch does not qualify (nothing launched).
+static int synthesis_overlapped(const ShtPlan& p, int ncol, const void* alm_t, int ld, void* map, long map_stride,
+                                void* phase, hipStream_t st) {
+    const int nrb = (p.npairs + 127) / 128;
+    const int rbA = (p.nside - 1 + 127) / 128;  // blocks holding ring pairs 0 .. Nside - 2
+    const bool pow2_nside = (p.nside & (p.nside - 1)) == 0;
+    if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb ||
+        shape_of(p, ncol, SHT_STAGE_SYNTH) != 0)
+        return 1;
+    if (!p.side) {
+        PS_CHECK_HIP(hipStreamCreateWithFlags(&p.side, hipStreamNonBlocking));
+        PS_CHECK_HIP(hipEventCreateWithFlags(&p.ev_fork, hipEventDisableTiming));
+        PS_CHECK_HIP(hipEventCreateWithFlags(&p.ev_join, hipEventDisableTiming));
+    }
+    int rc = launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, 0, rbA);
+    if (rc) return rc;
+    PS_CHECK_HIP(hipEventRecord(p.ev_fork, st));
+    if ((rc = launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, rbA, nrb))) return rc;
+    PS_CHECK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
+    rc = launch_ring_synth_list<float>(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol, phase, map,
+                                       map_stride, p.side, shape_of(p, ncol, SHT_STAGE_RING_BLUE));
+    if (rc) return rc;
+    PS_CHECK_HIP(hipEventRecord(p.ev_join, p.side));
+    rc = launch_ring_synth_list<float>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st,
+                                       shape_of(p, ncol, SHT_STAGE_RING_POW2));
+    PS_CHECK_HIP(hipStreamWaitEvent(st, p.ev_join, 0));
+    return rc;
+}
+
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,
                          void* phase, hipStream_t st) {
+    const int rc0 = synthesis_overlapped(p, nmap * p.ncomp, alm_t, ld, map, map_stride, phase, st);
+    if (rc0 <= 0) return rc0;
     int rc = sht_legendre_synthesis_lmajor(p, nmap, alm_t, ld, phase, st);
     if (rc) return rc;
     return sht_ring_synthesis(p, nmap, phase, map, map_stride, st);
