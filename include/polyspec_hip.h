@@ -63,7 +63,10 @@ int ps_sht_synthesis(const ps_sht_plan *plan, int nmaps, const void *alm, void *
                      long long work_bytes, void *stream);
 
 /* Same, with alm given l-major: alm_t[idx(l,m) * ld + col], col = map*ncomp + comp,
- * ld >= nmaps*ncomp a multiple of ps_sht_lmajor_multiple(); padding columns must be zero. */
+ * ld >= nmaps*ncomp a multiple of ps_sht_lmajor_multiple(); padding columns must be zero.
+ * FP32 spin-0 batches on the matrix-core shape (Nside a power of two) fork the Bluestein-ring
+ * FFTs onto the plan's own side stream and join it back into `stream` before returning (event
+ * waits only: the call stays asynchronous and capturable into a HIP graph). */
 int ps_sht_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_t, int ld, void *map,
                             void *work, long long work_bytes, void *stream);
 
