@@ -70,53 +70,9 @@ def spectra(nside, fwhm=30., noise=30.):
     return raw, tot, [b, b]
 
 
-def pixel_z_phi(nside):
-    """z = cos(theta) and phi of every RING pixel (vectorised HEALPix geometry)."""
-    N = nside
-    npix = 12 * N * N
-    z = np.empty(npix)
-    ph = np.empty(npix)
-    p = 0
-    for i in range(1, 4 * N):
-        ir = min(i, 4 * N - i)
-        if ir < N:
-            zz, n, p0 = 1 - ir * ir / (3. * N * N), 4 * ir, np.pi / (4. * ir)
-        else:
-            zz, n = 2. * (2 * N - ir) / (3. * N), 4 * N
-            p0 = np.pi / (4. * N) if ((ir - N) & 1) == 0 else 0.
-        if i > 2 * N:
-            zz = -zz
-        z[p:p + n] = zz
-        ph[p:p + n] = p0 + 2 * np.pi * np.arange(n) / n
-        p += n
-    return z, ph
-
-
 def synthetic_mask(nside, nholes=200, seed=1, cut_deg=20., rhole_deg=0.5, apo_deg=1.):
-    z, ph = pixel_z_phi(nside)
-    b = np.degrees(np.arcsin(np.clip(z, -1, 1)))
-    d = np.abs(b) - cut_deg
-    m = np.clip(d / apo_deg, 0, 1)
-    m = np.where(d <= 0, 0., 0.5 * (1 - np.cos(np.pi * m)))
-    rng = np.random.default_rng(seed)
-    zc = rng.uniform(-1, 1, nholes)
-    pc = rng.uniform(0, 2 * np.pi, nholes)
-    st = np.sqrt(1 - z * z)
-    rad = np.radians(rhole_deg + apo_deg)
-    negz = -z  # RING order: z decreases with the pixel index
-    for zi, pi in zip(zc, pc):
-        # only the pixels whose colatitude is within the hole's reach
-        th = np.arccos(zi)
-        lo = np.searchsorted(negz, -np.cos(max(th - rad, 0.)), side='left')
-        hi = np.searchsorted(negz, -np.cos(min(th + rad, np.pi)), side='right')
-        sl = slice(lo, hi)
-        cosd = z[sl] * zi + st[sl] * np.sqrt(1 - zi * zi) * np.cos(ph[sl] - pi)
-        near = cosd > np.cos(rad)
-        dd = np.degrees(np.arccos(np.clip(cosd[near], -1, 1))) - rhole_deg
-        taper = np.where(dd <= 0, 0., 0.5 * (1 - np.cos(np.pi * np.clip(dd / apo_deg, 0, 1))))
-        idx = np.arange(lo, hi)[near]
-        m[idx] *= taper
-    return m
+    from polyspec_amd.masks import galactic_cut_mask
+    return galactic_cut_mask(nside, cut_deg=cut_deg, nholes=nholes, hole_deg=rhole_deg, apo_deg=apo_deg, seed=seed)
 
 
 def build(nside, lmax, nr, precision, batch, nk=200):

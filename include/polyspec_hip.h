@@ -70,6 +70,13 @@ int ps_sht_synthesis(const ps_sht_plan *plan, int nmaps, const void *alm, void *
 int ps_sht_synthesis_lmajor(const ps_sht_plan *plan, int nmaps, const void *alm_t, int ld, void *map,
                             void *work, long long work_bytes, void *stream);
 
+/* Transpose ncol m-major alm rows [col][nlm] into the l-major layout alm_t[idx(l,m) * ld + col]
+ * (columns ncol..ld-1 zeroed), in the plan's precision.  No reference counterpart: the layout
+ * change that lets batched consumers (the l-major synthesis input above, ps_tau_sum_lmajor
+ * below) read whole rows of columns; it replaces the transposes of
+ * `self.base.to_lm_vec(...).T[self.nLminfilt]` (tspec_template.py:1294-1303). */
+int ps_sht_alm_to_lmajor(const ps_sht_plan *plan, int ncol, const void *alm, int ld, void *alm_t, void *stream);
+
 /* The two stages of ps_sht_synthesis_lmajor, for callers that schedule or time them
  * separately: the Legendre stage writes ring phases F_m(ring) exp(i m phi0(ring)) in
  * tiles of 16 rings, [ring / 16][mmax+1][ring % 16][nmaps*ncomp] complex
@@ -162,7 +169,9 @@ int ps_mc_accumulate(int precision, const void *P, const void *Q, int nsim, long
 int ps_pixel_products(int precision, int mode, const void *a, const void *b, const void *c, const void *d,
                       long long n, double scale, void *out, void *stream);
 /* The same over a batch: a and out are (nbatch, n), b, c, d (n) shared by every
- * stack (the weight maps of the batched CG below). */
+ * stack (the weight maps of the batched CG below); mode 4 is mode 1 with c batched like a,
+ * out[s] = a[s]*b + c[s]*d (multiplyRR_sym of each simulation's (Q_s, P_s) with the data's
+ * (P, Q), tspec_template.py:1294-1303 as called at 1665-1687). */
 int ps_pixel_products_batch(int precision, int mode, int nbatch, const void *a, const void *b, const void *c,
                             const void *d, long long n, double scale, void *out, void *stream);
 
@@ -177,6 +186,17 @@ int ps_radial_sum(int precision, const void *lm_map, int nr, int lmax, int lmin,
 int ps_tau_sum_n0(int precision, const void *A, const void *B, int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
                   const double *FLL, const double *w, int n1, double *scratch, double *out, int accumulate,
                   void *stream);
+
+/* tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646), summed over a batch of nmu (A, B) pairs, with
+ * A and B l-major: At[idx(L,M) * ld + mu * nr + j] over L = 0..Lmaxs (ld >= nmu * nr, nr <= 64):
+ *   out (+)= sum_mu tau_sum_n0(A_mu, B_mu)
+ * = pref sum_L sum_{j,k} F_L(j,k) w_j w_k sum_mu sum_M (1+[M>0]) Re(A_mu,j(L,M) conj(B_mu,k(L,M))),
+ * pref = 1/(sqrt(4 pi) sqrt(2 n1 + 1)).  Per-L Gram matrices on FP64 matrix cores (FP32 data is
+ * widened), F_L contraction fused; replaces the per-simulation tau_sum_n0 calls of the
+ * Monte-Carlo terms (tspec_template.py:1665-1687).  At == Bt is the symmetric case tau(A, A). */
+int ps_tau_sum_lmajor(int precision, const void *At, const void *Bt, long long ld, int nmu, int nr, int Lmaxs,
+                      int Lmin, int Lmax, const double *FLL, const double *w, int n1, double *scratch, double *out,
+                      int accumulate, void *stream);
 
 /* ideal_fisher.outer_product (ideal_fisher.pyx:423-459, sym = 0): Qa (4, nta, nin),
  * Qb (4, ntb, nin) complex -> fish (nta, ntb) double (added when accumulate != 0). */

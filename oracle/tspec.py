@@ -15,7 +15,10 @@ Citations:
   Fisher contribution  2590-3052 (gNL-loc 2713-2740, tauNL-loc 2842-2848)
   F_PQ / Q4            _compute_PQ_vecs 1967-1980, _convolve_F 2155-2162, _F_PQ 2287-2306,
                        _Q4_contributions 2361-2436, _get_Q4_perms 2521-2554
-  weighting of Q       _weight_Q_maps 2556-2572
+  weighting of Q       _weight_Q_maps 2556-2572 (unit mask: harmonic S^-1; otherwise
+                       S^-1 of the masked map mask * to_map(beam Q), 2567-2569)
+  masked Fisher GRFs   compute_fisher_contribution 2631-2644: 'Sinv' S^-1(mask to_map(beam a)),
+                       'Ainv' applyAinv(a) at base lmax then the l filter
   applySinv (ideal)    weighting.py:186-214
 """
 import numpy as np
@@ -27,8 +30,13 @@ FOURPI15 = (4. * np.pi) ** 1.5
 
 
 class OracleTSpec:
-    def __init__(self, nside, lmin, lmax, Lmin, Lmax, Cl_tot, beam, pol, templates, plLX0, qlXs, FLL, r_weights):
+    def __init__(self, nside, lmin, lmax, Lmin, Lmax, Cl_tot, beam, pol, templates, plLX0, qlXs, FLL, r_weights,
+                 mask=None):
         self.nside, self.lmin, self.lmax, self.Lmin, self.Lmax = nside, lmin, lmax, Lmin, Lmax
+        # mask None (or all ones): the unit-mask branches of the reference (ones_mask,
+        # tspec_template.py:153); otherwise the masked-sky branches
+        m = None if mask is None else np.asarray(mask, dtype=np.float64)
+        self.mask = None if m is None or (np.std(m) < 1e-12 and abs(np.mean(m) - 1) < 1e-12) else m
         self.pol = pol
         self.templates = list(templates)
         self.Npix = 12 * nside ** 2
@@ -179,10 +187,18 @@ class OracleTSpec:
 
         def compute_Q4(weighting):
             lf = self.l_arr <= self.lmax
-            if weighting == 'Sinv':
-                U = [self.applySinv(self.beam[:, self.l_arr[lf]] * a, 'harmonic')[:, self.lminfilt] for a in a_maps]
-            else:
-                U = [self.applyAinv(a)[:, self.lminfilt] for a in a_maps]
+            if self.mask is None:  # a_maps at lmax (generate_data(lmax=self.lmax), 2618-2619)
+                if weighting == 'Sinv':
+                    U = [self.applySinv(self.beam[:, self.l_arr[lf]] * a, 'harmonic')[:, self.lminfilt]
+                         for a in a_maps]
+                else:
+                    U = [self.applyAinv(a)[:, self.lminfilt] for a in a_maps]
+            else:  # a_maps at base lmax (2620-2622)
+                if weighting == 'Sinv':
+                    U = [self.applySinv(self.mask * to_map(self.beam[:, self.l_arr] * a, self.nside, self.base_lmax),
+                                        'map')[:, self.lminfilt] for a in a_maps]
+                else:
+                    U = [self.applyAinv(a, lmax=self.base_lmax)[:, self.lfilt] for a in a_maps]
             Pm = [self.weighted_maps(np.ascontiguousarray(u), self.plLX0, 1. / np.sqrt(4 * np.pi)) for u in U]
             Qm = [self.weighted_maps(np.ascontiguousarray(u), self.qlXs, 1.) for u in U]
             Qs = np.zeros((4, nT, nslot, nl), dtype=np.complex128)
@@ -225,7 +241,11 @@ class OracleTSpec:
                     if weighting == 'Ainv':
                         full = np.zeros((nslot, len(self.lminfilt)), dtype=np.complex128)
                         full[:, self.lminfilt] = self.beam[:, self.ls] * Qs[q, ii]
-                        Qs[q, ii] = self.applySinv(full, 'harmonic')[:, self.lminfilt]
+                        if self.mask is None:
+                            Qs[q, ii] = self.applySinv(full, 'harmonic')[:, self.lminfilt]
+                        else:
+                            Qs[q, ii] = self.applySinv(self.mask * to_map(full, self.nside, self.lmax),
+                                                       'map')[:, self.lminfilt]
                     else:
                         Qs[q, ii] = self.m_weight * Qs[q, ii]
             return Qs.reshape(4, nT, -1)

@@ -44,6 +44,8 @@ SIGNATURES = {
     "ps_pixel_products": (_I, [_I, _I, _P, _P, _P, _P, _LL, _D, _P, _P]),
     "ps_radial_sum": (_I, [_I, _P, _I, _I, _I, _P, _I, _I, _P, _D, _I, _P, _P]),
     "ps_tau_sum_n0": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
+    "ps_tau_sum_lmajor": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
+    "ps_sht_alm_to_lmajor": (_I, [_P, _I, _P, _I, _P, _P]),
     "ps_outer_product": (_I, [_I, _P, _P, _I, _I, _LL, _P, _P, _I, _P]),
     "ps_convolve_F_lmajor": (_I, [_I, _P, _I, _I, _I, _I, _P, _P, _I, _P, _P]),
     "ps_lmat_apply": (_I, [_I, _I, _I, _P, _P, _P, _P, _LL, _P, _LL, _D, _I, _P]),
@@ -258,6 +260,19 @@ class ShtPlan:
         ws = workspace(wb, alm_t.device)
         check(load().ps_sht_synthesis_lmajor(self._h, int(nmaps), ptr(alm_t), ld, ptr(out), ptr(ws), wb,
                                              stream_handle()))
+        return out
+
+    def to_lmajor(self, alm, ld=None, out=None):
+        """alm (ncol, nlm) or (nmaps, ncomp, nlm) m-major -> (nlm, ld) l-major, columns >= ncol zero."""
+        require_device(alm, self.cdtype, "alm")
+        assert alm.shape[-1] == self.nlm, "Wrong alm shape"
+        ncol = alm.numel() // self.nlm
+        ld = ncol if ld is None else int(ld)
+        if out is None:
+            out = torch.empty((self.nlm, ld), dtype=self.cdtype, device=alm.device)
+        require_device(out, self.cdtype, "alm_t")
+        assert out.shape == (self.nlm, ld) and ld >= ncol, "out must be (nlm, ld >= ncol)"
+        check(load().ps_sht_alm_to_lmajor(self._h, ncol, ptr(alm), ld, ptr(out), stream_handle()))
         return out
 
     def phase_buffer(self, nmaps):

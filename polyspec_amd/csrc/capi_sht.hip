@@ -113,6 +113,12 @@ int ps_sht_synthesis(const ps_sht_plan* plan, int nmaps, const void* alm, void* 
     return sht_synthesis_lmajor(p, nmaps, alm_t, ld, map, p.npix, phase, st);
 }
 
+int ps_sht_alm_to_lmajor(const ps_sht_plan* plan, int ncol, const void* alm, int ld, void* alm_t, void* stream) {
+    PS_REQUIRE(plan && alm && alm_t, "ps_sht_alm_to_lmajor: NULL argument");
+    PS_REQUIRE(ncol >= 1 && ld >= ncol && ld <= 65535 * 16, "ps_sht_alm_to_lmajor: need 1 <= ncol <= ld <= 1048560");
+    return sht_alm_to_lmajor(plan->p, ncol, alm, plan->p.nlm, ld, alm_t, (hipStream_t)stream);
+}
+
 int ps_sht_synthesis_lmajor(const ps_sht_plan* plan, int nmaps, const void* alm_t, int ld, void* map,
                             void* work, long long work_bytes, void* stream) {
     PS_REQUIRE(plan && alm_t && map && work, "ps_sht_synthesis_lmajor: NULL argument");

@@ -265,18 +265,22 @@ __global__ void k_mc_accumulate(const T* __restrict__ P, const T* __restrict__ Q
 // mode 1: out = a*b + c*d             (multiplyRR_sym, ideal_fisher.pyx:112-120)
 // mode 2: out = a*b*c                 (tauNL_utils.multiply, tauNL_utils.pyx:103-111)
 // mode 3: out = a*(2 b c + a d)       (tauNL_utils.multiply_asym, tauNL_utils.pyx:116-124)
+// mode 4: out = a*b + c*d with c batched like a (multiplyRR_sym of each simulation's maps
+//         (a, c) with the shared data maps (b, d): Q_s P + P_s Q, tspec_template.py:1294-1303)
 template <typename T>
 __global__ void k_products(int mode, const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
                            const T* __restrict__ d, long n, T scale, T* __restrict__ out) {
     // blockIdx.y: map stack of a batch (a and out n elements apart; b, c, d shared)
     a += (long)blockIdx.y * n;
     out += (long)blockIdx.y * n;
+    if (mode == 4) c += (long)blockIdx.y * n;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         T v;
         switch (mode) {
             case 0: v = a[i] * b[i]; break;
-            case 1: v = a[i] * b[i] + c[i] * d[i]; break;
+            case 1:
+            case 4: v = a[i] * b[i] + c[i] * d[i]; break;
             case 2: v = a[i] * b[i] * c[i]; break;
             default: v = a[i] * (T(2) * b[i] * c[i] + a[i] * d[i]); break;
         }
@@ -350,6 +354,97 @@ __global__ __launch_bounds__(256) void k_tau_gram(const cplx<T>* __restrict__ A,
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc * pref;
+}
+
+// The same sum on FP64 matrix cores, with A and B l-major and batched over simulations:
+// At[idx(L,M) * ld + mu * nr + j] (idx the m-major index over 0..Lmaxs; mu < nmu, j < nr).
+//   tau = pref sum_L sum_{j,k} F_L(j,k) w_j w_k G_L(j,k),
+//   G_L(j,k) = sum_mu sum_M (1+[M>0]) Re(A_mu,j(L,M) conj(B_mu,k(L,M))),
+// i.e. the sum over mu of tau_sum_n0 of each (A_mu, B_mu) pair.  G_L = X^T Y is a real GEMM
+// with K = (M, mu, re/im): v_mfma_f64_16x16x4f64 takes two (M, mu) entries per instruction
+// (lanes 16q + c: q = 0,1 the re, im parts of the first entry, q = 2,3 of the second; c the
+// column within a 16-wide tile of j or k).  Lane layout (verified on gfx950 for the Fisher
+// derivative GEMM): A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k, D[i][j] at lane
+// j + 16 (i % 4), register i / 4.  One block per L, heaviest (largest L) first; the four waves
+// take alternate entry pairs and contract their partial G_L with F_L in the epilogue (the
+// contraction is linear in G), so no Gram matrix is stored.  NT = ceil(nr / 16) tiles per side.
+typedef double ps_dbl4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int NT, bool SAME>
+__global__ __launch_bounds__(256) void k_tau_gram_mfma(const cplx<T>* __restrict__ At, const cplx<T>* __restrict__ Bt,
+                                                       long ld, int nmu, int nr, int Lmaxs, int Lmax,
+                                                       const double* __restrict__ FLL, const double* __restrict__ w,
+                                                       double pref, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int L = Lmax - (int)blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, q = lane >> 4, half = q >> 1;
+    const bool im = (q & 1) != 0;
+    ps_dbl4 acc[NT][NT];
+#pragma unroll
+    for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+        for (int tk = 0; tk < NT; tk++) acc[ti][tk] = ps_dbl4{0.0, 0.0, 0.0, 0.0};
+    const long nent = (long)(L + 1) * nmu;  // entries (M, mu), mu fastest
+    const long rowL = L;                     // idx(L, M) = M (2 Lmaxs + 1 - M) / 2 + L
+    auto fetch = [&](long e, double* a, double* b) {
+        if (e < nent) {
+            const int M = (int)(e / nmu), mu = (int)(e - (long)M * nmu);
+            const long row = (long)M * (2 * Lmaxs + 1 - M) / 2 + rowL;
+            const cplx<T>* ra = At + row * ld + (long)mu * nr;
+            const cplx<T>* rb = Bt + row * ld + (long)mu * nr;
+            const double wm = M == 0 ? 1.0 : 2.0;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int j = 16 * t + c;
+                double va = 0.0, vb = 0.0;
+                if (j < nr) {
+                    const cplx<T> x = ra[j];
+                    va = (double)(im ? x.im : x.re);
+                    if constexpr (SAME) {
+                        vb = va;
+                    } else {
+                        const cplx<T> y = rb[j];
+                        vb = (double)(im ? y.im : y.re);
+                    }
+                }
+                a[t] = va;
+                b[t] = wm * vb;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; t++) a[t] = b[t] = 0.0;
+        }
+    };
+    double a0[NT], b0[NT], a1[NT], b1[NT];
+    long e = 2 * wave + half;
+    fetch(e, a0, b0);
+    for (; e - half < nent; e += 8) {
+        fetch(e + 8, a1, b1);  // next pair of entries in flight while this one multiplies
+#pragma unroll
+        for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+            for (int tk = 0; tk < NT; tk++)
+                acc[ti][tk] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[ti], b0[tk], acc[ti][tk], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            a0[t] = a1[t];
+            b0[t] = b1[t];
+        }
+    }
+    // epilogue: register v of tile (ti, tk) holds G_L(16 ti + 4 v + q, 16 tk + c)
+    double sum = 0.0;
+#pragma unroll
+    for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+        for (int tk = 0; tk < NT; tk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = 16 * ti + 4 * v + q, k = 16 * tk + c;
+                if (j < nr && k < nr) sum += FLL[((long)L * nr + j) * nr + k] * w[j] * w[k] * acc[ti][tk][v];
+            }
+    sum = block_sum(sum, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum * pref;
 }
 
 // ---------------------------------------------------------- outer product
@@ -475,6 +570,30 @@ static int reduce_launch_fused(const void* P, const void* Q, const void* U, cons
     return 0;
 }
 
+template <typename T, int NT>
+static void launch_tau_mfma(const void* At, const void* Bt, long ld, int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
+                            const double* FLL, const double* w, double pref, double* scratch, hipStream_t st) {
+    const dim3 grid((unsigned)(Lmax - Lmin + 1));
+    if (At == Bt)
+        hipLaunchKernelGGL((k_tau_gram_mfma<T, NT, true>), grid, dim3(256), 0, st, (const cplx<T>*)At,
+                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, pref, scratch);
+    else
+        hipLaunchKernelGGL((k_tau_gram_mfma<T, NT, false>), grid, dim3(256), 0, st, (const cplx<T>*)At,
+                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, pref, scratch);
+}
+
+template <typename T>
+static void launch_tau_mfma_nt(int nt, const void* At, const void* Bt, long ld, int nmu, int nr, int Lmaxs, int Lmin,
+                               int Lmax, const double* FLL, const double* w, double pref, double* scratch,
+                               hipStream_t st) {
+    switch (nt) {
+        case 1: launch_tau_mfma<T, 1>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
+        case 2: launch_tau_mfma<T, 2>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
+        case 3: launch_tau_mfma<T, 3>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
+        default: launch_tau_mfma<T, 4>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
+    }
+}
+
 extern "C" {
 
 long long ps_reduce_scratch_bytes(void) { return (long long)RED_BLOCKS * 64 * sizeof(double); }
@@ -554,9 +673,9 @@ int ps_mc_accumulate(int precision, const void* P, const void* Q, int nsim, long
 
 static int products_launch(int precision, int mode, int nbatch, const void* a, const void* b, const void* c,
                            const void* d, long n, double scale, void* out, hipStream_t st) {
-    PS_REQUIRE(mode >= 0 && mode <= 3, "ps_pixel_products: mode must be 0..3");
+    PS_REQUIRE(mode >= 0 && mode <= 4, "ps_pixel_products: mode must be 0..4");
     PS_REQUIRE(a && b && out, "ps_pixel_products: NULL argument");
-    PS_REQUIRE(!(mode == 1 || mode == 3) || (c && d), "ps_pixel_products: c and d required for this mode");
+    PS_REQUIRE(!(mode == 1 || mode == 3 || mode == 4) || (c && d), "ps_pixel_products: c and d required for this mode");
     PS_REQUIRE(!(mode == 2) || c, "ps_pixel_products: c required for mode 2");
     PS_REQUIRE(nbatch >= 1 && nbatch <= 65535, "ps_pixel_products: nbatch must be 1..65535");
     const dim3 grid((unsigned)std::min<long>((n + 255) / 256, nbatch == 1 ? 16384 : 2048), (unsigned)nbatch);
@@ -618,6 +737,29 @@ int ps_tau_sum_n0(int precision, const void* A, const void* B, int nmu, int nr, 
                            (const cplx<double>*)B, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch);
     else
         PS_REQUIRE(false, "ps_tau_sum_n0: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)scratch, nL, out, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_tau_sum_lmajor(int precision, const void* At, const void* Bt, long long ld, int nmu, int nr, int Lmaxs,
+                      int Lmin, int Lmax, const double* FLL, const double* w, int n1, double* scratch, double* out,
+                      int accumulate, void* stream) {
+    PS_REQUIRE(At && Bt && FLL && w && scratch && out, "ps_tau_sum_lmajor: NULL argument");
+    PS_REQUIRE(nr >= 1 && nr <= 64, "ps_tau_sum_lmajor: need 1 <= nr <= 64");
+    PS_REQUIRE(nmu >= 1 && ld >= (long long)nmu * nr, "ps_tau_sum_lmajor: need ld >= nmu * nr");
+    PS_REQUIRE(Lmin >= 0 && Lmax >= Lmin && Lmax <= Lmaxs, "ps_tau_sum_lmajor: need 0 <= Lmin <= Lmax <= Lmaxs");
+    PS_REQUIRE(Lmax - Lmin + 1 <= RED_BLOCKS * 64, "ps_tau_sum_lmajor: L range too large for the scratch");
+    hipStream_t st = (hipStream_t)stream;
+    const int nL = Lmax - Lmin + 1, nt = (nr + 15) / 16;
+    const double pref = 1.0 / (sqrt(4.0 * M_PI) * sqrt(2.0 * n1 + 1.0));
+    if (precision == 32)
+        launch_tau_mfma_nt<float>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st);
+    else if (precision == 64)
+        launch_tau_mfma_nt<double>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st);
+    else
+        PS_REQUIRE(false, "ps_tau_sum_lmajor: precision must be 32 or 64");
     PS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)scratch, nL, out, accumulate);
     PS_LAUNCH_CHECK();

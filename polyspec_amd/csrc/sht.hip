@@ -3029,8 +3029,9 @@ static int synthesis_overlapped(const ShtPlan& p, int ncol, const void* alm_t, i
     const int nrb = (p.npairs + 127) / 128;
     const int rbA = (p.nside - 1 + 127) / 128;  // blocks holding ring pairs 0 .. Nside - 2
     const bool pow2_nside = (p.nside & (p.nside - 1)) == 0;
-    if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb ||
-        shape_of(p, ncol, SHT_STAGE_SYNTH) != 0)
+    int shape = shape_of(p, ncol, SHT_STAGE_SYNTH);
+    if (shape < 0) shape = synth_shape_rule<float>(p, ncol);  // no tuned or set shape: the built-in rule
+    if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb || shape != 0)
         return 1;
     if (!p.side) {
         PS_CHECK_HIP(hipStreamCreateWithFlags(&p.side, hipStreamNonBlocking));
@@ -3042,14 +3043,18 @@ static int synthesis_overlapped(const ShtPlan& p, int ncol, const void* alm_t, i
     PS_CHECK_HIP(hipEventRecord(p.ev_fork, st));
     if ((rc = launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, rbA, nrb))) return rc;
     PS_CHECK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
+    // once the side stream is forked it is always joined back into `st`, whatever fails in
+    // between: no unjoined fork under capture, no side-stream work unordered with `st`
     rc = launch_ring_synth_list<float>(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol, phase, map,
                                        map_stride, p.side, shape_of(p, ncol, SHT_STAGE_RING_BLUE));
+    const hipError_t ej = hipEventRecord(p.ev_join, p.side);
+    if (rc == 0 && ej == hipSuccess)
+        rc = launch_ring_synth_list<float>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st,
+                                           shape_of(p, ncol, SHT_STAGE_RING_POW2));
+    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, p.ev_join, 0) : ej;
     if (rc) return rc;
-    PS_CHECK_HIP(hipEventRecord(p.ev_join, p.side));
-    rc = launch_ring_synth_list<float>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, map, map_stride, st,
-                                       shape_of(p, ncol, SHT_STAGE_RING_POW2));
-    PS_CHECK_HIP(hipStreamWaitEvent(st, p.ev_join, 0));
-    return rc;
+    PS_CHECK_HIP(ew);
+    return 0;
 }
 
 int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* map, long map_stride,

@@ -43,6 +43,30 @@ def _chk(*ts):
             assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "device, contiguous tensors only"
 
 
+def _nlm(lmax):
+    return (int(lmax) + 1) * (int(lmax) + 2) // 2
+
+
+def _same(ref, *ts, what="operands"):
+    """Every tensor of ts has the shape and dtype of ref (the kernels read them with ref's extent)."""
+    for t in ts:
+        if t is not None:
+            assert t.shape == ref.shape and t.dtype == ref.dtype, \
+                "%s must share shape and dtype: %s %s vs %s %s" % (what, tuple(ref.shape), ref.dtype, tuple(t.shape),
+                                                                  t.dtype)
+
+
+def _stack(t, nr, npix, what):
+    """t holds an (nr, npix) stack (any leading shape of nr * npix values)."""
+    assert t.numel() == nr * npix, "%s must be an (%d, %d) stack, got %s" % (what, nr, npix, tuple(t.shape))
+
+
+def _table(f, lmax, nr, what):
+    """Radial table (>= lmax+1, nslot, nr)."""
+    assert f.dim() == 3 and f.shape[0] > lmax and f.shape[2] == nr, \
+        "%s must be (>= %d, nslot, %d), got %s" % (what, lmax + 1, nr, tuple(f.shape))
+
+
 def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
     """apply_fl_weights (tauNL_utils.pyx:80-98) into l-major synthesis input columns.
 
@@ -51,6 +75,10 @@ def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
     _chk(f, h, out_t)
     nr = f.shape[2]
     npol = min(f.shape[1], h.shape[0], 2) if npol is None else npol
+    _table(f, lmax, nr, "f")
+    assert h.dim() == 2 and h.shape[0] >= npol and h.shape[1] == _nlm(lmax), "h must be (npol, nlm(lmax))"
+    assert out_t.dim() == 2 and out_t.shape[0] >= _nlm(lmax) and out_t.shape[1] >= col_offset + nr, \
+        "out_t must be (>= nlm(lmax), >= col_offset + nr)"
     check(load().ps_weight_alm_lmajor(_prec(h), ptr(f), f.shape[1], nr, ptr(h), npol, int(lmax), int(lmin),
                                       float(weight), out_t.shape[1], int(col_offset), ptr(out_t), stream_handle()))
     return out_t
@@ -65,6 +93,9 @@ def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npo
     assert min(fP.shape[0], fQ.shape[0]) > lmax, "the tables must cover l <= lmax"
     nr = fP.shape[2]
     npol = min(fP.shape[1], h.shape[0], 2) if npol is None else npol
+    assert h.dim() == 2 and h.shape[0] >= npol and h.shape[1] == _nlm(lmax), "h must be (npol, nlm(lmax))"
+    assert out_t.dim() == 2 and out_t.shape[0] >= _nlm(lmax) and out_t.shape[1] >= col_offset + 2 * nr, \
+        "out_t must be (>= nlm(lmax), >= col_offset + 2 nr)"
     check(load().ps_weight_alm_pq_lmajor(_prec(h), ptr(fP), ptr(fQ), fP.shape[1], nr, ptr(h), npol, int(lmax),
                                          int(lmin), float(wP), float(wQ), out_t.shape[1], int(col_offset),
                                          ptr(out_t), stream_handle()))
@@ -74,11 +105,18 @@ def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npo
 def gnl_loc_fused_sum(P, Q, w, c4=1.0, U=None, V=None, c2=0.0, out=None, accumulate=False):
     """c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] as a device scalar (U, V FP64)."""
     _chk(P, Q, U, V)
+    assert (U is None) == (V is None), "U and V go together"
     assert U is None or (U.dtype == torch.float64 and V.dtype == torch.float64), "U, V must be float64"
+    assert P.dim() >= 2, "P must be an (nr, npix) stack"
+    nr, npix = P.shape[-2], P.shape[-1]
+    _same(P, Q, what="P and Q")
+    if U is not None:
+        _stack(U, nr, npix, "U")
+        _stack(V, nr, npix, "V")
     w = _dev_f64(w, P.device)
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
     if out is None:
         out = torch.zeros((), dtype=torch.float64, device=P.device)
-    nr, npix = P.shape[-2], P.shape[-1]
     check(load().ps_gnl_loc_fused_sum(_prec(P), ptr(P), ptr(Q), ptr(U) if U is not None else None,
                                       ptr(V) if V is not None else None, ptr(w), nr, npix, float(c4), float(c2),
                                       ptr(reduce_scratch(P.device)), ptr(out), int(accumulate), stream_handle()))
@@ -93,7 +131,10 @@ def gnl_loc_sum(w, P, Q):
 def gnl_loc_disc_sum(w, P1, P2, Q1, Q2, out=None, accumulate=False):
     """tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285)."""
     _chk(P1, P2, Q1, Q2)
+    assert P1.dim() >= 2, "P1 must be an (nr, npix) stack"
+    _same(P1, P2, Q1, Q2, what="P1, P2, Q1 and Q2")
     w = _dev_f64(w, P1.device)
+    assert w.numel() == P1.shape[-2], "w must have nr = %d entries" % P1.shape[-2]
     if out is None:
         out = torch.zeros((), dtype=torch.float64, device=P1.device)
     check(load().ps_gnl_loc_disc_sum(_prec(P1), ptr(P1), ptr(P2), ptr(Q1), ptr(Q2), ptr(w), P1.shape[-2],
@@ -106,15 +147,26 @@ def mc_accumulate(P, Q, U, V):
     """U += sum_s P_s Q_s, V += sum_s P_s^2 over the leading (simulation) axis of P, Q (U, V FP64)."""
     _chk(P, Q, U, V)
     assert U.dtype == torch.float64 and V.dtype == torch.float64, "Monte-Carlo sums U, V must be float64"
+    _same(P, Q, what="P and Q")
+    assert U.shape == V.shape, "U and V must share their shape"
     n = U.numel()
     nsim = P.numel() // n
+    assert n > 0 and nsim * n == P.numel() and tuple(P.shape[P.dim() - U.dim():]) == tuple(U.shape), \
+        "P must stack whole copies of U's shape: %s vs %s" % (tuple(P.shape), tuple(U.shape))
     check(load().ps_mc_accumulate(_prec(P), ptr(P), ptr(Q), nsim, n, ptr(U), ptr(V), stream_handle()))
 
 
 def _products(mode, a, b, c=None, d=None, scale=1.0, out=None):
     _chk(a, b, c, d)
+    assert mode in (0, 2) or d is not None, "this product needs four operands"
+    assert mode in (0,) or c is not None, "this product needs a third operand"
+    for t in (b, c, d):  # the kernel reads a.numel() values of every operand
+        assert t is None or (t.numel() == a.numel() and t.dtype == a.dtype), \
+            "map operands must share size and dtype: %s %s vs %s %s" % (tuple(a.shape), a.dtype, tuple(t.shape), t.dtype)
     if out is None:
         out = torch.empty_like(a)
+    _chk(out)
+    assert out.numel() == a.numel() and out.dtype == a.dtype, "out must match the operands"
     check(load().ps_pixel_products(_prec(a), mode, ptr(a), ptr(b), ptr(c) if c is not None else None,
                                    ptr(d) if d is not None else None, a.numel(), float(scale), ptr(out),
                                    stream_handle()))
@@ -144,9 +196,16 @@ def multiply_asym(a1A, a1B, a3A, a3B, scale=1.0, out=None):
 def radial_sum(lm_map, w, f, lmax, lmin, npol, out=None, scale=1.0, accumulate=False):
     """tauNL_utils.radial_sum (tauNL_utils.pyx:220-235): (nr, nlm) -> (npol, nlm)."""
     _chk(lm_map, f)
+    assert lm_map.dim() == 2 and lm_map.shape[1] == _nlm(lmax), "lm_map must be (nr, nlm(lmax))"
+    nr = lm_map.shape[0]
+    _table(f, lmax, nr, "f")
+    assert npol <= f.shape[1], "npol exceeds the table's slots"
     w = _dev_f64(w, lm_map.device)
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
     if out is None:
         out = torch.zeros((npol, lm_map.shape[-1]), dtype=lm_map.dtype, device=lm_map.device)
+    _chk(out)
+    assert out.dtype == lm_map.dtype and out.numel() == npol * lm_map.shape[1], "out must be (npol, nlm(lmax))"
     check(load().ps_radial_sum(_prec(lm_map), ptr(lm_map), lm_map.shape[0], int(lmax), int(lmin), ptr(f),
                                f.shape[1], int(npol), ptr(w), float(scale), int(accumulate), ptr(out),
                                stream_handle()))
@@ -156,8 +215,14 @@ def radial_sum(lm_map, w, f, lmax, lmin, npol, out=None, scale=1.0, accumulate=F
 def tau_sum_n0(A, B, FLL, w, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False):
     """tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646); A, B: (nmu, nr, nlm(Lmaxs))."""
     _chk(A, B)
+    _same(A, B, what="A and B")
+    assert A.dim() == 3 and A.shape[2] == _nlm(Lmaxs), "A, B must be (nmu, nr, nlm(Lmaxs))"
+    nr = A.shape[1]
     FLL = _dev_f64(FLL, A.device)
     w = _dev_f64(w, A.device)
+    assert FLL.numel() == (int(Lmaxs) + 1) * nr * nr, "FLL must be (Lmaxs+1, nr, nr)"
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
+    assert 0 <= Lmin and Lmax <= Lmaxs, "L range outside the A maps"
     if out is None:
         out = torch.zeros((), dtype=torch.float64, device=A.device)
     check(load().ps_tau_sum_n0(_prec(A), ptr(A), ptr(B), A.shape[0], A.shape[1], int(Lmaxs), int(Lmin), int(Lmax),
@@ -166,11 +231,35 @@ def tau_sum_n0(A, B, FLL, w, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False
     return out
 
 
+def tau_sum_lmajor(At, Bt, FLL, w, nr, nmu, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False):
+    """Sum over mu < nmu of tau_sum_n0 (tauNL_utils.pyx:604-646) of the pairs (A_mu, B_mu), with
+    A, B l-major (nlm(Lmaxs), ld), column mu * nr + j; Bt may be At (tau(A, A))."""
+    _chk(At, Bt)
+    assert At.dim() == 2 and At.shape[0] == _nlm(Lmaxs), "At must be (nlm(Lmaxs), ld)"
+    _same(At, Bt, what="At and Bt")
+    assert At.shape[1] >= nmu * nr and 1 <= nr <= 64, "ld must cover nmu * nr columns (nr <= 64)"
+    FLL = _dev_f64(FLL, At.device)
+    w = _dev_f64(w, At.device)
+    assert FLL.numel() == (int(Lmaxs) + 1) * nr * nr, "FLL must be (Lmaxs+1, nr, nr)"
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
+    assert 0 <= Lmin and Lmax <= Lmaxs, "L range outside the A maps"
+    if out is None:
+        out = torch.zeros((), dtype=torch.float64, device=At.device)
+    check(load().ps_tau_sum_lmajor(_prec(At), ptr(At), ptr(Bt), At.shape[1], int(nmu), int(nr), int(Lmaxs),
+                                   int(Lmin), int(Lmax), ptr(FLL), ptr(w), int(n1), ptr(reduce_scratch(At.device)),
+                                   ptr(out), int(accumulate), stream_handle()))
+    return out
+
+
 def outer_product(Qa, Qb, out=None, accumulate=False):
     """ideal_fisher.outer_product (ideal_fisher.pyx:423-459), sym=False."""
     _chk(Qa, Qb)
+    assert Qa.dim() == 3 and Qb.dim() == 3 and Qa.shape[0] == 4 and Qb.shape[0] == 4, "Q must be (4, nt, nin)"
+    assert Qa.shape[2] == Qb.shape[2] and Qa.dtype == Qb.dtype, "Qa and Qb must share nin and dtype"
     if out is None:
         out = torch.zeros((Qa.shape[1], Qb.shape[1]), dtype=torch.float64, device=Qa.device)
+    _chk(out)
+    assert out.dtype == torch.float64 and out.shape == (Qa.shape[1], Qb.shape[1]), "fish must be (nta, ntb) float64"
     check(load().ps_outer_product(_prec(Qa), ptr(Qa), ptr(Qb), Qa.shape[1], Qb.shape[1], Qa.shape[2],
                                   ptr(reduce_scratch(Qa.device)), ptr(out), int(accumulate), stream_handle()))
     return out
@@ -179,8 +268,14 @@ def outer_product(Qa, Qb, out=None, accumulate=False):
 def convolve_F_lmajor(PQ, FLL, w, Lmaxs, Lmin, Lmax, out_t):
     """convolve_F_n000 (tauNL_utils.pyx:966-992) into l-major synthesis columns 0..nr-1 of out_t."""
     _chk(PQ, out_t)
+    assert PQ.dim() == 2 and PQ.shape[1] == _nlm(Lmaxs), "PQ must be (nr, nlm(Lmaxs))"
+    nr = PQ.shape[0]
     FLL = _dev_f64(FLL, PQ.device)
     w = _dev_f64(w, PQ.device)
+    assert FLL.numel() == (int(Lmaxs) + 1) * nr * nr, "FLL must be (Lmaxs+1, nr, nr)"
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
+    assert out_t.dim() == 2 and out_t.shape[0] >= _nlm(Lmaxs) and out_t.shape[1] >= nr and \
+        out_t.dtype == PQ.dtype, "out_t must be (>= nlm(Lmaxs), >= nr) of PQ's dtype"
     check(load().ps_convolve_F_lmajor(_prec(PQ), ptr(PQ), PQ.shape[0], int(Lmaxs), int(Lmin), int(Lmax), ptr(FLL),
                                       ptr(w), out_t.shape[1], ptr(out_t), stream_handle()))
     return out_t
@@ -245,6 +340,22 @@ def multiplyRR_batch(a, b, out=None):
     return out
 
 
+def multiplyRR_sym_batch(a, b, c, d, out=None):
+    """a[s] b + c[s] d for every stack s of a and c (B, ...) with the shared b, d (a[0].shape)."""
+    _chk(a, b, c, d)
+    nb = a.shape[0]
+    assert c.shape == a.shape and c.dtype == a.dtype, "a and c must share shape and dtype"
+    for t in (b, d):
+        assert t.numel() * nb == a.numel() and t.dtype == a.dtype, "b, d must have the shape of one stack of a"
+    if out is None:
+        out = torch.empty_like(a)
+    _chk(out)
+    assert out.shape == a.shape and out.dtype == a.dtype, "out must match a"
+    check(load().ps_pixel_products_batch(_prec(a), 4, nb, ptr(a), ptr(b), ptr(c), ptr(d), b.numel(), 1.0, ptr(out),
+                                         stream_handle()))
+    return out
+
+
 def lmat_apply_batch(x, lmax, mat=None, pre=None, post=None, out=None, alpha=1.0, accumulate=False):
     """lmat_apply on every vector of x (B, nf, nlm)."""
     _chk(x)
@@ -295,6 +406,6 @@ def cg_update_p_batch(num, den, active, z, p):
 
 
 __all__ = ["lmat_apply", "mdot", "cg_update_xr", "cg_update_p", "lmat_apply_batch", "mdot_batch",
-           "cg_update_xr_batch", "cg_update_p_batch", "multiplyRR_batch", "convolve_F_lmajor", "weight_alm_lmajor", "weight_alm_pq_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
-           "multiplyRR", "multiplyRR_sym", "multiply", "multiply_asym", "radial_sum", "tau_sum_n0",
+           "cg_update_xr_batch", "cg_update_p_batch", "multiplyRR_batch", "multiplyRR_sym_batch", "convolve_F_lmajor", "weight_alm_lmajor", "weight_alm_pq_lmajor", "gnl_loc_fused_sum", "gnl_loc_sum", "gnl_loc_disc_sum", "mc_accumulate",
+           "multiplyRR", "multiplyRR_sym", "multiply", "multiply_asym", "radial_sum", "tau_sum_n0", "tau_sum_lmajor",
            "outer_product", "_abi"]

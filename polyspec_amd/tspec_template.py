@@ -41,7 +41,7 @@ class TSpecTemplate:
     def __init__(self, base, mask, applySinv, templates, k_arr, Tl_arr, lmin, lmax, Lmin=None, Lmax=None,
                  Lmin_lens=None, Lmax_lens=None, ns=0.96, As=2.1e-9, k_pivot=0.05, r_values=[], r_weights={},
                  rtau_values=[], rtau_weights={}, C_phi=[], C_lens_weight={}, K_coll=0.1, k_coll=0.1, r_star=None,
-                 r_hor=None, batch_sims=True, sim_batch=None, capture_graph=False):
+                 r_hor=None, batch_sims=True, sim_batch=None, capture_graph=False, sim_memory_gb=None):
         for t in templates:
             if t not in SUPPORTED:
                 raise NotImplementedError("Template %s is not provided by the hip backend (only %s)" % (t, SUPPORTED))
@@ -56,10 +56,15 @@ class TSpecTemplate:
         self.ns, self.As, self.k_pivot = ns, As, k_pivot
         self.batch_sims = batch_sims
         self.sim_batch = sim_batch
+        self.sim_batch_tau = None  # simulations per A_sd batch of the tauNL-loc t2 term (None: 256 // N_r)
         # capture_graph: with batch_sims, the device part of each numerator (S^-1, radial
         # filtering, syntheses, fused sums) is captured once into a HIP graph and replayed;
         # applySinv must then be capturable (no host synchronisation, e.g. applySinv_ideal)
         self.capture_graph = capture_graph
+        # sim_memory_gb: device memory for the filtered simulation maps that the batched tauNL-loc
+        # terms revisit on every numerator (default: half of the free memory when they are made);
+        # simulations beyond it keep only their S^-1-filtered harmonics and are re-synthesised
+        self.sim_memory_gb = sim_memory_gb
         self._graph = None
         self._sims_version = 0
         self.kernel_events = None  # set to a list to time the Legendre stage of every filter call
@@ -126,8 +131,7 @@ class TSpecTemplate:
     def _prepare_templates(self):
         """p_l^X(r), q_l^X(r) and F_L(r, r') (tspec_template.py:520-708, n = 0 terms)."""
         t0 = time.time()
-        if hasattr(self, 't0_num'):
-            delattr(self, 't0_num')
+        self._drop_sim_state()
         nslot = 1 + 2 * self.pol
         jl = jl_kr(np.arange(self.lmin, self.lmax + 1), self.k_arr, self.r_arr)  # shared by p and q
         self.plLXs = p_integral_n0(self.k_arr, self.Pzeta(self.k_arr), self.Tl_arr, self.r_arr, self.lmin, self.lmax,
@@ -138,6 +142,18 @@ class TSpecTemplate:
             self.FLLs = F_integral_n0(self.r_arr, self.Lmin, self.Lmax, self.ns, self.As, self.k_pivot)[None]
         self._upload_tables()
         self.timers['precomputation'] += time.time() - t0
+
+    def _drop_sim_state(self):
+        """Forget everything derived from the previous radial nodes: the processed simulations
+        (their P/Q stacks and Monte-Carlo sums are per node), the cached t0 and a captured
+        numerator graph (it holds the old tables and workspaces).  Simulations must be
+        generated or loaded again after the nodes change."""
+        for name in ('t0_num', 'preload', 'proc_maps', 'U_d', 'V_d', 'SA_t', 'load_sim_data', 'N_it', '_sim_P', '_sim_Q',
+                     '_sim_h'):
+            if hasattr(self, name):
+                delattr(self, name)
+        self._sims_version += 1
+        self._graph = None
 
     def _upload_tables(self):
         dev, rdt = self.base.device, self.base.rdtype
@@ -443,8 +459,10 @@ class TSpecTemplate:
         return r * scale
 
     # ------------------------------------------------------------------ sims
-    def _process_pairs(self, pairs, input_type):
-        """applySinv + filters for a list of (alpha, beta) inputs -> list of (maps_a, maps_b)."""
+    def _process_pairs(self, pairs, input_type, with_stack=False):
+        """applySinv + filters for a list of (alpha, beta) inputs -> list of (maps_a, maps_b); with
+        with_stack also the (2 n, 2, N_r, Npix) stack [a0, b0, a1, b1, ...] and the filtered
+        harmonics h of each map."""
         t0 = time.time()
         hs = []
         for a, b in pairs:
@@ -452,7 +470,8 @@ class TSpecTemplate:
             hs.append(self._h_dev(self.applySinv(b, input_type=input_type, lmax=self.lmax)))
         self.timers['Sinv'] += time.time() - t0
         maps = self._filter(hs)
-        return [(maps[2 * i], maps[2 * i + 1]) for i in range(len(pairs))]
+        out = [(maps[2 * i], maps[2 * i + 1]) for i in range(len(pairs))]
+        return (out, maps, hs) if with_stack else out
 
     def _sim_pair(self, ii, Cl_input):
         if self.ones_mask:
@@ -465,6 +484,12 @@ class TSpecTemplate:
         b = self._masked(self.base.generate_data(int(2e5) + ii, Cl_input=Cl_input, deconvolve_beam=False))
         return (a, b), 'map'
 
+    def _beam_times(self, a):
+        """beam_lm * a for alm at base lmax (host or device arrays)."""
+        if isinstance(a, torch.Tensor):
+            return a * torch.from_numpy(np.asarray(self.base.beam_lm)).to(a.device)
+        return np.asarray(self.base.beam_lm) * a
+
     def _masked(self, m):
         if isinstance(m, torch.Tensor):
             if not hasattr(self, 'mask_d'):
@@ -474,43 +499,128 @@ class TSpecTemplate:
         return self.mask * m
 
     def _collapse_init(self):
-        dev, rdt = self.base.device, self.base.rdtype
+        dev = self.base.device
         self.t0_num = np.zeros(len(self.templates))
-        if 'gNL-loc' in self.templates:
-            # Monte-Carlo sums in FP64 whatever the working precision
-            self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
-            self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
-        if 'tauNL-loc' in self.templates:
-            self.SA_d = torch.zeros((1, self.N_r, self.base.plan(self.nLmax, 0).nlm), dtype=self.base.cdtype,
-                                    device=dev)
+        # Monte-Carlo sums U = sum_s P_s Q_s, V = sum_s P_s^2 in FP64 whatever the working precision
+        # (gNL-loc uses both; tauNL-loc uses U: sum_s A_ss = Y^dag[sum_s Q_s P_s] = Y^dag[U])
+        self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
+        self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
+        self._t0_acc = torch.zeros(len(self.templates), dtype=torch.float64, device=dev)
 
-    def _collapse_add(self, maps_a, maps_b):
-        """Add one simulation pair to the collapsed Monte-Carlo sums and to t0."""
-        Pa, Qa = maps_a[0], maps_a[1]
-        Pb, Qb = maps_b[0], maps_b[1]
+    def _A_batch_lmajor(self, prods):
+        """Y^dag at nLmax of a (n, N_r, Npix) stack of product maps -> l-major (nlm(nLmax), n N_r),
+        column s N_r + r (the layout of ps_tau_sum_lmajor)."""
+        t0 = time.time()
+        n = prods.shape[0]
+        plan = self.base.plan(self.nLmax, 0)
+        A = plan.adjoint(prods.view(n * self.N_r, 1, -1), np.pi / (3. * self.base.Nside ** 2))
+        self.base.n_SHTs_forward += n * self.N_r
+        At = plan.to_lmajor(A)
+        self.timers['tauNL_products'] += time.time() - t0
+        return At
+
+    def _tau_lmajor(self, At, Bt, nmu, t, out, scale=1.):
+        """out += scale * sum_mu tau_sum_n0(A_mu, B_mu) (device scalar)."""
+        t0 = time.time()
+        if scale == 1.:
+            K.tau_sum_lmajor(At, Bt, self.FLL_d, self.w_d[t], self.N_r, nmu, self.nLmax, self.Lmin, self.Lmax,
+                             out=out, accumulate=True)
+        else:
+            r = K.tau_sum_lmajor(At, Bt, self.FLL_d, self.w_d[t], self.N_r, nmu, self.nLmax, self.Lmin, self.Lmax)
+            out.add_(r, alpha=scale)
+        self.timers['tauNL_summation'] += time.time() - t0
+
+    def _collapse_add(self, stack):
+        """Add a chunk of simulation pairs, stack (2 n, 2, N_r, Npix) = [a0, b0, a1, b1, ...], to
+        the Monte-Carlo sums and to t0 (device accumulators; no host synchronisation)."""
+        n = stack.shape[0] // 2
+        P = stack[:, 0].contiguous()
+        Q = stack[:, 1].contiguous()
+        K.mc_accumulate(P, Q, self.U_d, self.V_d)
         for ii, t in enumerate(self.templates):
             if t == 'gNL-loc':
-                K.mc_accumulate(Pa, Qa, self.U_d, self.V_d)
-                K.mc_accumulate(Pb, Qb, self.U_d, self.V_d)
-                d = K.gnl_loc_disc_sum(self.w_d[t], Pa, Pb, Qa, Qb)
-                self.t0_num[ii] += 54. / 100. * float(d) / self.N_it * self.base.A_pix * FOURPI15
+                for i in range(n):
+                    K.gnl_loc_disc_sum(self.w_d[t], P[2 * i], P[2 * i + 1], Q[2 * i], Q[2 * i + 1],
+                                       out=self._t0_acc[ii:ii + 1].view(()), accumulate=True)
             if t == 'tauNL-loc':
-                A_aa = self._A_maps(Pa, Qa)
-                A_bb = self._A_maps(Pb, Qb)
-                self.SA_d += A_aa
-                self.SA_d += A_bb
-                A_ab = self._A_maps(Pa, Qa, Pb, Qb)
-                tau = 4 * float(self._tau_sum(A_aa, A_bb, t)) + 2 * float(self._tau_sum(A_ab, A_ab, t))
-                self.t0_num[ii] += 1. / 24. * tau * 3 / self.N_it * FOURPI15
+                # t0 (tspec_template.py:1680-1687): sum over pairs of 4 tau(A_aa, A_bb) + 2 tau(A_ab, A_ab),
+                # A_xy = Y^dag[Q_x P_y (+ Q_y P_x)], as l-major batches over the pairs
+                Pa, Pb = P[0::2].contiguous(), P[1::2].contiguous()
+                Qa, Qb = Q[0::2].contiguous(), Q[1::2].contiguous()
+                A_aa = self._A_batch_lmajor(K.multiplyRR(Qa, Pa))
+                A_bb = self._A_batch_lmajor(K.multiplyRR(Qb, Pb))
+                tau = self._t0_acc[ii:ii + 1].view(())
+                self._tau_lmajor(A_aa, A_bb, n, t, tau, scale=4.)
+                del A_aa, A_bb
+                A_ab = self._A_batch_lmajor(K.multiplyRR_sym(Qb, Pa, Qa, Pb))
+                self._tau_lmajor(A_ab, A_ab, n, t, tau, scale=2.)
+
+    def _collapse_finish(self):
+        pref = np.zeros(len(self.templates))
+        for ii, t in enumerate(self.templates):
+            if t == 'gNL-loc':
+                pref[ii] = 54. / 100. / self.N_it * self.base.A_pix * FOURPI15
+            if t == 'tauNL-loc':
+                pref[ii] = 1. / 24. * 3 / self.N_it * FOURPI15
+        self.t0_num = self._t0_acc.cpu().numpy() * pref
+        del self._t0_acc
+        if 'tauNL-loc' in self.templates:
+            # sum_s A_ss = Y^dag[U], l-major (one column per radial node)
+            self.SA_t = self._A_batch_lmajor(self.U_d.to(self.base.rdtype)[None])
+        if 'gNL-loc' not in self.templates:
+            del self.U_d, self.V_d
+
+    # ------------------------------------------- simulation store (tauNL-loc)
+    def _sim_store_init(self, n_sims):
+        """Room for the filtered P, Q stacks of the simulations that the per-numerator tauNL-loc
+        A_sd terms revisit: as many as fit in the memory budget stay resident on the device, the
+        others keep their filtered harmonics h (npol x nlm(lmax)) and are re-synthesised."""
+        per = 2 * self.N_r * self.base.Npix * torch.tensor([], dtype=self.base.rdtype).element_size()
+        if self.sim_memory_gb is None:
+            free, _ = torch.cuda.mem_get_info(self.base.device)
+            budget = 0.5 * free
+        else:
+            budget = self.sim_memory_gb * 2 ** 30
+        n_res = int(min(n_sims, budget // per))
+        dev, rdt = self.base.device, self.base.rdtype
+        self._sim_P = torch.empty((n_res, self.N_r, self.base.Npix), dtype=rdt, device=dev)
+        self._sim_Q = torch.empty((n_res, self.N_r, self.base.Npix), dtype=rdt, device=dev)
+        self._sim_h = [None] * n_sims
+        self._sim_count = 0
+
+    def _sim_store_add(self, stack, hs):
+        n_res = self._sim_P.shape[0]
+        for i in range(stack.shape[0]):
+            s = self._sim_count
+            if s < n_res:
+                self._sim_P[s].copy_(stack[i, 0])
+                self._sim_Q[s].copy_(stack[i, 1])
+            else:
+                self._sim_h[s] = hs[i]
+            self._sim_count += 1
+
+    def _sim_chunks(self, nb):
+        """(P, Q) stacks (n, N_r, Npix) of consecutive simulations, n <= nb, resident or re-made."""
+        n_res, n_all = self._sim_P.shape[0], len(self._sim_h)
+        for i0 in range(0, n_res, nb):
+            i1 = min(n_res, i0 + nb)
+            yield self._sim_P[i0:i1], self._sim_Q[i0:i1]
+        for i0 in range(n_res, n_all, nb):
+            i1 = min(n_all, i0 + nb)
+            maps = self._filter(self._sim_h[i0:i1])
+            yield maps[:, 0].contiguous(), maps[:, 1].contiguous()
 
     def _ingest_sims(self, get_pair, N_pairs, preload, verb):
         self._sims_version += 1  # invalidates a captured numerator graph
         self.N_it = N_pairs
         print("Using %d pairs of Monte Carlo simulations" % self.N_it)
-        keep_maps = preload and (not self.batch_sims or 'tauNL-loc' in self.templates)
+        keep_maps = preload and not self.batch_sims
+        store_tau = self.batch_sims and 'tauNL-loc' in self.templates
         self.preload = preload
         if self.batch_sims:
             self._collapse_init()
+        if store_tau:
+            self._sim_store_init(2 * N_pairs)
         if keep_maps:
             self.proc_maps = []
         nb = self.sim_batch or max(1, min(4, int(64 // max(1, self.N_r // 4))))
@@ -523,12 +633,17 @@ class TSpecTemplate:
             for ii in idx:
                 pr, itype = get_pair(ii)
                 pairs.append(pr)
-            procs = self._process_pairs(pairs, itype)
-            for ma, mb in procs:
-                if self.batch_sims:
-                    self._collapse_add(ma, mb)
-                if keep_maps:
+            procs, stack, hs = self._process_pairs(pairs, itype, with_stack=True)
+            if self.batch_sims:
+                self._collapse_add(stack)
+            if store_tau:
+                self._sim_store_add(stack, hs)
+            if keep_maps:
+                for ma, mb in procs:
                     self.proc_maps.append((ma.clone(), mb.clone()))
+            del procs, stack, hs
+        if self.batch_sims:
+            self._collapse_finish()
         if not preload:
             def load_sim_data(ii):
                 pr, itype = get_pair(ii)
@@ -570,6 +685,9 @@ class TSpecTemplate:
         collapsed = include_disconnected_term and self.batch_sims
         graph_ok = (self.capture_graph and collapsed and input_type == 'map' and isinstance(data, torch.Tensor)
                     and data.is_cuda)
+        if graph_ok and not getattr(self.applySinv, 'capturable', False):
+            raise Exception("capture_graph needs an applySinv without host synchronisation "
+                            "(Weightings.applySinv_ideal); %r is not marked capturable" % (self.applySinv,))
         if graph_ok:
             t4_d, P, Q, A_dd = self._numerator_graph(data)
         else:
@@ -584,13 +702,14 @@ class TSpecTemplate:
                 if 'tauNL-loc' in self.templates:
                     ii = self.templates.index('tauNL-loc')
                     t = 'tauNL-loc'
+                    # sum_s 2 tau(A_sd, A_sd), A_sd = Y^dag[Q_s P + Q P_s], in batches of simulations
                     acc = torch.zeros((), dtype=torch.float64, device=self.base.device)
-                    for isim in range(self.N_it):
-                        ma, mb = self._sim_maps(isim)
-                        for ms in (ma, mb):
-                            A_sd = self._A_maps(ms[0], ms[1], P, Q)
-                            acc = acc + 2 * self._tau_sum(A_sd, A_sd, t)
-                    t2[ii] = -1. / 24. * float(acc) * 6 / self.N_it / 2. * FOURPI15
+                    nb = self.sim_batch_tau or max(1, 256 // self.N_r)
+                    for Ps, Qs in self._sim_chunks(nb):
+                        A_sd = self._A_batch_lmajor(K.multiplyRR_sym_batch(Qs, P, Ps, Q))
+                        self._tau_lmajor(A_sd, A_sd, Ps.shape[0], t, acc)
+                        del A_sd
+                    t2[ii] = -1. / 24. * 2. * float(acc) * 6 / self.N_it / 2. * FOURPI15
             else:
                 for isim in range(self.N_it):
                     ma, mb = self._sim_maps(isim)
@@ -646,11 +765,15 @@ class TSpecTemplate:
                     t4.append(K.gnl_loc_fused_sum(P, Q, self.w_d[t], c4))
                 self.timers['gNL_summation'] += time.time() - t1
             if t == 'tauNL-loc':
-                A_dd = self._A_maps(P, Q)
-                v = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)
                 if collapsed:
-                    v = v + self._tau_sum(self.SA_d, A_dd, t, scale=-1. / 24. * 6. / self.N_it / 2. *
-                                          FOURPI15 * 4.)
+                    Ad_t = self._A_batch_lmajor(K.multiplyRR(Q, P)[None])
+                    v = torch.zeros((), dtype=torch.float64, device=self.base.device)
+                    self._tau_lmajor(Ad_t, Ad_t, 1, t, v, scale=1. / 24. * 12. * FOURPI15)
+                    # 4 tau(sum_s A_ss, A_dd) of the t2 term, linear in the simulations
+                    self._tau_lmajor(self.SA_t, Ad_t, 1, t, v, scale=-1. / 24. * 6. / self.N_it / 2. * FOURPI15 * 4.)
+                else:
+                    A_dd = self._A_maps(P, Q)
+                    v = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)
                 t4.append(v)
         return torch.stack([x.reshape(()) for x in t4]), P, Q, A_dd
 
@@ -658,7 +781,7 @@ class TSpecTemplate:
         """_numerator_device through a captured HIP graph: captured on the first call for this
         data shape and set of simulations (after one eager warm-up run that sizes every cached
         workspace), then replayed with the data copied into the graph's input buffer."""
-        key = (tuple(data.shape), data.dtype, self._sims_version)
+        key = (tuple(data.shape), data.dtype, self._sims_version, id(self.applySinv))
         if self._graph is None or self._graph_key != key:
             self._graph = None
             self._numerator_device(data, 'map', True)  # warm-up: plans, workspaces, scratch
@@ -741,7 +864,7 @@ class TSpecTemplate:
                     if self.ones_mask:
                         r = self.applySinv(full, input_type='harmonic', lmax=self.lmax)
                     else:
-                        r = self.applySinv(self.mask * self.base.to_map(full, lmax=self.lmax), lmax=self.lmax)
+                        r = self.applySinv(self._masked(self.base.to_map(full, lmax=self.lmax)), lmax=self.lmax)
                     self.timers['Sinv'] += time.time() - t0
                     if not isinstance(r, torch.Tensor):
                         r = torch.from_numpy(np.asarray(r))
@@ -781,7 +904,7 @@ class TSpecTemplate:
                     U = [self.applySinv(self.base.beam_lm[:, lf] * a, input_type='harmonic', lmax=self.lmax)
                          for a in a_maps]
                 else:
-                    U = [self.applySinv(self.mask * self.base.to_map(self.base.beam_lm * a), lmax=self.lmax)
+                    U = [self.applySinv(self._masked(self.base.to_map(self._beam_times(a))), lmax=self.lmax)
                          for a in a_maps]
                 self.timers['Sinv'] += time.time() - t0
             else:

@@ -456,3 +456,8 @@ class BatchCGD:
 
     def get_ivar_lm(self):
         return K.lmat_apply_batch((self.x + self.x0).contiguous(), self.L, mat=self.weightings.inv_Cl_l_d)
+
+
+# applySinv_ideal issues no host synchronisation, so a numerator that calls it can be captured
+# into a HIP graph (TSpecTemplate(capture_graph=True) checks this flag)
+Weightings.applySinv_ideal.capturable = True

@@ -142,3 +142,56 @@ def test_pixel_kernels(gpu, prec):
     a3, b3 = a[:, :2999].copy(), b[:, :2999].copy()
     s = float(K.gnl_loc_sum(w, T(a3, rdt), T(b3, rdt)))
     assert abs(s - oracle.gnl_loc_sum(w, a3, b3)) < tol * np.sum(w[:, None] * np.abs(a3) ** 3 * np.abs(b3))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("nr,nmu,same", [(3, 1, True), (16, 3, False), (17, 2, True), (40, 2, False), (64, 1, True)])
+def test_tau_sum_lmajor_matches_oracle(gpu, prec, nr, nmu, same):
+    """ps_tau_sum_lmajor (per-L Gram products on FP64 matrix cores, l-major A/B batched over mu)
+    against the sum over mu of the oracle's tau_sum_n0 (tauNL_utils.pyx:604-646); the columns
+    beyond nmu * nr hold NaN, which must never be read."""
+    from polyspec_amd import kernels as K
+    from polyspec_amd._abi import ShtPlan
+    print(f"tau lmajor prec={prec} nr={nr} nmu={nmu} same={same}", flush=True)
+    rng = np.random.default_rng(nr * 10 + nmu)
+    Lmaxs, Lmin, Lmax = 23, 2, 21
+    Ls, Ms = oracle.getlm(Lmaxs)
+    nlm = len(Ls)
+    cdt = torch.complex128 if prec == 64 else torch.complex64
+    A = rng.standard_normal((nmu, nr, nlm)) + 1j * rng.standard_normal((nmu, nr, nlm))
+    B = A.copy() if same else rng.standard_normal((nmu, nr, nlm)) + 1j * rng.standard_normal((nmu, nr, nlm))
+    F = rng.standard_normal((Lmaxs + 1, nr, nr))
+    F = F + F.transpose(0, 2, 1)
+    w = rng.random(nr)
+    w[nr // 2] = 0.
+    # oracle: its (nmu, nLM, nr) layout over Lmin..Lmax rows, i.e. L >= Lmin of 0..Lmax
+    keep = Ls <= Lmax
+    sel = Ls[keep] >= Lmin
+    ref = 0.
+    for mu in range(nmu):
+        a = A[mu][:, keep][:, sel].T[None].copy()
+        b = B[mu][:, keep][:, sel].T[None].copy()
+        ref += oracle.tau_sum_n0(a, b, F, w, Ls[keep][sel].astype(np.int32), Ms[keep][sel].astype(np.int32), Lmin,
+                                 Lmax)
+    ld = nmu * nr + 5
+    plan = ShtPlan(8, Lmaxs, 0, prec)
+    def lmaj(X):
+        t = torch.full((nlm, ld), float('nan'), dtype=cdt, device=gpu)
+        t[:, :nmu * nr] = T(X.reshape(nmu * nr, nlm).T, cdt)
+        return t
+    At = lmaj(A)
+    Bt = At if same else lmaj(B)
+    got = float(K.tau_sum_lmajor(At, Bt, F, w, nr, nmu, Lmaxs, Lmin, Lmax))
+    assert abs(got - ref) < (1e-12 if prec == 64 else 2e-6) * np.sum(np.abs(F).max() * np.abs(A).max() ** 2 * nr ** 2), \
+        (got, ref)
+    assert abs(got - ref) < (1e-11 if prec == 64 else 1e-5) * abs(ref) + 1e-300, (got, ref)
+    # the transpose into that layout, and the batched symmetric product of simulation maps
+    Am = T(A.reshape(nmu * nr, nlm), cdt)
+    lt = plan.to_lmajor(Am, ld=ld)
+    assert torch.equal(lt[:, :nmu * nr], At[:, :nmu * nr]) and torch.all(lt[:, nmu * nr:] == 0)
+    rdt = torch.float64 if prec == 64 else torch.float32
+    Qs, Ps = T(rng.standard_normal((3, nr, 50)), rdt), T(rng.standard_normal((3, nr, 50)), rdt)
+    P, Q = T(rng.standard_normal((nr, 50)), rdt), T(rng.standard_normal((nr, 50)), rdt)
+    prod = K.multiplyRR_sym_batch(Qs, P, Ps, Q)
+    want = Qs.cpu().numpy() * P.cpu().numpy() + Ps.cpu().numpy() * Q.cpu().numpy()
+    assert rel(prod, want) < (1e-15 if prec == 64 else 1e-6)

@@ -84,22 +84,6 @@ def test_outer_product(rng):
     assert np.allclose(oracle.outer_product(Qa, Qb), ref, rtol=1e-13)
 
 
-def test_F_closed_form_vs_quadrature():
-    """F_L(r,r') = (2/pi) int k^2 P_zeta(k) j_L(kr) j_L(kr') dk (k_integrals.pyx:453-494)."""
-    from polyspec_amd.k_integrals import F_integral_n0, Pzeta
-    r = np.array([13600., 14200.])
-    As, ns, kp = 2.1e-9, 0.96, 0.05
-    F = F_integral_n0(r, 1, 6, ns, As, kp)
-    k = np.concatenate([np.linspace(1e-7, 0.02, 400001)[:-1], np.linspace(0.02, 2.0, 1200001)])
-    for L in (2, 5):
-        for i, j in ((0, 0), (0, 1)):
-            f = 2 / np.pi * k ** 2 * Pzeta(k, As, ns, kp) * spherical_jn(L, k * r[i]) * spherical_jn(L, k * r[j])
-            q = np.trapezoid(f, k)
-            assert abs(F[L, i, j] - q) < 2e-4 * abs(F[L, i, j]), (L, i, j, F[L, i, j], q)
-    assert np.allclose(F[:, 0, 1], F[:, 1, 0])
-    assert np.all(F[0] == 0)
-
-
 def test_q_integral_matches_definition():
     from polyspec_amd.k_integrals import q_integral
     k = np.geomspace(1e-4, 0.2, 80)

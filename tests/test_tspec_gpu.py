@@ -19,20 +19,20 @@ from estimator_setup import grf_alm, radial_nodes, spectra, transfer
 pytestmark = pytest.mark.gpu
 
 
-def build(nside, lmax, pol, nr, templates, precision, batch_sims, nk=60):
+def build(nside, lmax, pol, nr, templates, precision, batch_sims, nk=60, mask=None):
     from polyspec_amd.base import PolySpec
     from polyspec_amd.tspec_template import TSpecTemplate
     from polyspec_amd.weighting import Weightings
     raw, tot, beam = spectra(nside, pol)
     base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=precision)
-    mask = np.ones(12 * nside ** 2)
+    mask = np.ones(12 * nside ** 2) if mask is None else mask
     W = Weightings(base, mask, dict(raw))
     k, Tl = transfer(lmax, pol, nk)
     r, w = radial_nodes(nr)
     tm = TSpecTemplate(base, mask, W.applySinv_ideal, templates, k, Tl, 2, lmax, r_values=r,
                        r_weights={t: w for t in templates}, batch_sims=batch_sims)
     orc = OracleTSpec(nside, 2, lmax, tm.Lmin, tm.Lmax, tot, base.beam, pol, templates, tm.plLXs[0], tm.qlXs,
-                      tm.FLLs[0] if 'tauNL-loc' in templates else None, tm.r_weights)
+                      tm.FLLs[0] if 'tauNL-loc' in templates else None, tm.r_weights, mask=mask)
     return base, tm, orc, tot
 
 
@@ -142,3 +142,40 @@ def test_graph_numerator_matches_eager(gpu, pol, precision):
             got = tms[1].Tl_numerator(d)
             assert np.allclose(got, want, rtol=1e-12, atol=0), (rep, got, want)
     assert tms[1]._graph is not None
+
+
+@pytest.mark.parametrize("pol,precision", [(False, 64), (True, 64), (True, 32)])
+def test_masked_sky_numerator_and_fisher_match_oracle(gpu, pol, precision):
+    """The masked-sky branches: masked data and simulation maps through the map-space S^-1, and
+    the Fisher contribution's 'Sinv' GRFs S^-1(mask to_map(beam a)), 'Ainv' GRFs applyAinv at
+    base lmax, and the 'Ainv' weighting of the Q derivatives S^-1(mask to_map(beam Q))
+    (tspec_template.py:2556-2572, 2616-2644), with a cut-plus-holes apodised mask."""
+    from polyspec_amd.masks import galactic_cut_mask
+    nside, lmax, nr = 16, 24, 3
+    templates = ['gNL-loc', 'tauNL-loc']
+    print(f"masked numerator+fisher nside={nside} lmax={lmax} pol={pol} prec={precision}", flush=True)
+    mask = galactic_cut_mask(nside, cut_deg=20., nholes=12, hole_deg=6., apo_deg=8., seed=5)
+    base, tm, orc, tot = build(nside, lmax, pol, nr, templates, precision, True, mask=mask)
+    assert not tm.ones_mask and orc.mask is not None
+    rng = np.random.default_rng(99)
+    L = 3 * nside - 1
+    masked = lambda: mask * oracle.to_map(grf_alm(rng, tot, L, pol), nside, L)
+    data = masked()
+    sims = [(masked(), masked()) for _ in range(2)]
+    tm.load_sims(lambda ii: sims[ii], len(sims), input_type='map')
+    got = tm.Tl_numerator(data)
+    ref = orc.numerator(data, sims, input_type='map', sim_type='map')
+    ref4 = orc.numerator(data, sims, input_type='map', sim_type='map', include_disconnected_term=False)
+    tol = 1e-8 if precision == 64 else 2e-4
+    for i, t in enumerate(templates):
+        assert abs(got[i] - ref[i]) <= tol * abs(ref4[i]), (t, got[i], ref[i], ref4[i])
+    # Fisher GRFs at base lmax with the beam deconvolved (generate_data(deconvolve_beam=True))
+    ls = oracle.getlm(L)[0]
+    a_maps = []
+    for _ in range(2):
+        a = grf_alm(rng, tot, L, pol)
+        b = base.beam[:, ls]
+        a_maps.append(np.where(b != 0, a / np.where(b != 0, b, 1), 0))
+    got = tm.fisher_contribution_from_grfs(a_maps)
+    ref = orc.fisher_from_grfs(a_maps)
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (got, ref)
