@@ -198,6 +198,15 @@ int ps_tau_sum_lmajor(int precision, const void *At, const void *Bt, long long l
                       int Lmin, int Lmax, const double *FLL, const double *w, int n1, double *scratch, double *out,
                       int accumulate, void *stream);
 
+/* ps_tau_sum_lmajor without a scratch argument and without allocating: the per-L partial sums
+ * go to a device array built into the library (8192 doubles per GPU, held until the process
+ * ends), so Lmax - Lmin + 1 <= 8192 (larger ranges are rejected with an error).  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Calls that overlap on one GPU share that array and race. */
+int ps_tau_sum_lmajor_noalloc(int precision, const void *At, const void *Bt, long long ld, int nmu, int nr,
+                              int Lmaxs, int Lmin, int Lmax, const double *FLL, const double *w, int n1, double *out,
+                              int accumulate, void *stream);
+
 /* ideal_fisher.outer_product (ideal_fisher.pyx:423-459, sym = 0): Qa (4, nta, nin),
  * Qb (4, ntb, nin) complex -> fish (nta, ntb) double (added when accumulate != 0). */
 int ps_outer_product(int precision, const void *Qa, const void *Qb, int nta, int ntb, long long nin, double *scratch,

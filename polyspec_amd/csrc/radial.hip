@@ -9,6 +9,7 @@
 
 #include "polyspec_hip.h"
 #include <algorithm>
+#include <mutex>
 
 #include "ps_common.h"
 
@@ -594,6 +595,33 @@ static void launch_tau_mfma_nt(int nt, const void* At, const void* Bt, long ld, 
     }
 }
 
+// ------------------------------------------------- allocation-free tau sum
+// Partial sums (one per L) of ps_tau_sum_lmajor_noalloc live in this file-scope device array
+// instead of a caller workspace.  Host code never passes the symbol itself: its device address,
+// which differs per GPU, comes from hipGetSymbolAddress on the first call on each device (that
+// device current) and is cached per device.  Every call on a GPU shares that GPU's copy, so
+// calls that overlap on one GPU (concurrent streams or threads) race.
+static constexpr int TAU_NOALLOC_MAX_L = 8192;  // 64 KiB of partials per device
+static __device__ __align__(256) double scratch[TAU_NOALLOC_MAX_L];
+static constexpr int TAU_NOALLOC_MAX_DEVICES = 64;
+static double* g_scratch_ptr[TAU_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int tau_noalloc_scratch(double** out) {
+    int dev = -1;
+    PS_CHECK_HIP(hipGetDevice(&dev));
+    PS_REQUIRE(dev >= 0 && dev < TAU_NOALLOC_MAX_DEVICES, "ps_tau_sum_lmajor_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        PS_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PS_REQUIRE(ptr != nullptr, "ps_tau_sum_lmajor_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 extern "C" {
 
 long long ps_reduce_scratch_bytes(void) { return (long long)RED_BLOCKS * 64 * sizeof(double); }
@@ -764,6 +792,19 @@ int ps_tau_sum_lmajor(int precision, const void* At, const void* Bt, long long l
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)scratch, nL, out, accumulate);
     PS_LAUNCH_CHECK();
     return 0;
+}
+
+int ps_tau_sum_lmajor_noalloc(int precision, const void* At, const void* Bt, long long ld, int nmu, int nr, int Lmaxs,
+                              int Lmin, int Lmax, const double* FLL, const double* w, int n1, double* out,
+                              int accumulate, void* stream) {
+    PS_REQUIRE(Lmin >= 0 && Lmax >= Lmin, "ps_tau_sum_lmajor_noalloc: need 0 <= Lmin <= Lmax");
+    PS_REQUIRE(Lmax - Lmin + 1 <= TAU_NOALLOC_MAX_L,
+               "ps_tau_sum_lmajor_noalloc: Lmax - Lmin + 1 exceeds the 8192 partial sums of the built-in scratch");
+    double* part = nullptr;
+    const int rc = tau_noalloc_scratch(&part);
+    if (rc) return rc;
+    return ps_tau_sum_lmajor(precision, At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, n1, part, out, accumulate,
+                             stream);
 }
 
 int ps_outer_product(int precision, const void* Qa, const void* Qb, int nta, int ntb, long long nin, double* scratch,
