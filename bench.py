@@ -93,13 +93,58 @@ def build(nside, lmax, nr, precision, batch, nk=200):
     return base, raw, tot, (k, np.asarray([Tl, Tl]), r, w)
 
 
-def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None, graph=False):
+def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None, graph=False, templates=('gNL-loc',)):
     from polyspec_amd.tspec_template import TSpecTemplate
     from polyspec_amd.weighting import Weightings
     k, Tl, r, w = tables
     W = Weightings(base, mask, dict(raw))
-    return TSpecTemplate(base, mask, W.applySinv_ideal, ['gNL-loc'], k, Tl, 2, lmax, r_values=r,
-                         r_weights={'gNL-loc': w}, batch_sims=batch, capture_graph=graph), W
+    return TSpecTemplate(base, mask, W.applySinv_ideal, list(templates), k, Tl, 2, lmax, r_values=r,
+                         r_weights={t: w for t in templates}, batch_sims=batch, capture_graph=graph), W
+
+
+def bench_tau(base, raw, mask, tables, lmax, nsims, datas, steps=3, fisher_steps=2):
+    """tauNL-loc at the same workload: numerators/s of the batched FP32 path (t2's A_sd terms over
+    every simulation, in batches of 256 // N_r, simulation maps resident up to half the free
+    memory, re-synthesised beyond), then gNL-loc + tauNL-loc Fisher contributions/s (masked sky)."""
+    import torch
+    tm, _ = make_template(base, raw, mask, tables, lmax, True, templates=('tauNL-loc',))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    tm.generate_sims(nsims, preload=True)
+    torch.cuda.synchronize()
+    t_mc = time.time() - t0
+    n_res = int(tm._sim_P.shape[0])
+    tm.Tl_numerator(datas[0])  # warm-up: plans and workspaces of the nLmax adjoints
+    torch.cuda.synchronize()
+    vals = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        vals.append(float(tm.Tl_numerator(datas[(i + 1) % len(datas)])[0]))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out = {"numerators_per_s": 1. / dt, "s_per_numerator": dt, "mc_precompute_s": t_mc,
+           "sim_maps_resident": n_res, "sim_maps_resynthesised": 2 * nsims - n_res,
+           "adjoint_maps_per_numerator": 2 * nsims * tm.N_r + tm.N_r, "numerators": vals,
+           "path": "batched FP32: sum_s A_ss = Y^dag[sum_s Q_s P_s]; A_sd = Y^dag[Q_s P + Q P_s] in batches; "
+                   "tau sums as per-L Gram products on FP64 matrix cores"}
+    log("tauNL-loc: %.3f s/numerator (%d of %d simulation maps resident)" % (dt, n_res, 2 * nsims))
+    del tm
+    torch.cuda.empty_cache()
+    tmf, _ = make_template(base, raw, mask, tables, lmax, True, templates=('gNL-loc', 'tauNL-loc'))
+    tmf.compute_fisher_contribution(0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fish = [tmf.compute_fisher_contribution(1 + i) for i in range(fisher_steps)]
+    torch.cuda.synchronize()
+    dtf = (time.perf_counter() - t0) / fisher_steps
+    out["fisher"] = {"contributions_per_s": 1. / dtf, "s_per_contribution": dtf, "templates": ['gNL-loc', 'tauNL-loc'],
+                     "mean_contribution": np.mean(fish, axis=0).tolist(),
+                     "path": "masked sky: S^-1(mask to_map(beam a)), A^-1 at base lmax, 'Ainv' weighting of the "
+                             "Q derivatives through the masked S^-1; FP32 transforms, FP64 outer product"}
+    log("fisher (gNL+tauNL): %.3f s/contribution" % dtf)
+    del tmf
+    torch.cuda.empty_cache()
+    return out
 
 
 def lognormal_noise(base, mask, sigma_ln=0.5, fwhm_deg=5., seed=11, noise_uK_arcmin=30.):
@@ -119,9 +164,17 @@ def lognormal_noise(base, mask, sigma_ln=0.5, fwhm_deg=5., seed=11, noise_uK_arc
     return np.stack([v, 2 * v, 2 * v])
 
 
-def _cg_trace(W, d, precond, nstep, thresh_rz=1e-6, rb_marks=(1e-3, 1e-4)):
+def _true_r_over_b(cg):
+    """|b0 - A (x0 + x)| / |b0| of the current iterate (one extra application of A, untimed)."""
+    import torch
+    r = cg.b0 - cg._apply_A((cg.x0 + cg.x).contiguous())
+    return float(torch.sqrt(cg.dot(r, r))) / cg.bnorm
+
+
+def _cg_trace(W, d, precond, nstep, thresh_rz=1e-6, rb_marks=(1e-3, 1e-4, 1e-5, 1e-6)):
     """One solve from the Wiener start, timed to the reference stopping rule rz/rz0 < thresh_rz
-    (weighting.py:342-351), then continued to record when |r|/|b| first passes rb_marks."""
+    (weighting.py:342-351), then continued to record when |r|/|b| (the recursively updated
+    residual) first passes rb_marks; at each mark the true residual |b - A x|/|b| is recorded too."""
     import torch
     from polyspec_amd.learned import FlexibleCGD
     from polyspec_amd.weighting import CGD
@@ -138,16 +191,23 @@ def _cg_trace(W, d, precond, nstep, thresh_rz=1e-6, rb_marks=(1e-3, 1e-4)):
             out.update(iterations=cg.i, wall_s=time.perf_counter() - t0, final_r_over_b=cg.history[-1][2])
         for th in rb_marks:
             if cg.history[-1][2] < th and th not in marks:
+                torch.cuda.synchronize()
                 marks[th] = {"iterations": cg.i, "wall_s": time.perf_counter() - t0}
+                marks[th]["true_r_over_b"] = _true_r_over_b(cg)
+                t0 += time.perf_counter() - t0 - marks[th]["wall_s"]  # the check is not part of the solve
         if out["iterations"] is not None and len(marks) == len(rb_marks):
             break
     out["ms_per_iteration"] = out["wall_s"] / out["iterations"] * 1e3 if out["iterations"] else None
     out["r_over_b_marks"] = {"%.0e" % k: v for k, v in sorted(marks.items(), reverse=True)}
+    out["r_over_b_marks_not_reached"] = ["%.0e" % th for th in rb_marks if th not in marks]
+    out["final_iteration"] = cg.i
+    out["final_r_over_b"] = cg.history[-1][2]
     out["fallback_at"] = getattr(cg, "fallback_at", None)
     return out
 
 
-def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=2000, nside_lo=128, batch=16):
+def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=2000, nside_lo=128, batch=16,
+             fp64_nstep=400):
     """Inverse-variance CG (applySinv_optimal's solver) at base_lmax: plain harmonic CG against the
     flexible CG with the learned preconditioner, both judged by rz/rz0 < thresh."""
     import torch
@@ -176,6 +236,15 @@ def bench_cg(nside, mask, thresh=1e-6, nstep=400, precision=32, learned_pairs=20
     plain["preconditioner"] = "harmonic (Sigma_lm, weighting.py:304-305)"
     log("cg plain: %s" % plain)
     res = {"plain": plain, "setup": info}
+    # the same solve with FP64 SHTs, to the 1e-6 relative residual (which FP32 transforms may not reach)
+    base64 = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=64)
+    W64 = Weightings(base64, masks, dict(raw), noise_cov=ncov)
+    p64 = _cg_trace(W64, d.to(torch.float64), None, fp64_nstep, thresh)
+    p64["preconditioner"] = "harmonic, FP64 SHTs"
+    log("cg plain fp64: %s" % p64)
+    res["plain_fp64_sht"] = p64
+    del W64, base64
+    torch.cuda.empty_cache()
     if batch:
         # batch maps solved in one lock-step CG against the same maps solved one by one
         ds = torch.stack([data(100 + i) for i in range(batch)])
@@ -582,38 +651,69 @@ def bench_radial(base, raw, mask, tables, lmax, tolerance=1e-3, cpu_nodes=24):
     return out
 
 
-def cpu_baseline(data_np, base, tm, nr_sample, lmax):
-    """FP64 oracle on the host: S^-1 of the data plus `nr_sample` of the 40 radial
-    nodes (2 syntheses each + gnl sums), extrapolated to one numerator."""
+def _cpu_baseline_child(path):
+    """Runs in a child process started with OMP_PROC_BIND=close and OMP_NUM_THREADS = the parent's
+    core count (the OpenMP runtime reads them when it starts; setting them in the GPU process would
+    bind that whole process to one core).  Inputs and result go through the .npz / .json at path."""
     import oracle
-    import torch
-    ncores = len(os.sched_getaffinity(0))
-    nside = base.Nside
-    L = base.lmax
+    z = np.load(path + ".npz")
+    nside, L, lmax, nr_sample = (int(z[k]) for k in ("nside", "L", "lmax", "nr_sample"))
     oracle.lib()
     t0 = time.perf_counter()
-    alm = oracle.to_lm(data_np, nside, L)
-    lf = base.l_arr <= lmax
-    ll = base.l_arr[lf]
-    h = base.beam[:, ll] * np.einsum('abl,bl->al', base.inv_Cl_tot_mat[:, :, ll], alm[:, lf])
+    alm = oracle.to_lm(z["data"], nside, L)
+    l_arr = oracle.getlm(L)[0]
+    lf = l_arr <= lmax
+    ll = l_arr[lf]
+    h = z["beam"][:, ll] * np.einsum('abl,bl->al', z["invC"][:, :, ll], alm[:, lf])
     t_sinv = time.perf_counter() - t0
     h = np.ascontiguousarray(h[:2])
     ls = ll.astype(np.int32)
-    fP = np.ascontiguousarray(tm.plLXs[0][:, :, :nr_sample])
-    fQ = np.ascontiguousarray(tm.qlXs[:, :, :nr_sample])
     t0 = time.perf_counter()
-    summP = oracle.apply_fl_weights(fP, h, ls, 1. / np.sqrt(4 * np.pi))
-    summQ = oracle.apply_fl_weights(fQ, h, ls, 1.)
+    summP = oracle.apply_fl_weights(np.ascontiguousarray(z["fP"]), h, ls, 1. / np.sqrt(4 * np.pi))
+    summQ = oracle.apply_fl_weights(np.ascontiguousarray(z["fQ"]), h, ls, 1.)
     P = oracle.to_map_vec(summP, nside, lmax)
     Q = oracle.to_map_vec(summQ, nside, lmax)
-    s = oracle.gnl_loc_sum(tm.r_weights['gNL-loc'][:nr_sample], P, Q)
+    oracle.gnl_loc_sum(z["w"], P, Q)
     t_nodes = time.perf_counter() - t0
+    with open(path + ".json", "w") as f:
+        json.dump({"t_sinv": t_sinv, "t_nodes": t_nodes}, f)
+
+
+def cpu_baseline(data_np, base, tm, nr_sample, lmax):
+    """FP64 oracle on the host cores: S^-1 of the data plus `nr_sample` of the N_r radial nodes
+    (2 syntheses each + gnl sums), extrapolated linearly in nodes to one numerator."""
+    import subprocess
+    import tempfile
+    ncores = len(os.sched_getaffinity(0))
+    L = base.lmax
+    tmp = tempfile.mkdtemp(prefix="ps_cpu_")
+    path = os.path.join(tmp, "cpu")
+    np.savez(path + ".npz", data=data_np, nside=base.Nside, L=L, lmax=lmax, nr_sample=nr_sample,
+             beam=np.asarray(base.beam), invC=np.asarray(base.inv_Cl_tot_mat),
+             fP=np.ascontiguousarray(tm.plLXs[0][:, :, :nr_sample]), fQ=np.ascontiguousarray(tm.qlXs[:, :, :nr_sample]),
+             w=tm.r_weights['gNL-loc'][:nr_sample])
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(ncores))
+    subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", path], env=env, check=True,
+                   stdout=sys.stderr)
+    with open(path + ".json") as f:
+        t = json.load(f)
+    for ext in (".npz", ".json"):
+        os.remove(path + ext)
+    os.rmdir(tmp)
+    t_sinv, t_nodes = t["t_sinv"], t["t_nodes"]
     per_num = t_sinv + (tm.N_r / nr_sample) * t_nodes
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": 1.0 / per_num, "unit": "numerators/s", "cores": ncores, "kind": "port",
-            "sample": ("FP64 C/OpenMP oracle on %d host threads: S^-1 of one TQU data map at lmax %d "
-                       "(%.2f s) + %d of %d radial nodes (2 syntheses each at lmax %d + gnl sums, %.2f s), "
+            "cpu_model": model, "omp": "OMP_PROC_BIND=close OMP_PLACES=cores OMP_NUM_THREADS=%d" % ncores,
+            "sample": ("FP64 C/OpenMP oracle on %d host threads (%s; OMP_PROC_BIND=close): S^-1 of one TQU data map "
+                       "at lmax %d (%.2f s) + %d of %d radial nodes (2 syntheses each at lmax %d + gnl sums, %.2f s), "
                        "extrapolated linearly in nodes to one numerator (%.1f s)") %
-                      (ncores, L, t_sinv, nr_sample, tm.N_r, lmax, t_nodes, per_num)}
+                      (ncores, model, L, t_sinv, nr_sample, tm.N_r, lmax, t_nodes, per_num)}
 
 
 def main():
@@ -627,7 +727,9 @@ def main():
     ap.add_argument("--nsims", type=int, default=100)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-simulation FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-nodes", type=int, default=2)
+    ap.add_argument("--cpu-nodes", type=int, default=8)
+    ap.add_argument("--no-tau", action="store_true", help="skip the tauNL-loc numerator and Fisher section")
+    ap.add_argument("--tau-steps", type=int, default=3)
     ap.add_argument("--no-cg", action="store_true", help="skip the CG S^-1 section")
     ap.add_argument("--cg-batch", type=int, default=16, help="maps of the batched CG measurement (0: skip)")
     ap.add_argument("--learned-pairs", type=int, default=2000,
@@ -739,6 +841,10 @@ def main():
                                                                         np.abs(vals[:len(gvals)])))}
         log("graph replay: %.2f ms/numerator, %.1f numerators/s" % (dtg / args.steps * 1e3, args.steps / dtg))
 
+    if not args.no_tau:
+        log("tauNL-loc numerators and Fisher contributions ...")
+        result["tauNL_loc"] = bench_tau(base32, raw, mask, tables, lmax, args.nsims, datas, steps=args.tau_steps)
+
     if args.copy_calib:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -806,6 +912,9 @@ def main():
 
 
 if __name__ == "__main__":
+    if len(sys.argv) == 3 and sys.argv[1] == "--cpu-baseline-child":
+        _cpu_baseline_child(sys.argv[2])
+        sys.exit(0)
     # the library prints progress as the reference does; keep stdout for the one JSON line
     import contextlib
     with contextlib.redirect_stdout(sys.stderr):

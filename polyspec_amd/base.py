@@ -18,6 +18,7 @@ import time
 import numpy as np
 import torch
 
+from . import kernels as K
 from ._abi import ShtPlan
 
 
@@ -462,10 +463,16 @@ class PolySpec:
         if lmax == -1:
             lmax = self.lmax
         assert input_type in ['harmonic', 'map'], "Valid input types are 'harmonic' and 'map' only!"
+        npy = not isinstance(input_map, torch.Tensor)
         if input_type == 'map':
             input_map_lm = self.to_lm(input_map, lmax=lmax)
         else:
-            input_map_lm = input_map.copy()
-        lfilt = self.l_arr <= lmax
-        bl = self.beam_lm[:, lfilt]
-        return bl * np.einsum('ijk,jk->ik', self.inv_Cl_tot_lm_mat[:, :, lfilt], bl * input_map_lm, order='C')
+            input_map_lm = input_map
+        a = self._to_dev(input_map_lm, True)
+        assert a.dim() == 2 and a.shape[1] == (lmax + 1) * (lmax + 2) // 2, "Wrong number of alm coefficients!"
+        nf = a.shape[0]
+        # B C_tot^-1 B per (l, m) as one device pass of the per-l 3x3 product
+        out = K.lmat_apply(a, lmax, mat=np.ascontiguousarray(self.inv_Cl_tot_mat[:nf, :nf, :lmax + 1]),
+                           pre=np.ascontiguousarray(self.beam[:nf, :lmax + 1]),
+                           post=np.ascontiguousarray(self.beam[:nf, :lmax + 1]))
+        return self._ret(out, npy)

@@ -880,6 +880,155 @@ __global__ __launch_bounds__(256) void k_leg_synth_s0_mfma(ShtDev<float> d, int 
     }
 }
 
+// ------------------- Legendre synthesis, spin 0, FP32 matrix cores, recursion per wave in registers
+// As k_leg_synth_s0_mfma (each wave runs the FP64 recursion of its 32 ring pairs in both lane
+// halves and feeds the matrix cores from registers), but one block covers NT column tiles (all
+// 80 columns of the Nside-512 numerator at NT = 5), so each recursion value feeds 2 NT MFMAs, and
+// the l loop is one compact loop over double-buffered LDS chunks: the alm rows and the recursion
+// coefficients of the next chunk are copied global -> LDS by global_load_lds_dwordx4 (16 bytes
+// per lane, no staging registers) while the current chunk runs; one barrier per chunk.  Rows past
+// the end of the transform and columns past ld read clamped, finite entries: such rows only ever
+// meet zero lambda values and such columns land in outputs that are not stored.
+typedef __attribute__((address_space(3))) void ps_lds_void;
+
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_leg_synth_s0_wr(ShtDev<float> d, int ncol, int ld,
+                                                           const cplx<float>* __restrict__ alm_t,
+                                                           cplx<float>* __restrict__ phase, int rb0, int nrbs) {
+    constexpr int BCM = 16 * NT;
+    constexpr int RW = 2 * BCM;
+    __shared__ __align__(16) float sa[2][LCH * RW];
+    __shared__ __align__(16) double sco[2][LCH * 4];  // (alpha, beta, gamma, 0) per l
+    __shared__ cplx<float> rot[128];
+    const int nchunk = (ncol + BCM - 1) / BCM;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = rb0 + (int)(lb % nrbs);
+    const int chunk = (lb / nrbs) % nchunk;
+    const int m = __builtin_amdgcn_readfirstlane(m_of_q(lb / (nrbs * nchunk), d.mmax));
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int h = lane >> 5, li32 = lane & 31;
+    const int jp = rb * 128 + wave * 32 + li32;
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    if (tid < 128) {
+        const int jr = min(rb * 128 + tid, d.npairs - 1);
+        const RingInfo ri = d.rings[jr];
+        rot[tid] = expi_pi<float>((long)m * ri.phi_p, ri.phi_q);
+    }
+    v16f Dev[NT], Dod[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) Dev[c] = Dod[c] = v16f{0};
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const float* a = reinterpret_cast<const float*>(alm_t + (size_t)chunk * BCM);
+    const int skip = __builtin_amdgcn_readfirstlane(__syncthreads_and(d.dead[(size_t)m * d.npairs + jpc]));
+    if (!skip) {
+        const double x = d.cth[jpc];
+        Rec r;
+        rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
+        const int nl = d.lmax + 1 - m;
+        const int nch = (nl + LCH - 1) / LCH;
+        static_assert(LCH * RW == 4 * 256 * NT, "NT 16-byte copies per thread per chunk");
+        static_assert(LCH * 4 * 8 == 64 * 16, "one wave copies a chunk's coefficients");
+        // chunk c -> buffer buf: thread tid copies the 16-byte pieces q = tid + 256 e of the rows
+        auto stage = [&](int c, int buf) {
+            const int l0n = m + c * LCH, nn = min(LCH, nl - c * LCH);
+#pragma unroll
+            for (int e = 0; e < NT; e++) {
+                const int q = tid + 256 * e, li = min(q / (8 * NT), nn - 1), c4 = q % (8 * NT);
+                const int col = min(chunk * BCM + 2 * c4, ld - 2);  // ld is even
+                const float* src = reinterpret_cast<const float*>(alm_t) + ((size_t)(base + l0n + li) * ld + col) * 2;
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&sa[buf][4 * (256 * e + 64 * wave)],
+                                                 16, 0, 0);
+            }
+            if (wave == 0) {
+                const int l = min(l0n + (lane >> 1), d.lmax);
+                const char* src = reinterpret_cast<const char*>(d.coef + 4 * (base + l)) + 16 * (lane & 1);
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&sco[buf][0], 16, 0, 0);
+            }
+        };
+        stage(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        bool fast = __all(r.k == 0);
+        for (int c = 0; c < nch; c++) {
+            const int cur = c & 1;
+            if (c + 1 < nch) stage(c + 1, cur ^ 1);
+            const int n = __builtin_amdgcn_readfirstlane(min(LCH, nl - c * LCH));
+            const double* co = sco[cur];
+            const float* sab = sa[cur];
+            for (int g = 0; g < n; g += 4) {
+                double v[4];
+                bool live = true;
+                if (g + 4 <= n) {  // a full group: four steps, no tail tests
+                    double ca[4], cg[4];
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        ca[cc] = co[4 * (g + cc)] * x;
+                        cg[cc] = co[4 * (g + cc) + 2];
+                    }
+                    if (fast) {
+#pragma unroll
+                        for (int cc = 0; cc < 4; cc++) {
+                            v[cc] = r.f;
+                            rec_fast(r, ca[cc], cg[cc]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int cc = 0; cc < 4; cc++) {
+                            v[cc] = r.f * r.cf;
+                            rec_step(r, ca[cc], cg[cc]);
+                        }
+                    }
+                } else {  // the last, partial group of the transform
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        v[cc] = g + cc < n ? r.f * r.cf : 0.0;
+                        if (g + cc < n) rec_step(r, co[4 * (g + cc)] * x, co[4 * (g + cc) + 2]);
+                    }
+                }
+                if (!fast) {
+                    rec_rescale(r);
+                    fast = __all(r.k == 0);
+                    live = __any(r.k >= scale_consts<float>::KMIN - 1);  // all below FP32 range: skip
+                }
+                if (!live) continue;
+                const float aev = (float)(h ? v[2] : v[0]), aod = (float)(h ? v[3] : v[1]);
+#pragma unroll
+                for (int cc = 0; cc < NT; cc++) {
+                    const float bev = sab[(g + 2 * h) * RW + 32 * cc + li32];
+                    const float bod = sab[(g + 1 + 2 * h) * RW + 32 * cc + li32];
+                    Dev[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[cc], 0, 0, 0);
+                    Dod[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[cc], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // the next chunk's copies have landed in LDS
+            __syncthreads();
+        }
+    }
+    const bool re = (li32 & 1) == 0;
+#pragma unroll
+    for (int c = 0; c < NT; c++) {
+        const int col = chunk * BCM + 16 * c + (li32 >> 1);
+#pragma unroll
+        for (int vv = 0; vv < 16; vv++) {
+            const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+            const int jr = rb * 128 + wave * 32 + i;
+            const float e = Dev[c][vv], o = Dod[c][vv];
+            float nv = e + o, sv = e - o;
+            const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);
+            const cplx<float> w = rot[wave * 32 + i];
+            const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+            const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+            if (jr < d.npairs && col < ncol) {
+                const int rn = jr, rs = d.nrings - 1 - jr;
+                reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+                if (rs != rn)
+                    reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+            }
+        }
+    }
+}
+
 // ------------------------------------- Legendre synthesis, spin s, FP32 matrix cores
 // The tiling of the spin-0 kernel with two recursions per ring (+s and (-1)^s (-s)):
 // A = lam+ = (f+ + f-)/2 or lam- = (f+ - f-)/2, B = the alm rows of 8 maps (16 complex
@@ -2918,13 +3067,45 @@ static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     return 0;
 }
 
-// Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32 only), otherwise the batch
-// columns (spin 0) or maps (spin s) per block of the vector kernel.
+// The wide matrix-core synthesis (shape SYN_SHAPE_WR, k_leg_synth_s0_wr): NT = the column tiles
+// of one block, up to 5 (80 columns); wider batches take several column chunks.
+constexpr int SYN_SHAPE_WR = 65;
+static int synth_lt_nt(int ncol) {
+    const int tiles = (ncol + 15) / 16;
+    int nt = tiles <= 5 ? tiles : 5;  // wider batches: the tile count of least padding, largest first
+    for (int c = 5; tiles > 5 && c >= 3; c--)
+        if ((tiles + c - 1) / c * c < (tiles + nt - 1) / nt * nt) nt = c;
+    return nt;
+}
+static int launch_synth_s0_wr(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st,
+                              int rb0 = 0, int rb1 = -1) {
+    ShtDev<float> d = dev_view<float>(p);
+    const int nt = synth_lt_nt(ncol);
+    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * nt - 1) / (16 * nt);
+    if (rb1 < 0) rb1 = nrb;
+    const unsigned nb = (unsigned)((rb1 - rb0) * nchunk * (p.mmax + 1));
+    const cplx<float>* a = (const cplx<float>*)alm_t;
+    cplx<float>* ph = (cplx<float>*)phase;
+    switch (nt) {
+        case 1: hipLaunchKernelGGL(k_leg_synth_s0_wr<1>, dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph, rb0, rb1 - rb0); break;
+        case 2: hipLaunchKernelGGL(k_leg_synth_s0_wr<2>, dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph, rb0, rb1 - rb0); break;
+        case 3: hipLaunchKernelGGL(k_leg_synth_s0_wr<3>, dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph, rb0, rb1 - rb0); break;
+        case 4: hipLaunchKernelGGL(k_leg_synth_s0_wr<4>, dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph, rb0, rb1 - rb0); break;
+        default: hipLaunchKernelGGL(k_leg_synth_s0_wr<5>, dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph, rb0, rb1 - rb0); break;
+    }
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+// Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32 only), SYN_SHAPE_WR = its wide form
+// (FP32 spin 0), otherwise the batch columns (spin 0) or maps (spin s) per block of the vector
+// kernel.  Measured at Nside 256 and 512: the wide form is fastest from 24 columns up (1.2-1.4x),
+// the 16-column one from 5 to 16 columns, the vector kernel up to 4.
 template <typename T>
 static int synth_shape_rule(const ShtPlan& p, int ncol) {
     if (p.spin == 0) {
         constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
-        if (sizeof(T) == 4 && ncol > 8) return 0;
+        if (sizeof(T) == 4 && ncol > 16) return SYN_SHAPE_WR;
+        if (sizeof(T) == 4 && ncol > 4) return 0;
         return ncol <= 1 ? 1 : ncol <= 2 ? 2 : ncol <= 4 ? 4 : (BCMAX > 8 && ncol <= 8) ? 8 : BCMAX;
     }
     constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
@@ -2940,6 +3121,9 @@ static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* a
     switch (shape) {
         case 0:
             if constexpr (sizeof(T) == 4) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
+            break;
+        case SYN_SHAPE_WR:
+            if constexpr (sizeof(T) == 4) return launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st);
             break;
         case 1: return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
         case 2: return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
@@ -3031,17 +3215,22 @@ static int synthesis_overlapped(const ShtPlan& p, int ncol, const void* alm_t, i
     const bool pow2_nside = (p.nside & (p.nside - 1)) == 0;
     int shape = shape_of(p, ncol, SHT_STAGE_SYNTH);
     if (shape < 0) shape = synth_shape_rule<float>(p, ncol);  // no tuned or set shape: the built-in rule
-    if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb || shape != 0)
+    if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb ||
+        (shape != 0 && shape != SYN_SHAPE_WR))
         return 1;
+    auto legendre = [&](int r0, int r1) {
+        return shape == SYN_SHAPE_WR ? launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st, r0, r1)
+                                     : launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, r0, r1);
+    };
     if (!p.side) {
         PS_CHECK_HIP(hipStreamCreateWithFlags(&p.side, hipStreamNonBlocking));
         PS_CHECK_HIP(hipEventCreateWithFlags(&p.ev_fork, hipEventDisableTiming));
         PS_CHECK_HIP(hipEventCreateWithFlags(&p.ev_join, hipEventDisableTiming));
     }
-    int rc = launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, 0, rbA);
+    int rc = legendre(0, rbA);
     if (rc) return rc;
     PS_CHECK_HIP(hipEventRecord(p.ev_fork, st));
-    if ((rc = launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, rbA, nrb))) return rc;
+    if ((rc = legendre(rbA, nrb))) return rc;
     PS_CHECK_HIP(hipStreamWaitEvent(p.side, p.ev_fork, 0));
     // once the side stream is forked it is always joined back into `st`, whatever fails in
     // between: no unjoined fork under capture, no side-stream work unordered with `st`
@@ -3138,6 +3327,7 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
             int top = 1;
             while (top < w && top < bcmax) top *= 2;
             if (p.prec == 4) add(0);
+            if (p.prec == 4 && p.spin == 0) add(SYN_SHAPE_WR);
             for (int bc = std::max(1, top / 2); bc <= top; bc *= 2) add(bc);
             break;
         }

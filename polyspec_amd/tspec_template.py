@@ -484,11 +484,14 @@ class TSpecTemplate:
         b = self._masked(self.base.generate_data(int(2e5) + ii, Cl_input=Cl_input, deconvolve_beam=False))
         return (a, b), 'map'
 
-    def _beam_times(self, a):
-        """beam_lm * a for alm at base lmax (host or device arrays)."""
+    def _beam_times(self, a, lmax=None):
+        """beam_lm * a for alm at lmax (default: base lmax), host or device arrays."""
+        b = np.asarray(self.base.beam_lm)
+        if lmax is not None:
+            b = b[:, self.base.l_arr <= lmax]
         if isinstance(a, torch.Tensor):
-            return a * torch.from_numpy(np.asarray(self.base.beam_lm)).to(a.device)
-        return np.asarray(self.base.beam_lm) * a
+            return a * torch.from_numpy(np.ascontiguousarray(b)).to(a.device)
+        return b * a
 
     def _masked(self, m):
         if isinstance(m, torch.Tensor):
@@ -900,8 +903,7 @@ class TSpecTemplate:
             t0 = time.time()
             if weighting == 'Sinv':
                 if self.ones_mask:
-                    lf = self.base.l_arr <= self.lmax
-                    U = [self.applySinv(self.base.beam_lm[:, lf] * a, input_type='harmonic', lmax=self.lmax)
+                    U = [self.applySinv(self._beam_times(a, self.lmax), input_type='harmonic', lmax=self.lmax)
                          for a in a_maps]
                 else:
                     U = [self.applySinv(self._masked(self.base.to_map(self._beam_times(a))), lmax=self.lmax)
@@ -912,7 +914,8 @@ class TSpecTemplate:
                     U = [self.base.applyAinv(a, input_type='harmonic', lmax=self.lmax) for a in a_maps]
                 else:
                     lf = self.base.l_arr <= self.lmax
-                    U = [self.base.applyAinv(a, input_type='harmonic')[:, lf] for a in a_maps]
+                    U = [self.base.applyAinv(a, input_type='harmonic') for a in a_maps]
+                    U = [u[:, torch.from_numpy(lf).to(u.device)] if isinstance(u, torch.Tensor) else u[:, lf] for u in U]
                 self.timers['Ainv'] += time.time() - t0
             hs = [self._h_dev(u) for u in U]
             maps = self._filter(hs)  # (2, 2, N_r, Npix)

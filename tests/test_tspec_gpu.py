@@ -104,6 +104,8 @@ def test_fisher_matches_oracle(gpu, nside, lmax, pol, nr, templates, precision):
     ref = orc.fisher_from_grfs(a_maps)
     tol = 1e-8 if precision == 64 else 2e-4
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (got, ref)
+    got_d = tm.fisher_contribution_from_grfs([torch.from_numpy(a).to(gpu) for a in a_maps])
+    assert np.abs(got_d - ref).max() <= tol * np.abs(ref).max(), (got_d, ref)
 
 
 @pytest.mark.parametrize("pol,precision", [(True, 32), (True, 64), (False, 32)])
@@ -179,3 +181,6 @@ def test_masked_sky_numerator_and_fisher_match_oracle(gpu, pol, precision):
     got = tm.fisher_contribution_from_grfs(a_maps)
     ref = orc.fisher_from_grfs(a_maps)
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (got, ref)
+    # the same GRFs handed over as device tensors (generate_data with grf_device=True)
+    got_d = tm.fisher_contribution_from_grfs([torch.from_numpy(a).to(gpu) for a in a_maps])
+    assert np.abs(got_d - ref).max() <= tol * np.abs(ref).max(), (got_d, ref)
