@@ -69,3 +69,35 @@ def test_p_and_q_integrals_reduce_to_F():
         assert np.all(np.abs(p[l, 0] - want) < 3e-7 * np.abs(want)), (l, p[l, 0], want)
         assert np.all(np.abs(q[l, 0] - want) < 3e-7 * np.abs(want)), (l, q[l, 0], want)
     assert np.all(p[:lmin] == 0) and np.all(q[:lmin] == 0)
+
+
+def test_hyp2f1_large_L_near_one_matches_mpmath():
+    """The F_L closed form needs 2F1(L+(ns-1)/2, (ns-2)/2; L+3/2; z) up to L = 3 Nside - 1 with z =
+    (r_</r_>)^2 up to 1 (neighbouring radial nodes and the diagonal).  scipy's hyp2f1 returns inf /
+    nan there from L ~ 170 on (0 at z = 1, L = 170); hyp2f1_F must match 30-digit mpmath."""
+    import mpmath
+    from polyspec_amd.k_integrals import hyp2f1_F
+    zs = np.array([1.0, 1 - 1e-6, 1 - 1e-4, 1 - 8e-4, 1 - 2e-3, 1 - 3e-3, 0.99, 0.95, 0.85, 0.5])
+    with mpmath.workdps(30):
+        for L in (1, 50, 170, 171, 500, 1535, 3000):
+            a, b, c = L + (NS - 1) / 2, (NS - 2) / 2, L + 1.5
+            got = hyp2f1_F(a, b, c, zs)
+            want = np.array([float(mpmath.hyp2f1(mpmath.mpf(L) + (mpmath.mpf(NS) - 1) / 2, (mpmath.mpf(NS) - 2) / 2,
+                                                 mpmath.mpf(L) + mpmath.mpf(1.5), mpmath.mpf(z))) for z in zs])
+            assert np.all(np.abs(got / want - 1) < 1e-8), (L, got, want)
+
+
+def test_F_table_finite_at_the_nside512_workload():
+    """40 Gauss-Legendre nodes on [13400, 14600] Mpc, L up to 1535 (Nside 512): every entry finite,
+    symmetric, and on the diagonal equal to the Gauss-sum value of the closed form."""
+    from scipy.special import gammaln, roots_legendre
+    x, _ = roots_legendre(40)
+    r = 600. * x + 14000.
+    F = F_integral_n0(r, 1, 1535, NS, AS, KP)
+    assert np.all(np.isfinite(F)) and np.allclose(F, F.transpose(0, 2, 1), rtol=1e-13, atol=0)
+    L = np.arange(1, 1536)
+    a, b, c = L + (NS - 1) / 2, (NS - 2) / 2, L + 1.5
+    pref = 2. ** (NS - 3) * 2 * np.pi ** 2 * AS / KP ** (NS - 1)
+    gauss = np.exp(gammaln(c) + gammaln(c - a - b) - gammaln(c - a) - gammaln(c - b))
+    diag = pref * np.exp(gammaln(a) - gammaln(1 - b) - gammaln(c)) * gauss * r[7] ** (1 - NS)
+    assert np.allclose(F[1:, 7, 7], diag, rtol=1e-11, atol=0)
