@@ -75,7 +75,8 @@ def test_spin_synthesis_and_adjoint_match_oracle(gpu, nside, lmax, prec, nmaps, 
     assert relerr(out, ref) < tol
 
 
-@pytest.mark.parametrize("nside,lmax,spin", [(512, 1024, 0), (512, 1535, 2), (256, 767, 1), (1024, 2048, 0)])
+@pytest.mark.parametrize("nside,lmax,spin", [(512, 1024, 0), (512, 1535, 2), (256, 767, 1), (1024, 2048, 0),
+                                               (1024, 2048, 2)])
 def test_adjointness_full_size(gpu, nside, lmax, spin):
     """<Y a, m> = <a, Y^T m>_w with w_m = 1 + [m > 0]: holds at any size (FP64 kernels)."""
     from polyspec_amd._abi import ShtPlan
