@@ -197,6 +197,15 @@ int ps_tau_sum_n0(int precision, const void *A, const void *B, int nmu, int nr, 
 int ps_tau_sum_lmajor(int precision, const void *At, const void *Bt, long long ld, int nmu, int nr, int Lmaxs,
                       int Lmin, int Lmax, const double *FLL, const double *w, int n1, double *scratch, double *out,
                       int accumulate, void *stream);
+/* The same sum over shifted maps, sum_mu tau_sum_n0(A_mu - s_A M, B_mu - s_B M) with s_A = shiftA,
+ * s_B = shiftB in {0, 1} and Mt (nlm(Lmaxs), nr) l-major in the data precision, the shift applied
+ * as the values are widened to FP64.  No reference counterpart: the batched tauNL-loc numerator
+ * subtracts the Monte-Carlo mean field M = mean_s A_ss from every self-product A map, which
+ * leaves t4 + t2 + t0 (tspec_template.py:1420-1432, 1655-1687) unchanged in exact arithmetic but
+ * turns its terms from ~1e3 sigma (masked sky) into ~sigma, so FP32 A maps suffice. */
+int ps_tau_sum_lmajor_shift(int precision, const void *At, const void *Bt, long long ld, int nmu, int nr, int Lmaxs,
+                            int Lmin, int Lmax, const double *FLL, const double *w, int n1, const void *Mt, int shiftA,
+                            int shiftB, double *scratch, double *out, int accumulate, void *stream);
 
 /* ps_tau_sum_lmajor without a scratch argument and without allocating: the per-L partial sums
  * go to a device array built into the library (8192 doubles per GPU, held until the process

@@ -46,6 +46,7 @@ SIGNATURES = {
     "ps_tau_sum_n0": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
     "ps_tau_sum_lmajor": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
     "ps_tau_sum_lmajor_noalloc": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P]),
+    "ps_tau_sum_lmajor_shift": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _I, _P]),
     "ps_sht_alm_to_lmajor": (_I, [_P, _I, _P, _I, _P, _P]),
     "ps_outer_product": (_I, [_I, _P, _P, _I, _I, _LL, _P, _P, _I, _P]),
     "ps_convolve_F_lmajor": (_I, [_I, _P, _I, _I, _I, _I, _P, _P, _I, _P, _P]),

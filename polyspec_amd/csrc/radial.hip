@@ -369,12 +369,16 @@ __global__ __launch_bounds__(256) void k_tau_gram(const cplx<T>* __restrict__ A,
 // j + 16 (i % 4), register i / 4.  One block per L, heaviest (largest L) first; the four waves
 // take alternate entry pairs and contract their partial G_L with F_L in the epilogue (the
 // contraction is linear in G), so no Gram matrix is stored.  NT = ceil(nr / 16) tiles per side.
+// Optional shift: Mt (l-major, [idx(L,M) * nr + j]) is subtracted from every A_mu (shiftA) and/or
+// B_mu (shiftB) column block as the values are widened to FP64, i.e. the sum is taken over
+// (A_mu - M, B_mu - M).
 typedef double ps_dbl4 __attribute__((ext_vector_type(4)));
 
 template <typename T, int NT, bool SAME>
 __global__ __launch_bounds__(256) void k_tau_gram_mfma(const cplx<T>* __restrict__ At, const cplx<T>* __restrict__ Bt,
                                                        long ld, int nmu, int nr, int Lmaxs, int Lmax,
                                                        const double* __restrict__ FLL, const double* __restrict__ w,
+                                                       const cplx<T>* __restrict__ Mt, int shiftA, int shiftB,
                                                        double pref, double* __restrict__ part) {
     __shared__ double sh[256];
     const int L = Lmax - (int)blockIdx.x;
@@ -394,19 +398,25 @@ __global__ __launch_bounds__(256) void k_tau_gram_mfma(const cplx<T>* __restrict
             const long row = (long)M * (2 * Lmaxs + 1 - M) / 2 + rowL;
             const cplx<T>* ra = At + row * ld + (long)mu * nr;
             const cplx<T>* rb = Bt + row * ld + (long)mu * nr;
+            const cplx<T>* rm = Mt ? Mt + row * nr : nullptr;
             const double wm = M == 0 ? 1.0 : 2.0;
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const int j = 16 * t + c;
                 double va = 0.0, vb = 0.0;
                 if (j < nr) {
+                    double sh = 0.0;
+                    if (rm) {
+                        const cplx<T> z = rm[j];
+                        sh = (double)(im ? z.im : z.re);
+                    }
                     const cplx<T> x = ra[j];
-                    va = (double)(im ? x.im : x.re);
+                    va = (double)(im ? x.im : x.re) - (shiftA ? sh : 0.0);
                     if constexpr (SAME) {
                         vb = va;
                     } else {
                         const cplx<T> y = rb[j];
-                        vb = (double)(im ? y.im : y.re);
+                        vb = (double)(im ? y.im : y.re) - (shiftB ? sh : 0.0);
                     }
                 }
                 a[t] = va;
@@ -573,25 +583,28 @@ static int reduce_launch_fused(const void* P, const void* Q, const void* U, cons
 
 template <typename T, int NT>
 static void launch_tau_mfma(const void* At, const void* Bt, long ld, int nmu, int nr, int Lmaxs, int Lmin, int Lmax,
-                            const double* FLL, const double* w, double pref, double* scratch, hipStream_t st) {
+                            const double* FLL, const double* w, const void* Mt, int shA, int shB, double pref,
+                            double* scratch, hipStream_t st) {
     const dim3 grid((unsigned)(Lmax - Lmin + 1));
-    if (At == Bt)
+    if (At == Bt && shA == shB)
         hipLaunchKernelGGL((k_tau_gram_mfma<T, NT, true>), grid, dim3(256), 0, st, (const cplx<T>*)At,
-                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, pref, scratch);
+                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, (const cplx<T>*)Mt, shA, shB, pref,
+                           scratch);
     else
         hipLaunchKernelGGL((k_tau_gram_mfma<T, NT, false>), grid, dim3(256), 0, st, (const cplx<T>*)At,
-                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, pref, scratch);
+                           (const cplx<T>*)Bt, ld, nmu, nr, Lmaxs, Lmax, FLL, w, (const cplx<T>*)Mt, shA, shB, pref,
+                           scratch);
 }
 
 template <typename T>
 static void launch_tau_mfma_nt(int nt, const void* At, const void* Bt, long ld, int nmu, int nr, int Lmaxs, int Lmin,
-                               int Lmax, const double* FLL, const double* w, double pref, double* scratch,
-                               hipStream_t st) {
+                               int Lmax, const double* FLL, const double* w, const void* Mt, int shA, int shB,
+                               double pref, double* scratch, hipStream_t st) {
     switch (nt) {
-        case 1: launch_tau_mfma<T, 1>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
-        case 2: launch_tau_mfma<T, 2>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
-        case 3: launch_tau_mfma<T, 3>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
-        default: launch_tau_mfma<T, 4>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st); break;
+        case 1: launch_tau_mfma<T, 1>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, Mt, shA, shB, pref, scratch, st); break;
+        case 2: launch_tau_mfma<T, 2>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, Mt, shA, shB, pref, scratch, st); break;
+        case 3: launch_tau_mfma<T, 3>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, Mt, shA, shB, pref, scratch, st); break;
+        default: launch_tau_mfma<T, 4>(At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, Mt, shA, shB, pref, scratch, st); break;
     }
 }
 
@@ -774,7 +787,15 @@ int ps_tau_sum_n0(int precision, const void* A, const void* B, int nmu, int nr, 
 int ps_tau_sum_lmajor(int precision, const void* At, const void* Bt, long long ld, int nmu, int nr, int Lmaxs,
                       int Lmin, int Lmax, const double* FLL, const double* w, int n1, double* scratch, double* out,
                       int accumulate, void* stream) {
+    return ps_tau_sum_lmajor_shift(precision, At, Bt, ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, n1, nullptr, 0, 0,
+                                   scratch, out, accumulate, stream);
+}
+
+int ps_tau_sum_lmajor_shift(int precision, const void* At, const void* Bt, long long ld, int nmu, int nr, int Lmaxs,
+                            int Lmin, int Lmax, const double* FLL, const double* w, int n1, const void* Mt,
+                            int shiftA, int shiftB, double* scratch, double* out, int accumulate, void* stream) {
     PS_REQUIRE(At && Bt && FLL && w && scratch && out, "ps_tau_sum_lmajor: NULL argument");
+    PS_REQUIRE(Mt || (!shiftA && !shiftB), "ps_tau_sum_lmajor_shift: a shift needs Mt");
     PS_REQUIRE(nr >= 1 && nr <= 64, "ps_tau_sum_lmajor: need 1 <= nr <= 64");
     PS_REQUIRE(nmu >= 1 && ld >= (long long)nmu * nr, "ps_tau_sum_lmajor: need ld >= nmu * nr");
     PS_REQUIRE(Lmin >= 0 && Lmax >= Lmin && Lmax <= Lmaxs, "ps_tau_sum_lmajor: need 0 <= Lmin <= Lmax <= Lmaxs");
@@ -782,10 +803,13 @@ int ps_tau_sum_lmajor(int precision, const void* At, const void* Bt, long long l
     hipStream_t st = (hipStream_t)stream;
     const int nL = Lmax - Lmin + 1, nt = (nr + 15) / 16;
     const double pref = 1.0 / (sqrt(4.0 * M_PI) * sqrt(2.0 * n1 + 1.0));
+    const void* M = (shiftA || shiftB) ? Mt : nullptr;
     if (precision == 32)
-        launch_tau_mfma_nt<float>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st);
+        launch_tau_mfma_nt<float>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, M, shiftA, shiftB, pref,
+                                  scratch, st);
     else if (precision == 64)
-        launch_tau_mfma_nt<double>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, pref, scratch, st);
+        launch_tau_mfma_nt<double>(nt, At, Bt, (long)ld, nmu, nr, Lmaxs, Lmin, Lmax, FLL, w, M, shiftA, shiftB, pref,
+                                   scratch, st);
     else
         PS_REQUIRE(false, "ps_tau_sum_lmajor: precision must be 32 or 64");
     PS_LAUNCH_CHECK();

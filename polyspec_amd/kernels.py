@@ -231,10 +231,15 @@ def tau_sum_n0(A, B, FLL, w, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False
     return out
 
 
-def tau_sum_lmajor(At, Bt, FLL, w, nr, nmu, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False):
+def tau_sum_lmajor(At, Bt, FLL, w, nr, nmu, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False, M=None,
+                   shiftA=False, shiftB=False):
     """Sum over mu < nmu of tau_sum_n0 (tauNL_utils.pyx:604-646) of the pairs (A_mu, B_mu), with
-    A, B l-major (nlm(Lmaxs), ld), column mu * nr + j; Bt may be At (tau(A, A))."""
-    _chk(At, Bt)
+    A, B l-major (nlm(Lmaxs), ld), column mu * nr + j; Bt may be At (tau(A, A)).  With M
+    (nlm(Lmaxs), nr) l-major: the pairs (A_mu - M, B_mu - M), shifted on the sides selected by
+    shiftA / shiftB."""
+    _chk(At, Bt, M)
+    if shiftA or shiftB:
+        assert M is not None and M.dtype == At.dtype and M.numel() == _nlm(Lmaxs) * nr, "M must be (nlm(Lmaxs), nr)"
     assert At.dim() == 2 and At.shape[0] == _nlm(Lmaxs), "At must be (nlm(Lmaxs), ld)"
     _same(At, Bt, what="At and Bt")
     assert At.shape[1] >= nmu * nr and 1 <= nr <= 64, "ld must cover nmu * nr columns (nr <= 64)"
@@ -245,9 +250,10 @@ def tau_sum_lmajor(At, Bt, FLL, w, nr, nmu, Lmaxs, Lmin, Lmax, n1=0, out=None, a
     assert 0 <= Lmin and Lmax <= Lmaxs, "L range outside the A maps"
     if out is None:
         out = torch.zeros((), dtype=torch.float64, device=At.device)
-    check(load().ps_tau_sum_lmajor(_prec(At), ptr(At), ptr(Bt), At.shape[1], int(nmu), int(nr), int(Lmaxs),
-                                   int(Lmin), int(Lmax), ptr(FLL), ptr(w), int(n1), ptr(reduce_scratch(At.device)),
-                                   ptr(out), int(accumulate), stream_handle()))
+    check(load().ps_tau_sum_lmajor_shift(_prec(At), ptr(At), ptr(Bt), At.shape[1], int(nmu), int(nr), int(Lmaxs),
+                                         int(Lmin), int(Lmax), ptr(FLL), ptr(w), int(n1),
+                                         ptr(M) if M is not None else None, int(bool(shiftA)), int(bool(shiftB)),
+                                         ptr(reduce_scratch(At.device)), ptr(out), int(accumulate), stream_handle()))
     return out
 
 

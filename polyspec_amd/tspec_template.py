@@ -148,7 +148,7 @@ class TSpecTemplate:
         (their P/Q stacks and Monte-Carlo sums are per node), the cached t0 and a captured
         numerator graph (it holds the old tables and workspaces).  Simulations must be
         generated or loaded again after the nodes change."""
-        for name in ('t0_num', 'preload', 'proc_maps', 'U_d', 'V_d', 'SA_t', 'load_sim_data', 'N_it', '_sim_P', '_sim_Q',
+        for name in ('t0_num', 'preload', 'proc_maps', 'U_d', 'V_d', 'D_t', '_M_t', 'load_sim_data', 'N_it', '_sim_P', '_sim_Q',
                      '_sim_h'):
             if hasattr(self, name):
                 delattr(self, name)
@@ -509,6 +509,14 @@ class TSpecTemplate:
         self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
         self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
         self._t0_acc = torch.zeros(len(self.templates), dtype=torch.float64, device=dev)
+        # tauNL-loc mean-field shift M (l-major (nlm(nLmax), N_r)): every self-product map A_xx
+        # (A_dd, A_aa, A_bb, and sum_s A_ss -> sum_s A_ss - 2 N_it M) enters the tau sums as A_xx - M.
+        # For any fixed M the tau(M, .) terms of t4, t2 and t0 cancel exactly (tau is bilinear and
+        # the prefactors are 12, -12 / N_it and 12 / N_it per pair); with M near E[A_ss] -- on a
+        # masked sky a mean field far larger than the fluctuations -- each term stays near the size
+        # of the result, so FP32 A maps lose no more than FP32 accuracy of the result.  M is the
+        # mean of A_ss over the first chunk of simulations, fixed before any tau sum uses it.
+        self._M_t = None
 
     def _A_batch_lmajor(self, prods):
         """Y^dag at nLmax of a (n, N_r, Npix) stack of product maps -> l-major (nlm(nLmax), n N_r),
@@ -522,14 +530,17 @@ class TSpecTemplate:
         self.timers['tauNL_products'] += time.time() - t0
         return At
 
-    def _tau_lmajor(self, At, Bt, nmu, t, out, scale=1.):
-        """out += scale * sum_mu tau_sum_n0(A_mu, B_mu) (device scalar)."""
+    def _tau_lmajor(self, At, Bt, nmu, t, out, scale=1., shift=(False, False)):
+        """out += scale * sum_mu tau_sum_n0(A_mu, B_mu) (device scalar); shift = (on A, on B)
+        subtracts the mean-field map M from that side."""
         t0 = time.time()
+        kw = dict(M=self._M_t, shiftA=shift[0], shiftB=shift[1]) if any(shift) else {}
         if scale == 1.:
             K.tau_sum_lmajor(At, Bt, self.FLL_d, self.w_d[t], self.N_r, nmu, self.nLmax, self.Lmin, self.Lmax,
-                             out=out, accumulate=True)
+                             out=out, accumulate=True, **kw)
         else:
-            r = K.tau_sum_lmajor(At, Bt, self.FLL_d, self.w_d[t], self.N_r, nmu, self.nLmax, self.Lmin, self.Lmax)
+            r = K.tau_sum_lmajor(At, Bt, self.FLL_d, self.w_d[t], self.N_r, nmu, self.nLmax, self.Lmin, self.Lmax,
+                                 **kw)
             out.add_(r, alpha=scale)
         self.timers['tauNL_summation'] += time.time() - t0
 
@@ -552,8 +563,13 @@ class TSpecTemplate:
                 Qa, Qb = Q[0::2].contiguous(), Q[1::2].contiguous()
                 A_aa = self._A_batch_lmajor(K.multiplyRR(Qa, Pa))
                 A_bb = self._A_batch_lmajor(K.multiplyRR(Qb, Pb))
+                if self._M_t is None:
+                    nlm = A_aa.shape[0]
+                    c128 = torch.complex128
+                    self._M_t = ((A_aa.view(nlm, n, self.N_r).to(c128).sum(1) +
+                                  A_bb.view(nlm, n, self.N_r).to(c128).sum(1)) / (2 * n)).to(A_aa.dtype)
                 tau = self._t0_acc[ii:ii + 1].view(())
-                self._tau_lmajor(A_aa, A_bb, n, t, tau, scale=4.)
+                self._tau_lmajor(A_aa, A_bb, n, t, tau, scale=4., shift=(True, True))
                 del A_aa, A_bb
                 A_ab = self._A_batch_lmajor(K.multiplyRR_sym(Qb, Pa, Qa, Pb))
                 self._tau_lmajor(A_ab, A_ab, n, t, tau, scale=2.)
@@ -568,8 +584,12 @@ class TSpecTemplate:
         self.t0_num = self._t0_acc.cpu().numpy() * pref
         del self._t0_acc
         if 'tauNL-loc' in self.templates:
-            # sum_s A_ss = Y^dag[U], l-major (one column per radial node)
-            self.SA_t = self._A_batch_lmajor(self.U_d.to(self.base.rdtype)[None])
+            # sum_s A_ss = Y^dag[U], l-major (one column per radial node), shifted: D = sum_s (A_ss - M),
+            # formed in FP64 (2 N_it M is exact there for an FP32 M)
+            SA = self._A_batch_lmajor(self.U_d.to(self.base.rdtype)[None])
+            c128 = torch.complex128
+            self.D_t = (SA.to(c128) - (2 * self.N_it) * self._M_t.to(c128)).to(SA.dtype)
+            del SA
         if 'gNL-loc' not in self.templates:
             del self.U_d, self.V_d
 
@@ -771,9 +791,10 @@ class TSpecTemplate:
                 if collapsed:
                     Ad_t = self._A_batch_lmajor(K.multiplyRR(Q, P)[None])
                     v = torch.zeros((), dtype=torch.float64, device=self.base.device)
-                    self._tau_lmajor(Ad_t, Ad_t, 1, t, v, scale=1. / 24. * 12. * FOURPI15)
-                    # 4 tau(sum_s A_ss, A_dd) of the t2 term, linear in the simulations
-                    self._tau_lmajor(self.SA_t, Ad_t, 1, t, v, scale=-1. / 24. * 6. / self.N_it / 2. * FOURPI15 * 4.)
+                    self._tau_lmajor(Ad_t, Ad_t, 1, t, v, scale=1. / 24. * 12. * FOURPI15, shift=(True, True))
+                    # 4 tau(sum_s A_ss, A_dd) of the t2 term, linear in the simulations (both shifted by M)
+                    self._tau_lmajor(self.D_t, Ad_t, 1, t, v, scale=-1. / 24. * 6. / self.N_it / 2. * FOURPI15 * 4.,
+                                     shift=(False, True))
                 else:
                     A_dd = self._A_maps(P, Q)
                     v = self._tau_sum(A_dd, A_dd, t, scale=1. / 24. * 12. * FOURPI15)

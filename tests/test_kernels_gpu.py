@@ -185,6 +185,22 @@ def test_tau_sum_lmajor_matches_oracle(gpu, prec, nr, nmu, same):
     assert abs(got - ref) < (1e-12 if prec == 64 else 2e-6) * np.sum(np.abs(F).max() * np.abs(A).max() ** 2 * nr ** 2), \
         (got, ref)
     assert abs(got - ref) < (1e-11 if prec == 64 else 1e-5) * abs(ref) + 1e-300, (got, ref)
+    # the mean-field shift: pairs (A_mu - M, B_mu - M) on the selected sides, against the oracle on
+    # explicitly shifted maps (the shift is applied in FP64 inside the kernel)
+    Mh = rng.standard_normal((nr, nlm)) + 1j * rng.standard_normal((nr, nlm))
+    Mh = T(Mh, cdt).cpu().numpy()  # the values the kernel sees
+    Ah, Bh = At[:, :nmu * nr].T.cpu().numpy().reshape(nmu, nr, nlm), Bt[:, :nmu * nr].T.cpu().numpy().reshape(nmu, nr, nlm)
+    Mt = T(Mh.T, cdt).contiguous()
+    for sa, sb in ([(True, True), (False, True)] if same else [(True, False), (True, True)]):
+        ref_s = 0.
+        for mu in range(nmu):
+            a = (Ah[mu].astype(np.complex128) - (Mh if sa else 0))[:, keep][:, sel].T[None].copy()
+            b = (Bh[mu].astype(np.complex128) - (Mh if sb else 0))[:, keep][:, sel].T[None].copy()
+            ref_s += oracle.tau_sum_n0(a, b, F, w, Ls[keep][sel].astype(np.int32), Ms[keep][sel].astype(np.int32),
+                                       Lmin, Lmax)
+        got_s = float(K.tau_sum_lmajor(At, Bt, F, w, nr, nmu, Lmaxs, Lmin, Lmax, M=Mt, shiftA=sa, shiftB=sb))
+        assert abs(got_s - ref_s) < 1e-12 * np.sum(np.abs(F).max() * 4 * np.abs(A).max() ** 2 * nr ** 2), \
+            (sa, sb, got_s, ref_s)
     # the transpose into that layout, and the batched symmetric product of simulation maps
     Am = T(A.reshape(nmu * nr, nlm), cdt)
     lt = plan.to_lmajor(Am, ld=ld)
