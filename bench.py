@@ -31,15 +31,29 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gNL-loc numerators/s at Nside 512 (batched vs plain GPU); CG iters to 1e-6"
 PEAK_FP32 = 157.3e12
-# HBM bytes of one k_leg_synth_s0_mfma launch (80 maps, Nside 512, lmax 1024): rocprofv3 FETCH_SIZE x 2
-# + WRITE_SIZE x 1 (factors from a 1 GiB copy), profiles/leg_synth_s0_mfma_nside512_lmax1024_80maps_pmc.md
-LEG_TRAFFIC = {(512, 1024, 80): (189299.2 * 2 + 1311359.4) * 1024}
+# HBM bytes of one Legendre-synthesis launch (k_leg_synth_s0_wr<5>, 80 maps, Nside 512, lmax 1024):
+# rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 in KiB (factors from a 1 GiB copy),
+# profiles/leg_synth_s0_wr_nside512_lmax1024_80maps_pmc.md
+LEG_TRAFFIC = {(512, 1024, 80): (206249.7 * 2 + 1311359.4) * 1024}
 PEAK_HBM = 8.0e12
 PEAK_FP64 = 78.6e12
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def leg_kernel_name(base, lmax, ncol):
+    """Name of the spin-0 Legendre-synthesis kernel the plan of (lmax, ncol maps) launches."""
+    shape = autotune_report(base).get("lmax%d_spin0_%dmaps" % (lmax, ncol), {}).get("shapes", {}).get("synthesis")
+    if shape == 65:
+        tiles = (ncol + 15) // 16  # the kernel's column tiles of 16 per block, as synth_lt_nt in sht.hip
+        nt = min(tiles, 5)
+        for c in range(5, 2, -1):
+            if tiles > 5 and -(-tiles // c) * c < -(-tiles // nt) * nt:
+                nt = c
+        return "k_leg_synth_s0_wr<%d>" % nt
+    return "k_leg_synth_s0_mfma (shape %s)" % shape
 
 
 def autotune_report(base):
@@ -815,9 +829,9 @@ def main():
                    "nside": nside, "lmax": lmax, "n_r": nr, "n_sim_pairs": args.nsims},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": LEG_TRAFFIC.get((nside, lmax, ncol)),
-                     "kernel": "k_leg_synth_s0_mfma (Legendre synthesis, %d maps per launch, %.3f ms avg; "
+                     "kernel": "%s (Legendre synthesis, %d maps per launch, %.3f ms avg; "
                                "v_mfma_f32_32x32x2f32 contraction fed by the FP64 recursion, priced against the "
-                               "dense FP32 matrix peak)" % (ncol, leg_ms),
+                               "dense FP32 matrix peak)" % (leg_kernel_name(base32, lmax, ncol), ncol, leg_ms),
                      "algorithmic_flops_per_launch": flops},
         "mc_precompute_s": t_mc,
         "autotune": autotune_report(base32),
