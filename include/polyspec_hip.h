@@ -180,6 +180,13 @@ int ps_pixel_products_batch(int precision, int mode, int nbatch, const void *a, 
 int ps_radial_sum(int precision, const void *lm_map, int nr, int lmax, int lmin, const void *f, int nslot, int npol,
                   const double *w, double scale, int accumulate, void *out, void *stream);
 
+/* The per-node terms of that sum, ideal_fisher.integrate_pq_deriv (ideal_fisher.pyx:395-407) as
+ * used for the tauNL weight derivative (tspec_template.py:2385, 2436):
+ * out[r, X, lm] (+)= scale w_r f[l,X,r] lm_map[r, lm] for l >= lmin, else 0; out (nr, npol, nlm),
+ * nr <= 65535. */
+int ps_radial_scatter(int precision, const void *lm_map, int nr, int lmax, int lmin, const void *f, int nslot,
+                      int npol, const double *w, double scale, int accumulate, void *out, void *stream);
+
 /* tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646) as per-L Gram matrices.
  * A, B: (nmu, nr, nlm(Lmaxs)) complex m-major over L = 0..Lmaxs (the transpose
  * of the reference's (nmu, nLM, nr) layout); FLL (Lmaxs+1, nr, nr) = FLLs[nmax_F]. */

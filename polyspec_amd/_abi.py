@@ -43,6 +43,7 @@ SIGNATURES = {
     "ps_mc_accumulate": (_I, [_I, _P, _P, _I, _LL, _P, _P, _P]),
     "ps_pixel_products": (_I, [_I, _I, _P, _P, _P, _P, _LL, _D, _P, _P]),
     "ps_radial_sum": (_I, [_I, _P, _I, _I, _I, _P, _I, _I, _P, _D, _I, _P, _P]),
+    "ps_radial_scatter": (_I, [_I, _P, _I, _I, _I, _P, _I, _I, _P, _D, _I, _P, _P]),
     "ps_tau_sum_n0": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
     "ps_tau_sum_lmajor": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _P]),
     "ps_tau_sum_lmajor_noalloc": (_I, [_I, _P, _P, _LL, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P]),

@@ -458,6 +458,32 @@ __global__ __launch_bounds__(256) void k_tau_gram_mfma(const cplx<T>* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = sum * pref;
 }
 
+// Per-node form of k_radial_sum (ideal_fisher.integrate_pq_deriv, ideal_fisher.pyx:395-407, the
+// weight derivative of the tauNL ideal Fisher): out[r, X, lm] (+)= scale w_r f[l, X, r] lm_map[r, lm]
+// for l >= lmin, else 0.  out: (nr, npol, nlm).  grid (ceil(nlm / 256), nr).
+template <typename T>
+__global__ void k_radial_scatter(const cplx<T>* __restrict__ lmmap, int nr, int lmax, int lmin,
+                                 const T* __restrict__ f, int nslot, int npol, const double* __restrict__ w,
+                                 double scale, int accumulate, cplx<T>* __restrict__ out) {
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    const long lm = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (lm >= nlm) return;
+    const int l = l_of_lm(lm, lmax);
+    const cplx<T> v = lmmap[(long)r * nlm + lm];
+    for (int X = 0; X < npol; X++) {
+        const double fw = l >= lmin ? (double)f[((long)l * nslot + X) * nr + r] * w[r] * scale : 0.0;
+        cplx<T>* o = out + ((long)r * npol + X) * nlm + lm;
+        const double re = (double)v.re * fw, im = (double)v.im * fw;
+        if (accumulate) {
+            o->re += (T)re;
+            o->im += (T)im;
+        } else {
+            *o = cplx<T>{(T)re, (T)im};
+        }
+    }
+}
+
 // ---------------------------------------------------------- outer product
 // ideal_fisher.outer_product (ideal_fisher.pyx:423-459), sym = False:
 // fish[a,b] = Re sum_j [Q0a* Q0b + Q1a* Q1b + 9(Q2a* Q2b + Q3a* Q3b)
@@ -757,6 +783,25 @@ int ps_radial_sum(int precision, const void* lm_map, int nr, int lmax, int lmin,
                            lmin, (const double*)f, nslot, npol, w, scale, accumulate, (cplx<double>*)out);
     else
         PS_REQUIRE(false, "ps_radial_sum: precision must be 32 or 64");
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+int ps_radial_scatter(int precision, const void* lm_map, int nr, int lmax, int lmin, const void* f, int nslot,
+                      int npol, const double* w, double scale, int accumulate, void* out, void* stream) {
+    PS_REQUIRE(lm_map && f && w && out, "ps_radial_scatter: NULL argument");
+    PS_REQUIRE(nr >= 1 && nr <= 65535 && npol >= 1 && npol <= nslot, "ps_radial_scatter: bad nr/npol");
+    const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((unsigned)((nlm + 255) / 256), (unsigned)nr);
+    if (precision == 32)
+        hipLaunchKernelGGL(k_radial_scatter<float>, grid, dim3(256), 0, st, (const cplx<float>*)lm_map, nr, lmax, lmin,
+                           (const float*)f, nslot, npol, w, scale, accumulate, (cplx<float>*)out);
+    else if (precision == 64)
+        hipLaunchKernelGGL(k_radial_scatter<double>, grid, dim3(256), 0, st, (const cplx<double>*)lm_map, nr, lmax,
+                           lmin, (const double*)f, nslot, npol, w, scale, accumulate, (cplx<double>*)out);
+    else
+        PS_REQUIRE(false, "ps_radial_scatter: precision must be 32 or 64");
     PS_LAUNCH_CHECK();
     return 0;
 }

@@ -212,6 +212,25 @@ def radial_sum(lm_map, w, f, lmax, lmin, npol, out=None, scale=1.0, accumulate=F
     return out
 
 
+def radial_scatter(lm_map, w, f, lmax, lmin, npol, out=None, scale=1.0, accumulate=False):
+    """Per-node terms of radial_sum, ideal_fisher.integrate_pq_deriv (ideal_fisher.pyx:395-407):
+    (nr, nlm) -> (nr, npol, nlm)."""
+    _chk(lm_map, f)
+    assert lm_map.dim() == 2 and lm_map.shape[1] == _nlm(lmax), "lm_map must be (nr, nlm(lmax))"
+    nr = lm_map.shape[0]
+    _table(f, lmax, nr, "f")
+    assert npol <= f.shape[1], "npol exceeds the table's slots"
+    w = _dev_f64(w, lm_map.device)
+    assert w.numel() == nr, "w must have nr = %d entries" % nr
+    if out is None:
+        out = torch.zeros((nr, npol, lm_map.shape[1]), dtype=lm_map.dtype, device=lm_map.device)
+    _chk(out)
+    assert out.dtype == lm_map.dtype and out.numel() == nr * npol * lm_map.shape[1], "out must be (nr, npol, nlm(lmax))"
+    check(load().ps_radial_scatter(_prec(lm_map), ptr(lm_map), nr, int(lmax), int(lmin), ptr(f), f.shape[1], int(npol),
+                                   ptr(w), float(scale), int(accumulate), ptr(out), stream_handle()))
+    return out
+
+
 def tau_sum_n0(A, B, FLL, w, Lmaxs, Lmin, Lmax, n1=0, out=None, accumulate=False):
     """tauNL_utils.tau_sum_n0 (tauNL_utils.pyx:604-646); A, B: (nmu, nr, nlm(Lmaxs))."""
     _chk(A, B)

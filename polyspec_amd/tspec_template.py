@@ -207,18 +207,41 @@ class TSpecTemplate:
         return D
 
     def _compute_fisher_derivatives(self, templates, N_fish_optim=None, verb=False):
-        """Contact-template branch of tspec_template.py:3842-3900: {template: (sum D, D)}."""
+        """Fisher derivatives on the current (starting) grid, tspec_template.py:3842-4069 for the
+        local templates.  gNL-loc exactly: (sum D, D).  tauNL-loc from N_fish_optim pairs of
+        GRFs (seeds seed + (1 + ii) 1e9, harmonic, beam deconvolved): (mean score, mean D,
+        TauIdeal), the last holding each realization's maps and full-grid Q4 for the score."""
+        from .tau_ideal import TauIdeal
+        for t in templates:
+            if t not in SUPPORTED:
+                raise Exception("Template %s not implemented!" % t)
         output = {}
         t0 = time.time()
-        for t in templates:
-            if t != 'gNL-loc':
-                raise NotImplementedError("Fisher derivatives are provided for gNL-loc only "
-                                          "(the tauNL Monte-Carlo score is not part of the hip backend)")
+        if 'gNL-loc' in templates:
             if verb:
                 print("\tComputing gNL-loc Fisher matrix derivative exactly")
             D = self._fisher_deriv_gnl_loc(self.quad_weights_1d)
-            output[t] = (np.sum(D), D)
+            output['gNL-loc'] = (np.sum(D), D)
         self.timers['fish_deriv'] += time.time() - t0
+        if 'tauNL-loc' in templates:
+            n_it = 1 if N_fish_optim is None else int(N_fish_optim)
+            ti = TauIdeal(self)
+            score, D = 0., 0.
+            for seed in range(n_it):
+                print("\nComputing Fisher matrix derivatives from realization %d of %d" % (seed + 1, n_it))
+                t0 = time.time()
+                a_maps = [self.base.generate_data(seed=seed + int((1 + ii) * 1e9), output_type='harmonic',
+                                                  deconvolve_beam=True) for ii in range(2)]
+                self.timers['fish_grfs'] += time.time() - t0
+                t0 = time.time()
+                st = ti.add_realization(a_maps)
+                if verb:
+                    print("Computing tauNL-loc Fisher matrix derivative")
+                s1, D1 = ti.derivative(st)
+                score += s1 / n_it
+                D = D + D1 / n_it
+                self.timers['fish_deriv'] += time.time() - t0
+            output['tauNL-loc'] = (score, D, ti)
         return output
 
     @staticmethod
@@ -236,15 +259,15 @@ class TSpecTemplate:
                                     optimize_weights=False, optimize_skip=0, repeat_trials=True, r_guess=[],
                                     weight_guess={}):
         """Radial sampling points and weights by greedy optimisation of the Fisher distance
-        (tspec_template.py:3111-3672), for the contact template gNL-loc: the exact Fisher
-        derivative on the starting grid (one device launch sequence), then the reference's
-        Schur-complement selection on the host.  Returns (r_arr, r_weights) and re-runs the
-        k-space precompute on the selected nodes."""
-        for t in self.templates:
-            if t != 'gNL-loc':
-                raise NotImplementedError("optimize_radial_sampling_1d is provided for gNL-loc only")
-        if optimize_weights:
-            raise NotImplementedError("optimize_weights applies to tauNL templates only")
+        (tspec_template.py:3111-3672).  The Fisher derivative on the starting grid is computed on
+        the device (gNL-loc exactly; tauNL-loc from N_fish_optim GRF pairs), the nodes are then
+        chosen by the reference's Schur-complement selection, templates in the reference's order
+        (tauNL first) and each continuing from the nodes of the previous one.  A trial's score
+        is sum(G) for gNL-loc and, for tauNL-loc, the Monte-Carlo Fisher distance of the trial
+        subset (device work per trial); optimize_weights / optimize_skip refine the tauNL-loc
+        weights by BFGS on that score and its gradient.  Returns (r_arr, r_weights) and re-runs
+        the k-space precompute on the selected nodes."""
+        from scipy.optimize import minimize
         t_init = time.time()
         assert reduce_r > 0, "reduce_r parameter must be positive"
         if reduce_r < 0.5:
@@ -273,43 +296,80 @@ class TSpecTemplate:
         print("# Computing k integrals with fiducial radial grid")
         self._prepare_templates()
 
-        inds_guess = []
+        ordered = [t for t in self.templates if 'tauNL' in t] + [t for t in self.templates if 'tauNL' not in t]
+        inds = []
         for rg in r_guess:
             index = np.where(rg == r_init)[0]
             if len(index) == 0:
                 raise Exception("r_guess was computed with different settings! All points in r_guess must be in r_init.")
-            inds_guess.append(int(index[0]))
+            inds.append(int(index[0]))
+        inds_init = np.arange(self.N_r)
+        inds_guess = list(inds)
         if len(weight_guess.keys()) > 0:
-            assert len(inds_guess) > 0, "Must supply r_guess!"
+            assert len(inds) > 0, "Must supply r_guess!"
+            if verb:
+                print("Starting from a set of %d radial points" % len(inds))
+            for t in ordered:
+                if 'tauNL' in t:
+                    assert t in weight_guess.keys(), "weight_guess does not contain template %s" % t
         if verb:
             print("Computing all Fisher matrix derivatives")
-        derivs = self._compute_fisher_derivatives(self.templates, N_fish_optim=N_fish_optim, verb=verb)
+        derivs = self._compute_fisher_derivatives(ordered, N_fish_optim=N_fish_optim, verb=verb)
         # the starting grid and its tables, kept for diagnostics once the nodes are replaced
         self.start_grid = {'r': r_init.copy(), 'weights': self.quad_weights_1d.copy(),
                            'plXs': self.plLXs[self.nmax].copy(), 'qlXs': self.qlXs.copy(),
+                           'FLL': self.FLLs[self.nmax_F].copy() if hasattr(self, 'FLLs') else None,
                            'fisher_deriv': {t: derivs[t][1] for t in derivs}}
         if not hasattr(self, 'ideal_fisher'):
             self.ideal_fisher = {}
-        inds_init = np.arange(self.N_r)
+        for t in ordered:
+            self.ideal_fisher[t] = float(derivs[t][0])
+        self.optimization_score = {}
         r_weights = {}
-        for template in self.templates:
-            init_score, D = derivs[template]
-            self.ideal_fisher[template] = init_score
+        w_opt = np.zeros(0)
+        for template in ordered:
+            init_score, D = float(derivs[template][0]), derivs[template][1]
+            tau = derivs[template][2] if 'tauNL' in template else None
             if verb:
-                print("\nRunning optimization for template %s\nInitial score: %.2e" % (template, init_score))
-            inds = list(inds_guess)
+                print("\nRunning optimization for template %s%s" %
+                      (template, " (numerically maximizing each step)" if tau is not None and optimize_weights else ""))
+                print("Initial score: %.2e" % init_score)
+
+            def test_inds(sel, score_old, w_vals, G, iit):
+                """_test_inds (3514-3536): the trial's score, with the weight optimisation."""
+                if tau is None:
+                    return float(np.sum(G)), w_vals
+                fun = lambda x, scale: tau.score(sel, x, scale, gradient=True)
+                if optimize_weights:
+                    scale = init_score if score_old == np.inf else score_old
+                    res = minimize(fun, w_vals, method='BFGS', jac=True, options={"xrtol": 1e-6, "maxiter": 10},
+                                   args=(scale,))
+                    w_vals = res.x
+                    return (scale * res.fun if res.fun < 1 else scale), w_vals
+                score = tau.score(sel, w_vals)
+                if (optimize_skip != 0 and iit % optimize_skip == 0 and score / init_score > tolerance and iit > 0
+                        and score < score_old):
+                    res = minimize(fun, w_vals, method='BFGS', jac=True, options={"xrtol": 1e-6, "maxiter": 25},
+                                   args=(score,))
+                    w_vals = res.x
+                    if res.fun < 1:
+                        score = score * res.fun
+                    if verb:
+                        print("Optimization reduced score by %.2f" % res.fun)
+                return score, w_vals
+
             G = None
             if len(inds):
                 G, w_quad, notinds = self._schur(D, inds)
                 if template in weight_guess.keys():
                     n_guess = int(np.sum(weight_guess[template] != 0))
-                    _, w_quad_guess, _ = self._schur(D, inds[:n_guess])
+                    _, w_quad_guess, _ = self._schur(D, inds_guess[:n_guess])
                     w_vals = w_quad + np.concatenate([weight_guess[template][:n_guess] /
-                                                      self.quad_weights_1d[inds[:n_guess]] - w_quad_guess,
+                                                      self.quad_weights_1d[inds_guess[:n_guess]] - w_quad_guess,
                                                       np.zeros(len(inds) - n_guess)])
                 else:
                     w_vals = w_quad
-                score = float(np.sum(G))
+                score, w_vals = test_inds(list(inds), init_score, w_vals, G, 0)
                 if verb:
                     print("Unoptimized relative score: %.2e" % (score / init_score))
             else:
@@ -327,7 +387,7 @@ class TSpecTemplate:
                         inds[-1] = int(next_inds[trial])
                         G, w_quad, notinds = self._schur(D, inds)
                         w_vals = np.append(w_vals_old, 0.) + w_quad - np.append(w_quad_old, 0.)
-                        score = float(np.sum(G))
+                        score, w_vals = test_inds(list(inds), score_old, w_vals, G, iit)
                         if verb:
                             print("Iteration %d, trial %d, relative score: %.2e, old score: %.2e" %
                                   (iteration, trial, score / init_score, score_old / init_score))
@@ -350,17 +410,22 @@ class TSpecTemplate:
                 raise Exception("Failed to converge after %d iterations; this indicates a bug!" % N_fish_optim)
             if verb:
                 print("\nScore threshold met with %d indices" % len(inds))
-            w_opt = np.asarray(w_vals, dtype=np.float64)
-            fish = float(np.sum(np.outer(w_opt, w_opt) * D[np.ix_(inds, inds)]))
+            w_opt = np.asarray(w_vals, dtype=np.float64).copy()
+            if tau is None:
+                fish = float(np.sum(np.outer(w_opt, w_opt) * D[np.ix_(inds, inds)]))
+            else:
+                score, fish = tau.score(list(inds), w_opt, full_score=True)
             if verb:
                 print("Ideal %s Fisher: %.4e (initial), %.4e (optimized). Relative score: %.2e\n" %
                       (template, init_score, fish, score / init_score))
-            self.optimization_score = {template: score / init_score}
+            self.optimization_score[template] = score / init_score
             r_weights[template] = w_opt * self.quad_weights_1d[inds]
+        del derivs
         self.r_arr = r_init[inds]
         self.N_r = len(self.r_arr)
         self.r_weights = {}
-        for template in self.templates:
+        for template in ordered:
+            # weights of the nodes each template selected, zero on nodes added after it
             self.r_weights[template] = np.zeros(len(w_opt))
             self.r_weights[template][:len(r_weights[template])] = r_weights[template]
         if verb:
