@@ -15,7 +15,8 @@ HEALPix grid (Nside_lo, default 128), P^dag its adjoint times the coarse pixel
 area (a quadrature inverse of P for l <= 2 Nside_lo), g the coarse mask clipped
 to [0, 1] (inside the mask the data carry no information and a local correction
 only harms the conditioning), and f_theta a 4-layer, 64-channel
-9-point-stencil convolutional network over the coarse pixels.  Its input
+9-point-stencil convolutional network over the coarse pixels (each pixel and its 8 HEALPix
+neighbours, healpix.stencil9).  Its input
 channels are P r, P Sigma r and P C r, each divided by its rms, plus two static
 channels, the coarse mask and the normalised log noise variance; its output is
 in units of s = rms(P Sigma r), so M(a r) = a M(r) for a > 0.
@@ -41,59 +42,8 @@ import torch
 
 from . import kernels as K
 from ._abi import ShtPlan, check, load, ptr, stream_handle
+from .healpix import stencil9
 from .weighting import CGD
-
-
-# ---------------------------------------------------------------- stencil
-def ring_stencil(nside):
-    """9-point stencil of every RING pixel: itself, its two ring neighbours, and the
-    three pixels nearest in longitude on each adjacent ring (-1 where a pole ring has
-    no ring beyond it).  Returns (nbr (npix, 9), rev (npix, rw)) int32; rev lists
-    p*9 + k for every slot (p, k) that reads the pixel (-1 padding)."""
-    N = nside
-    nr = 4 * N - 1
-    starts, counts, phi0 = [], [], []
-    s = 0
-    for i in range(1, nr + 1):
-        ir = min(i, 4 * N - i)
-        if ir < N:
-            n, p0 = 4 * ir, np.pi / (4 * ir)
-        else:
-            n = 4 * N
-            p0 = np.pi / (4 * N) if ((i - N) & 1) == 0 else 0.
-        starts.append(s)
-        counts.append(n)
-        phi0.append(p0)
-        s += n
-    npix = s
-    nbr = -np.ones((npix, 9), dtype=np.int64)
-    for ri in range(nr):
-        n, st, p0 = counts[ri], starts[ri], phi0[ri]
-        j = np.arange(n)
-        phi = p0 + 2 * np.pi * j / n
-        pix = st + j
-        nbr[pix, 0] = pix
-        nbr[pix, 1] = st + (j - 1) % n
-        nbr[pix, 2] = st + (j + 1) % n
-        for slot, rj in ((3, ri - 1), (6, ri + 1)):
-            if rj < 0 or rj >= nr:
-                continue
-            m, st2, q0 = counts[rj], starts[rj], phi0[rj]
-            c = np.rint((phi - q0) * m / (2 * np.pi)).astype(np.int64)
-            for d in range(3):
-                nbr[pix, slot + d] = st2 + (c - 1 + d) % m
-    flat = nbr.reshape(-1)
-    ok = flat >= 0
-    src = np.nonzero(ok)[0]
-    dst = flat[ok]
-    order = np.argsort(dst, kind='stable')
-    dst_s, src_s = dst[order], src[order]
-    cnt = np.bincount(dst_s, minlength=npix)
-    rw = int(cnt.max())
-    rev = -np.ones((npix, rw), dtype=np.int64)
-    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    rev[dst_s, np.arange(len(dst_s)) - first[dst_s]] = src_s
-    return nbr.astype(np.int32), rev.astype(np.int32)
 
 
 # --------------------------------------------------------------- network
@@ -252,7 +202,7 @@ class LearnedPreconditioner:
             self.p2 = ShtPlan(nside_lo, self.Llo, 2, self.base.precision, autotune=tune) if self.nf == 3 else None
         self.npix_lo = 12 * self.nside_lo ** 2
         self.sel = torch.from_numpy(lm_select(self.L, self.Llo)).to(dev)
-        nbr, rev = ring_stencil(self.nside_lo)
+        nbr, rev = stencil9(self.nside_lo)
         self.nin = 4 * self.nf + 2
         ch = [self.nin] + [channels] * (nlayers - 1) + [self.nf]
         # static channels: coarse mask and normalised log noise variance (harmonic downgrades)

@@ -21,10 +21,11 @@ def _params(rng, ch):
 @pytest.mark.parametrize("nside,ch,B,gated", [(4, (5, 8, 8, 3), 1, False), (8, (11, 64, 64, 64, 3), 2, True)])
 def test_net_forward_backward_match_oracle(gpu, nside, ch, B, gated):
     import torch
-    from polyspec_amd.learned import StencilNet, ring_stencil
+    from polyspec_amd.healpix import stencil9
+    from polyspec_amd.learned import StencilNet
     print(f"net nside={nside} ch={ch} B={B}", flush=True)
     rng = np.random.default_rng(4)
-    nbr, rev = ring_stencil(nside)
+    nbr, rev = stencil9(nside)
     params = _params(rng, ch)
     X = rng.standard_normal((B, nbr.shape[0], ch[0]))
     T = rng.standard_normal((B, nbr.shape[0], ch[-1]))
@@ -97,8 +98,9 @@ def test_training_reduces_loss(gpu):
     """A few hundred Adam steps fit a fixed linear stencil map (the target is a 9-point
     smoothing of the first input channel): the relative loss falls by an order of magnitude."""
     import torch
-    from polyspec_amd.learned import StencilNet, ring_stencil
-    nbr, rev = ring_stencil(8)
+    from polyspec_amd.healpix import stencil9
+    from polyspec_amd.learned import StencilNet
+    nbr, rev = stencil9(8)
     rng = np.random.default_rng(3)
     X = rng.standard_normal((16, nbr.shape[0], 5))
     Wt = np.zeros((45, 1))
@@ -156,13 +158,13 @@ def _solve(W, d, precond=None, nstep=400, thresh=1e-6):
     return cg
 
 
-def test_learned_preconditioner_gain(gpu):
-    """Train on exact (A p, p) pairs of plain solves, then measure on unseen data.
-    The gain is measured, with bounds the result must keep:
-    - the network's held-out coarse error relative to Sigma alone (|f - T|^2 / |T|^2,
-      T = g P(A^-1 r - Sigma r) / s) falls well below 1;
-    - the flexible CG, judged by the reference statistic, reaches 1e-6 within a few
-      iterations of plain CG, and its solution passes the same judge."""
+def test_learned_preconditioner_measured(gpu):
+    """Train on exact (A p, p) pairs of plain solves, then measure on unseen data, holding the
+    measured outcome (Nside 64, Nside_lo 32): the network's held-out coarse error relative to
+    Sigma alone (|f - T|^2 / |T|^2, T = g P(A^-1 r - Sigma r) / s) falls from 1.01 to 0.33, and
+    the flexible CG, judged by the reference statistic, needs no more iterations than plain
+    harmonic CG (25 and 25 measured) -- but no fewer: the learned preconditioner does not gain
+    iterations here, and costs more per iteration."""
     import torch
     from polyspec_amd.learned import LearnedPreconditioner
     print("learned gain nside=64 nside_lo=32", flush=True)
@@ -183,7 +185,7 @@ def test_learned_preconditioner_gain(gpu):
     assert 0.9 < before < 1.1
     assert after < 0.6
     assert plain.err < 1e-6 and learned.err < 1e-6
-    assert learned.i <= plain.i + 10
+    assert learned.i <= plain.i
     # the learned solve's answer passes the plain judge (the statistic itself is the judge)
     assert learned.history[-1][1] < 1e-6
 

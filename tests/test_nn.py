@@ -7,18 +7,19 @@ import pytest
 
 import oracle
 from oracle import nn as onn
-from polyspec_amd.learned import lm_select, ring_stencil
+from polyspec_amd.healpix import stencil9
+from polyspec_amd.learned import lm_select
 
 
 @pytest.mark.parametrize("nside", [2, 4, 8])
 def test_stencil_geometry(nside):
-    nbr, rev = ring_stencil(nside)
+    nbr, rev = stencil9(nside)
     npix = 12 * nside ** 2
     assert nbr.shape == (npix, 9) and (nbr[:, 0] == np.arange(npix)).all()
     th, ph = oracle.pix_angles(nside)
     v = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
     ok = nbr >= 0
-    # only the two pole rings miss the ring beyond them: 2 rings x 4 pixels x 3 slots
+    # seven neighbours at the 8 three-face vertices (3 pixels each)
     assert (~ok).sum() == 24
     d = np.arccos(np.clip(np.einsum('pi,pki->pk', v, v[np.maximum(nbr, 0)]), -1, 1))
     assert d[ok].max() < 2.5 * np.sqrt(4 * np.pi / npix)
@@ -33,7 +34,7 @@ def test_stencil_geometry(nside):
 
 
 def test_reverse_table():
-    nbr, rev = ring_stencil(4)
+    nbr, rev = stencil9(4)
     slots = [[] for _ in range(nbr.shape[0])]
     for p in range(nbr.shape[0]):
         for k in range(9):
@@ -61,7 +62,7 @@ def _net(rng, ch, scale=0.3):
 @pytest.mark.parametrize("gated", [False, True])
 def test_oracle_gradients_finite_difference(gated):
     rng = np.random.default_rng(1)
-    nbr, _ = ring_stencil(2)
+    nbr, _ = stencil9(2)
     ch = [4, 5, 5, 2]
     params = _net(rng, ch)
     X = rng.standard_normal((2, nbr.shape[0], ch[0]))
@@ -87,7 +88,7 @@ def test_oracle_gradients_finite_difference(gated):
 def test_oracle_im2col_is_stencil_sum():
     """A layer with one-hot weights returns the neighbour values themselves."""
     rng = np.random.default_rng(2)
-    nbr, _ = ring_stencil(2)
+    nbr, _ = stencil9(2)
     X = rng.standard_normal((1, nbr.shape[0], 1))
     for k in range(9):
         W = np.zeros((9, 1))
