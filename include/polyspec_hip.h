@@ -140,6 +140,11 @@ int ps_weight_alm_lmajor(int precision, const void *f, int nslot, int nr, const 
 int ps_weight_alm_pq_lmajor(int precision, const void *fP, const void *fQ, int nslot, int nr, const void *h,
                             int npol, int lmax, int lmin, double wP, double wQ, int ld, int col_offset,
                             void *out_t, void *stream);
+/* The same weightings with the columns interleaved: P_r at col_offset + 2r, Q_r at
+ * col_offset + 2r + 1 (the layout ps_sht_ring_gnl reads). */
+int ps_weight_alm_pq_interleaved_lmajor(int precision, const void *fP, const void *fQ, int nslot, int nr,
+                                        const void *h, int npol, int lmax, int lmin, double wP, double wQ, int ld,
+                                        int col_offset, void *out_t, void *stream);
 
 /* c4 * tauNL_utils.gnl_loc_sum (tauNL_utils.pyx:260-270) plus, when U and V are
  * given, c2 * sum_r w_r sum_p (P^2 U + P Q V): with U = sum_s P_s Q_s and
@@ -149,6 +154,19 @@ int ps_weight_alm_pq_lmajor(int precision, const void *fP, const void *fQ, int n
 int ps_gnl_loc_fused_sum(int precision, const void *P, const void *Q, const void *U, const void *V,
                          const double *w, int nr, long long npix, double c4, double c2, double *scratch,
                          double *out, int accumulate, void *stream);
+
+/* The same sum without the maps: the ring stage of the numerator's synthesis fused with it.
+ * phase holds the Legendre-stage output (ps_sht_legendre_synthesis_lmajor) of 2*nr columns
+ * interleaved as (P_r, Q_r) (ps_weight_alm_pq_interleaved_lmajor); *out (added when
+ * accumulate) = c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] over the maps
+ * that ps_sht_ring_synthesis would produce (tspec_template.py:1391 and 1565-1575 through
+ * tauNL_utils.pyx:260-285), each ring block adding its pixels in FP64, fixed order.  FP32 spin-0
+ * plans; work: ps_sht_ring_gnl_work_bytes bytes, which is -1 when no staged ring kernel fits the
+ * plan (then use ps_sht_ring_synthesis and ps_gnl_loc_fused_sum). */
+long long ps_sht_ring_gnl_work_bytes(const ps_sht_plan *plan, int nr);
+int ps_sht_ring_gnl(const ps_sht_plan *plan, int nr, const void *phase, const double *U, const double *V,
+                    const double *w, double c4, double c2, void *work, long long work_bytes, double *out,
+                    int accumulate, void *stream);
 
 /* tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285). */
 int ps_gnl_loc_disc_sum(int precision, const void *P1, const void *P2, const void *Q1, const void *Q2,

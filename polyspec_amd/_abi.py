@@ -38,6 +38,9 @@ SIGNATURES = {
     "ps_reduce_scratch_bytes": (_LL, []),
     "ps_weight_alm_lmajor": (_I, [_I, _P, _I, _I, _P, _I, _I, _I, _D, _I, _I, _P, _P]),
     "ps_weight_alm_pq_lmajor": (_I, [_I, _P, _P, _I, _I, _P, _I, _I, _I, _D, _D, _I, _I, _P, _P]),
+    "ps_weight_alm_pq_interleaved_lmajor": (_I, [_I, _P, _P, _I, _I, _P, _I, _I, _I, _D, _D, _I, _I, _P, _P]),
+    "ps_sht_ring_gnl_work_bytes": (_LL, [_P, _I]),
+    "ps_sht_ring_gnl": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _P, _LL, _P, _I, _P]),
     "ps_gnl_loc_fused_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _D, _D, _P, _P, _I, _P]),
     "ps_gnl_loc_disc_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _P, _P, _I, _P]),
     "ps_mc_accumulate": (_I, [_I, _P, _P, _I, _LL, _P, _P, _P]),
@@ -297,6 +300,38 @@ class ShtPlan:
         if out is None:
             out = torch.empty((nmaps, self.ncomp, self.npix), dtype=self.rdtype, device=phase.device)
         check(load().ps_sht_ring_synthesis(self._h, int(nmaps), ptr(phase), ptr(out), stream_handle()))
+        return out
+
+    def ring_gnl_work_bytes(self, nr):
+        """Bytes of the partials buffer of ring_gnl for nr radial nodes, or -1 when no staged ring
+        kernel fits this plan (FP32 spin-0 plans only)."""
+        if self.precision != 32 or self.spin != 0:
+            return -1
+        n = int(load().ps_sht_ring_gnl_work_bytes(self._h, int(nr)))
+        return n if n >= 0 else -1
+
+    def ring_gnl(self, phase, nr, w, c4, U=None, V=None, c2=0.0, out=None, accumulate=False, ws=None):
+        """Ring stage fused with the gNL-loc sum: phases of 2 nr interleaved (P_r, Q_r) columns ->
+        c4 sum_r w_r sum_p P^3 Q [+ c2 sum_r w_r sum_p (P^2 U + P Q V)] as an FP64 device scalar."""
+        assert (U is None) == (V is None), "U and V go together"
+        for t in (U, V):
+            if t is not None:
+                require_device(t, torch.float64, "U/V")
+                assert t.numel() == nr * self.npix, "U and V must be (nr, npix)"
+        w = w if isinstance(w, torch.Tensor) else torch.as_tensor(w)
+        w = w.to(self.device, torch.float64).contiguous()
+        assert w.numel() == nr, "w must have nr entries"
+        nb = self.ring_gnl_work_bytes(nr)
+        if nb < 0:
+            raise RuntimeError("ring_gnl: no staged ring kernel fits this plan")
+        if ws is None:
+            ws = workspace(nb, self.device)
+        assert ws.numel() >= nb, "workspace too small"
+        if out is None:
+            out = torch.zeros((), dtype=torch.float64, device=self.device)
+        check(load().ps_sht_ring_gnl(self._h, int(nr), ptr(phase), ptr(U) if U is not None else None,
+                                     ptr(V) if V is not None else None, ptr(w), float(c4), float(c2), ptr(ws), nb,
+                                     ptr(out), int(accumulate), stream_handle()))
         return out
 
     def adjoint(self, maps, scale, out=None, ws=None):

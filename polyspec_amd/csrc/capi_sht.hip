@@ -153,6 +153,21 @@ int ps_sht_ring_synthesis(const ps_sht_plan* plan, int nmaps, const void* phase,
     return sht_ring_synthesis(plan->p, nmaps, phase, map, plan->p.npix, (hipStream_t)stream);
 }
 
+long long ps_sht_ring_gnl_work_bytes(const ps_sht_plan* plan, int nr) {
+    PS_REQUIRE(plan && nr >= 1, "ps_sht_ring_gnl_work_bytes: bad argument");
+    const long n = sht_ring_gnl_parts(plan->p, nr);
+    PS_REQUIRE(n >= 0, "ps_sht_ring_gnl_work_bytes: no staged ring kernel fits this plan");
+    return (long long)n * (long long)sizeof(double);
+}
+
+int ps_sht_ring_gnl(const ps_sht_plan* plan, int nr, const void* phase, const double* U, const double* V,
+                    const double* w, double c4, double c2, void* work, long long work_bytes, double* out,
+                    int accumulate, void* stream) {
+    PS_REQUIRE(plan && phase && w && work && out && nr >= 1, "ps_sht_ring_gnl: bad argument");
+    return sht_ring_gnl(plan->p, nr, phase, U, V, w, c4, c2, (double*)work, (long)(work_bytes / (long long)sizeof(double)),
+                        out, accumulate, (hipStream_t)stream);
+}
+
 int ps_sht_adjoint(const ps_sht_plan* plan, int nmaps, const void* map, void* alm, double scale, void* work,
                    long long work_bytes, void* stream) {
     PS_REQUIRE(plan && alm && map && work, "ps_sht_adjoint: NULL argument");

@@ -93,7 +93,8 @@ __global__ void k_weight_alm_lmajor(const T* __restrict__ f, int nslot, int nr, 
 // is V consecutive columns of one row (V = 4 when nr, ld and col_offset are multiples of 4 and
 // the output is 32-byte aligned: one index decode and one pair of h loads per four columns, and
 // 16-byte stores); every column is computed as in the one-column form, so both agree bit for bit.
-template <typename T, int V>
+// IL: the columns interleaved, P_r at col_offset + 2r and Q_r at col_offset + 2r + 1.
+template <typename T, int V, bool IL>
 __global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __restrict__ fQ, int nslot, int nr,
                                        const cplx<T>* __restrict__ h, int npol, int lmax, int lmin, T wP, T wQ,
                                        int ld, cplx<T>* __restrict__ out, int col_offset) {
@@ -114,9 +115,6 @@ __global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __rest
             g = (int)(i % ng);
         }
         const int c2 = g * V;
-        const bool isQ = c2 >= nr;  // a group never straddles P and Q (nr % V == 0)
-        const int c = isQ ? c2 - nr : c2;
-        const T* f = isQ ? fQ : fP;
         const int l = l_of_lm(lm, lmax);
         cplx<T> v[V];
 #pragma unroll
@@ -124,19 +122,40 @@ __global__ void k_weight_alm_pq_lmajor(const T* __restrict__ fP, const T* __rest
         if (l >= lmin) {
             const cplx<T> h0 = h[lm];
             const cplx<T> h1 = npol > 1 ? h[nlm + lm] : cplx<T>{T(0), T(0)};
-            const T w = isQ ? wQ : wP;
-            const T* f0 = f + ((long)l * nslot + 0) * nr + c;
-            const T* f1 = f0 + nr;
+            if constexpr (IL) {
+                // column c2 + k is P (k even) or Q (k odd) of node (c2 + k) / 2; c2 is even
 #pragma unroll
-            for (int k = 0; k < V; k++) {
-                v[k].re = f0[k] * h0.re;
-                v[k].im = f0[k] * h0.im;
-                if (npol > 1) {
-                    v[k].re += f1[k] * h1.re;
-                    v[k].im += f1[k] * h1.im;
+                for (int k = 0; k < V; k++) {
+                    const bool isQ = ((c2 + k) & 1) != 0;
+                    const T w = isQ ? wQ : wP;
+                    const T* f0 = (isQ ? fQ : fP) + (long)l * nslot * nr + ((c2 + k) >> 1);
+                    v[k].re = f0[0] * h0.re;
+                    v[k].im = f0[0] * h0.im;
+                    if (npol > 1) {
+                        v[k].re += f0[nr] * h1.re;
+                        v[k].im += f0[nr] * h1.im;
+                    }
+                    v[k].re *= w;
+                    v[k].im *= w;
                 }
-                v[k].re *= w;
-                v[k].im *= w;
+            } else {
+                const bool isQ = c2 >= nr;  // a group never straddles P and Q (nr % V == 0)
+                const int c = isQ ? c2 - nr : c2;
+                const T* f = isQ ? fQ : fP;
+                const T w = isQ ? wQ : wP;
+                const T* f0 = f + ((long)l * nslot + 0) * nr + c;
+                const T* f1 = f0 + nr;
+#pragma unroll
+                for (int k = 0; k < V; k++) {
+                    v[k].re = f0[k] * h0.re;
+                    v[k].im = f0[k] * h0.im;
+                    if (npol > 1) {
+                        v[k].re += f1[k] * h1.re;
+                        v[k].im += f1[k] * h1.im;
+                    }
+                    v[k].re *= w;
+                    v[k].im *= w;
+                }
             }
         }
         cplx<T>* o = out + lm * ld + col_offset + c2;
@@ -570,21 +589,23 @@ static int weight_alm_impl(const void* f, int nslot, int nr, const void* h, int 
     return 0;
 }
 
-template <typename T>
+template <typename T, bool IL>
 static int weight_alm_pq_impl(const void* fP, const void* fQ, int nslot, int nr, const void* h, int npol, int lmax,
                               int lmin, double wP, double wQ, int ld, void* out, int col_offset, void* stream) {
     const long nlm = (long)(lmax + 1) * (lmax + 2) / 2;
-    const bool vec = nr % 4 == 0 && ld % 4 == 0 && col_offset % 4 == 0 && (uintptr_t)out % 32 == 0;
+    // four-column groups: within P or within Q (nr % 4 == 0), or two interleaved (P, Q) pairs
+    const bool vec = (IL ? nr % 2 == 0 : nr % 4 == 0) && ld % 4 == 0 && col_offset % 4 == 0 &&
+                     (uintptr_t)out % 32 == 0;
     const long tot = nlm * 2 * nr / (vec ? 4 : 1);
     const long nblk = std::min<long>((tot + 255) / 256, 8192);
     if (vec)
-        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 4>), dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream,
-                           (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)wP, (T)wQ,
-                           ld, (cplx<T>*)out, col_offset);
+        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 4, IL>), dim3((unsigned)nblk), dim3(256), 0,
+                           (hipStream_t)stream, (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax,
+                           lmin, (T)wP, (T)wQ, ld, (cplx<T>*)out, col_offset);
     else
-        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 1>), dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream,
-                           (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax, lmin, (T)wP, (T)wQ,
-                           ld, (cplx<T>*)out, col_offset);
+        hipLaunchKernelGGL((k_weight_alm_pq_lmajor<T, 1, IL>), dim3((unsigned)nblk), dim3(256), 0,
+                           (hipStream_t)stream, (const T*)fP, (const T*)fQ, nslot, nr, (const cplx<T>*)h, npol, lmax,
+                           lmin, (T)wP, (T)wQ, ld, (cplx<T>*)out, col_offset);
     PS_LAUNCH_CHECK();
     return 0;
 }
@@ -684,9 +705,28 @@ int ps_weight_alm_pq_lmajor(int precision, const void* fP, const void* fQ, int n
     PS_REQUIRE(npol >= 1 && npol <= 2 && nslot >= npol, "ps_weight_alm_pq_lmajor: npol must be 1 or 2 and <= nslot");
     PS_REQUIRE(lmin >= 0 && lmax >= lmin, "ps_weight_alm_pq_lmajor: bad l range");
     if (precision == 32)
-        return weight_alm_pq_impl<float>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset, stream);
+        return weight_alm_pq_impl<float, false>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset,
+                                                stream);
     PS_REQUIRE(precision == 64, "ps_weight_alm_pq_lmajor: precision must be 32 or 64");
-    return weight_alm_pq_impl<double>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset, stream);
+    return weight_alm_pq_impl<double, false>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset,
+                                             stream);
+}
+
+int ps_weight_alm_pq_interleaved_lmajor(int precision, const void* fP, const void* fQ, int nslot, int nr,
+                                        const void* h, int npol, int lmax, int lmin, double wP, double wQ, int ld,
+                                        int col_offset, void* out_t, void* stream) {
+    PS_REQUIRE(fP && fQ && h && out_t, "ps_weight_alm_pq_interleaved_lmajor: NULL argument");
+    PS_REQUIRE(nr >= 1 && col_offset >= 0 && col_offset + 2 * nr <= ld,
+               "ps_weight_alm_pq_interleaved_lmajor: bad nr/ld/col_offset");
+    PS_REQUIRE(npol >= 1 && npol <= 2 && nslot >= npol,
+               "ps_weight_alm_pq_interleaved_lmajor: npol must be 1 or 2 and <= nslot");
+    PS_REQUIRE(lmin >= 0 && lmax >= lmin, "ps_weight_alm_pq_interleaved_lmajor: bad l range");
+    if (precision == 32)
+        return weight_alm_pq_impl<float, true>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset,
+                                               stream);
+    PS_REQUIRE(precision == 64, "ps_weight_alm_pq_interleaved_lmajor: precision must be 32 or 64");
+    return weight_alm_pq_impl<double, true>(fP, fQ, nslot, nr, h, npol, lmax, lmin, wP, wQ, ld, out_t, col_offset,
+                                            stream);
 }
 
 int ps_gnl_loc_fused_sum(int precision, const void* P, const void* Q, const void* U, const void* V,

@@ -1958,10 +1958,23 @@ __device__ __forceinline__ void ld_pair(const cplx<T>* p, cplx<T>& a, cplx<T>& b
     }
 }
 
-template <typename T, int NB, int Q, bool BLUE>
+// The gNL-loc sum fused into the ring synthesis (GNL): the columns are interleaved as
+// (P_r, Q_r), so the complex transform of column pair r gives P_r + i Q_r on the ring, and
+// instead of writing the maps each block adds w_r [c4 P^3 Q + c2 (P^2 U_r + P Q V_r)] over its
+// pixels (FP64; U, V may be null) and writes the sum to part[block].
+struct GnlArgs {
+    const double* U = nullptr;  // (nr, npix) Monte-Carlo sums, or null
+    const double* V = nullptr;
+    const double* w = nullptr;  // (nr,) radial weights
+    double c4 = 0.0, c2 = 0.0;
+    double* part = nullptr;     // one partial sum per block of the launch
+    long npix = 0;
+};
+
+template <typename T, int NB, int Q, bool BLUE, bool GNL>
 __global__ __launch_bounds__(256) void k_ring_synth_sg(ShtDev<T> d, const int* __restrict__ ring_list, int ncol,
                                                        const cplx<T>* __restrict__ phase, T* __restrict__ map,
-                                                       long map_stride) {
+                                                       long map_stride, GnlArgs g) {
     extern __shared__ __align__(16) unsigned char sg_smem[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(sg_smem);  // raw rows, then the FFT buffers
     constexpr int G = 2 * NB, IT = sg_items<NB, Q, BLUE>();
@@ -2071,11 +2084,32 @@ __global__ __launch_bounds__(256) void k_ring_synth_sg(ShtDev<T> d, const int* _
     }
     __syncthreads();
     const int nc = min(G, ncol - c0);
+    double gacc = 0.0;
+    // one output value pair (columns c0 + 2q, c0 + 2q + 1) at ring entry k
+    auto emit = [&](int q, int k, T a, T b) {
+        if constexpr (GNL) {
+            if (2 * q + 1 < nc) {
+                // w_r [c4 P^3 Q + c2 (P^2 U + P Q V)] = P^2 (w c4 P Q + w c2 U) + w c2 P Q V
+                const int r = (c0 >> 1) + q;
+                const double wc4 = g.w[r] * g.c4, wc2 = g.w[r] * g.c2;
+                const double P = (double)a, pq = P * (double)b, p2 = P * P;
+                if (g.U) {
+                    const long off = (long)r * g.npix + ri.start;  // uniform across the block
+                    const double* Ur = g.U + off;
+                    const double* Vr = g.V + off;
+                    gacc = fma(p2, fma(wc4, pq, wc2 * Ur[k]), gacc);
+                    gacc = fma(wc2 * pq, Vr[k], gacc);
+                } else {
+                    gacc = fma(p2, wc4 * pq, gacc);
+                }
+            }
+        } else {
+            if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = a;
+            if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = b;
+        }
+    };
     if constexpr (!BLUE) {
-        fft_stk<T, NB, Q>(lds, M, logM, d.tw, d.maxM, true, [&](int q, int k, cplx<T> z) {
-            if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re;
-            if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im;
-        });
+        fft_stk<T, NB, Q>(lds, M, logM, d.tw, d.maxM, true, [&](int q, int k, cplx<T> z) { emit(q, k, z.re, z.im); });
     } else {
         const cplx<T>* V = d.blue + d.blue_nat + ri.voff;
         fft_stk<T, NB, Q>(lds, M, logM, d.tw, d.maxM, false,
@@ -2085,10 +2119,20 @@ __global__ __launch_bounds__(256) void k_ring_synth_sg(ShtDev<T> d, const int* _
         fft_stk<T, NB, Q>(lds, M, logM, d.tw, d.maxM, true, [&](int q, int k, cplx<T> z) {
             if (k < n) {
                 z = cmul(z, d.chirp[ri.coff + k]);
-                if (2 * q < nc) map[(size_t)(c0 + 2 * q) * map_stride + ri.start + k] = z.re * invM;
-                if (2 * q + 1 < nc) map[(size_t)(c0 + 2 * q + 1) * map_stride + ri.start + k] = z.im * invM;
+                emit(q, k, z.re * invM, z.im * invM);
             }
         });
+    }
+    if constexpr (GNL) {
+        // fixed-order block sum: lanes by shuffles, then the four waves
+        double s = gacc;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        double* red = reinterpret_cast<double*>(part);  // the fold's partials are no longer read
+        __syncthreads();
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) g.part[lb] = (red[0] + red[1]) + (red[2] + red[3]);
     }
 }
 
@@ -2939,16 +2983,16 @@ static int launch_ring_synth_nb(const ShtPlan& p, const int* list, int nlist, lo
     return -1;
 }
 
-template <typename T, int NB, int Q, bool BLUE>
+template <typename T, int NB, int Q, bool BLUE, bool GNL = false>
 static int launch_ring_synth_sgq(const ShtPlan& p, const int* list, int nlist, int ncol, const void* phase,
-                                 void* map, long map_stride, hipStream_t st, size_t sh) {
+                                 void* map, long map_stride, hipStream_t st, size_t sh, const GnlArgs& g = GnlArgs()) {
     ShtDev<T> d = dev_view<T>(p);
     if (sh + 256 * sizeof(cplx<T>) > 65536)
-        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth_sg<T, NB, Q, BLUE>,
+        PS_CHECK_HIP(hipFuncSetAttribute((const void*)k_ring_synth_sg<T, NB, Q, BLUE, GNL>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     const int ngrp = (ncol + 2 * NB - 1) / (2 * NB);
-    hipLaunchKernelGGL((k_ring_synth_sg<T, NB, Q, BLUE>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d,
-                       list, ncol, (const cplx<T>*)phase, (T*)map, map_stride);
+    hipLaunchKernelGGL((k_ring_synth_sg<T, NB, Q, BLUE, GNL>), dim3((unsigned)(nlist * ngrp)), dim3(256), sh, st, d,
+                       list, ncol, (const cplx<T>*)phase, (T*)map, map_stride, g);
     PS_LAUNCH_CHECK();
     return 0;
 }
@@ -3486,6 +3530,146 @@ int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t,
     // the spin-0 kernels take batch columns, the spin-s kernels maps
     return legendre_synthesis(p, p.spin == 0 ? ncol : nmap, alm_t, ld, phase, st,
                               shape_of(p, ncol, SHT_STAGE_SYNTH));
+}
+
+// ------------------------------------------------ ring synthesis fused with the gNL-loc sum
+
+// Staged shape of one ring class for the fused sum: the plan's shape when it is a staged one,
+// else the staged rule; -1 when no staged kernel fits the class.
+static int ring_gnl_shape(const ShtPlan& p, int ncol, bool blue) {
+    const long maxM = blue ? p.maxM_blue : p.maxM_pow2;
+    const int id = shape_of(p, ncol, blue ? SHT_STAGE_RING_BLUE : SHT_STAGE_RING_POW2);
+    if (id >= 0 && (id & RING_SG) && ring_nb_valid(p, maxM, ncol, blue, id)) return id;
+    if (!blue && ring_nb_valid(p, maxM, ncol, blue, RING_SG | 2)) return RING_SG | 2;
+    if (ring_nb_valid(p, maxM, ncol, blue, RING_SG | 1)) return RING_SG | 1;
+    return -1;
+}
+
+template <bool BLUE>
+static int launch_ring_gnl_class(const ShtPlan& p, const int* list, int nlist, long maxM, int ncol,
+                                 const void* phase, hipStream_t st, int id, const GnlArgs& g) {
+    const int nb = id & ~RING_SG, q = ring_q_of(maxM);
+    const size_t sh = ring_sg_lds(p, maxM, nb);
+    if (nb == 2 && q == 1)
+        return launch_ring_synth_sgq<float, 2, 1, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    if (nb == 2 && q == 2)
+        return launch_ring_synth_sgq<float, 2, 2, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    if (nb == 1 && q == 1)
+        return launch_ring_synth_sgq<float, 1, 1, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    if (nb == 1 && q == 2)
+        return launch_ring_synth_sgq<float, 1, 2, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    if (nb == 1 && q == 4)
+        return launch_ring_synth_sgq<float, 1, 4, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    set_error("fused gNL sum: no staged ring kernel for this ring class");
+    return -1;
+}
+
+// Fixed-order sums of a partials array: each block adds GS_PER consecutive entries (a tree over
+// its threads), then one block adds the block sums.
+constexpr int GS_PER = 2048;
+
+__global__ __launch_bounds__(256) void k_gnl_parts_level1(const double* __restrict__ part, long n,
+                                                          double* __restrict__ out) {
+    __shared__ double sh[256];
+    const long base = (long)blockIdx.x * GS_PER;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < GS_PER / 256; k++) {
+        const long i = base + threadIdx.x + 256 * k;
+        s += i < n ? part[i] : 0.0;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_gnl_parts_level2(const double* __restrict__ part, int n, double* out,
+                                                          int accumulate) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = accumulate ? *out + sh[0] : sh[0];
+}
+
+// Partial sums the fused launch writes (per block of both ring classes, then per level-1 block);
+// -1 when the plan has no staged kernel for one of its ring classes.
+long sht_ring_gnl_parts(const ShtPlan& p, int nr) {
+    const int ncol = 2 * nr;
+    long n = 0;
+    if (p.n_pow2 > 0) {
+        const int id = ring_gnl_shape(p, ncol, false);
+        if (id < 0) return -1;
+        const int nb = id & ~RING_SG;
+        n += (long)p.n_pow2 * ((ncol + 2 * nb - 1) / (2 * nb));
+    }
+    if (p.nrings > p.n_pow2) {
+        const int id = ring_gnl_shape(p, ncol, true);
+        if (id < 0) return -1;
+        const int nb = id & ~RING_SG;
+        n += (long)(p.nrings - p.n_pow2) * ((ncol + 2 * nb - 1) / (2 * nb));
+    }
+    return n + (n + GS_PER - 1) / GS_PER;
+}
+
+int sht_ring_gnl(const ShtPlan& p, int nr, const void* phase, const double* U, const double* V, const double* w,
+                 double c4, double c2, double* part, long nparts, double* out, int accumulate, hipStream_t st) {
+    if (p.prec != 4 || p.spin != 0) {
+        set_error("fused gNL sum: needs an FP32 spin-0 plan");
+        return -1;
+    }
+    if (nr < 1 || (U == nullptr) != (V == nullptr) || !w || !phase || !part || !out) {
+        set_error("fused gNL sum: bad arguments");
+        return -1;
+    }
+    const long need = sht_ring_gnl_parts(p, nr);
+    if (need < 0) {
+        set_error("fused gNL sum: no staged ring kernel fits this plan");
+        return -1;
+    }
+    if (nparts < need) {
+        set_error("fused gNL sum: partials buffer too small");
+        return -1;
+    }
+    const int ncol = 2 * nr;
+    GnlArgs g;
+    g.U = U;
+    g.V = V;
+    g.w = w;
+    g.c4 = c4;
+    g.c2 = c2;
+    g.npix = p.npix;
+    g.part = part;
+    long n = 0;
+    if (p.n_pow2 > 0) {
+        const int id = ring_gnl_shape(p, ncol, false), nb = id & ~RING_SG;
+        const int rc = launch_ring_gnl_class<false>(p, p.d_rlist, p.n_pow2, p.maxM_pow2, ncol, phase, st, id, g);
+        if (rc) return rc;
+        n += (long)p.n_pow2 * ((ncol + 2 * nb - 1) / (2 * nb));
+    }
+    if (p.nrings > p.n_pow2) {
+        const int id = ring_gnl_shape(p, ncol, true), nb = id & ~RING_SG;
+        g.part = part + n;
+        const int rc = launch_ring_gnl_class<true>(p, p.d_rlist + p.n_pow2, p.nrings - p.n_pow2, p.maxM_blue, ncol,
+                                                   phase, st, id, g);
+        if (rc) return rc;
+        n += (long)(p.nrings - p.n_pow2) * ((ncol + 2 * nb - 1) / (2 * nb));
+    }
+    const int nb1 = (int)((n + GS_PER - 1) / GS_PER);
+    hipLaunchKernelGGL(k_gnl_parts_level1, dim3((unsigned)nb1), dim3(256), 0, st, (const double*)part, n, part + n);
+    PS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gnl_parts_level2, dim3(1), dim3(256), 0, st, (const double*)(part + n), nb1, out, accumulate);
+    PS_LAUNCH_CHECK();
+    return 0;
 }
 
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st) {

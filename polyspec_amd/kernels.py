@@ -84,10 +84,11 @@ def weight_alm_lmajor(f, h, lmax, lmin, weight, out_t, col_offset=0, npol=None):
     return out_t
 
 
-def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npol=None):
+def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npol=None, interleaved=False):
     """The P and Q weightings of one field (apply_fl_weights with fP, wP and with fQ, wQ) into the
-    l-major columns col_offset + [0, nr) and col_offset + [nr, 2 nr), one launch.  fP and fQ:
-    (>= lmax+1, nslot, nr) with the same nslot and nr (their l ranges may differ)."""
+    l-major columns col_offset + [0, nr) and col_offset + [nr, 2 nr) (interleaved: P_r at
+    col_offset + 2r, Q_r at col_offset + 2r + 1), one launch.  fP and fQ: (>= lmax+1, nslot, nr)
+    with the same nslot and nr (their l ranges may differ)."""
     _chk(fP, fQ, h, out_t)
     assert fP.shape[1:] == fQ.shape[1:] and fP.dtype == fQ.dtype, "P and Q tables must share their layout"
     assert min(fP.shape[0], fQ.shape[0]) > lmax, "the tables must cover l <= lmax"
@@ -96,9 +97,9 @@ def weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, wP, wQ, out_t, col_offset=0, npo
     assert h.dim() == 2 and h.shape[0] >= npol and h.shape[1] == _nlm(lmax), "h must be (npol, nlm(lmax))"
     assert out_t.dim() == 2 and out_t.shape[0] >= _nlm(lmax) and out_t.shape[1] >= col_offset + 2 * nr, \
         "out_t must be (>= nlm(lmax), >= col_offset + 2 nr)"
-    check(load().ps_weight_alm_pq_lmajor(_prec(h), ptr(fP), ptr(fQ), fP.shape[1], nr, ptr(h), npol, int(lmax),
-                                         int(lmin), float(wP), float(wQ), out_t.shape[1], int(col_offset),
-                                         ptr(out_t), stream_handle()))
+    fn = load().ps_weight_alm_pq_interleaved_lmajor if interleaved else load().ps_weight_alm_pq_lmajor
+    check(fn(_prec(h), ptr(fP), ptr(fQ), fP.shape[1], nr, ptr(h), npol, int(lmax), int(lmin), float(wP), float(wQ),
+             out_t.shape[1], int(col_offset), ptr(out_t), stream_handle()))
     return out_t
 
 

@@ -41,7 +41,8 @@ class TSpecTemplate:
     def __init__(self, base, mask, applySinv, templates, k_arr, Tl_arr, lmin, lmax, Lmin=None, Lmax=None,
                  Lmin_lens=None, Lmax_lens=None, ns=0.96, As=2.1e-9, k_pivot=0.05, r_values=[], r_weights={},
                  rtau_values=[], rtau_weights={}, C_phi=[], C_lens_weight={}, K_coll=0.1, k_coll=0.1, r_star=None,
-                 r_hor=None, batch_sims=True, sim_batch=None, capture_graph=False, sim_memory_gb=None):
+                 r_hor=None, batch_sims=True, sim_batch=None, capture_graph=False, sim_memory_gb=None,
+                 fuse_gnl=True):
         for t in templates:
             if t not in SUPPORTED:
                 raise NotImplementedError("Template %s is not provided by the hip backend (only %s)" % (t, SUPPORTED))
@@ -65,6 +66,10 @@ class TSpecTemplate:
         # terms revisit on every numerator (default: half of the free memory when they are made);
         # simulations beyond it keep only their S^-1-filtered harmonics and are re-synthesised
         self.sim_memory_gb = sim_memory_gb
+        # fuse_gnl: when only the gNL-loc sum of the data's P and Q maps is needed (gNL-loc alone,
+        # batched simulations or no disconnected term) and the FP32 plan has staged ring kernels,
+        # the ring stage of the synthesis adds the sum directly and the maps are never written
+        self.fuse_gnl = fuse_gnl
         self._graph = None
         self._sims_version = 0
         self.kernel_events = None  # set to a list to time the Legendre stage of every filter call
@@ -501,6 +506,49 @@ class TSpecTemplate:
         self.timers['map_transforms'] += time.time() - t0
         return maps.view(len(hs), 2, nr, self.base.Npix)
 
+    def _gnl_fused_ok(self):
+        """Whether the gNL-loc t4 sum can skip the maps (ps_sht_ring_gnl): gNL-loc alone, an FP32
+        plan, and staged ring kernels for both ring classes at this N_r."""
+        key = (self.N_r, self.lmax)
+        if getattr(self, '_gnl_fused_key', None) != key:
+            ok = self.fuse_gnl and self.templates == ['gNL-loc'] and self.base.precision == 32
+            if ok:
+                ok = self.base.plan(self.lmax, 0).ring_gnl_work_bytes(self.N_r) >= 0
+            self._gnl_fused, self._gnl_fused_key = ok, key
+        return self._gnl_fused
+
+    def _filter_gnl(self, h, c4, c2=0.0, P_weight=1. / np.sqrt(4. * np.pi)):
+        """The gNL-loc t4 sum (tspec_template.py:1391, plus the collapsed disconnected term when
+        c2 != 0) of one filtered field as an FP64 device scalar, without its maps: the P and Q
+        weightings interleaved as (P_r, Q_r), the Legendre stage, then the ring stage fused with
+        the pixel sum."""
+        t0 = time.time()
+        nr = self.N_r
+        ncol = 2 * nr
+        buf, plan = self._lmajor_buffer(ncol, self.lmax)
+        K.weight_alm_pq_lmajor(self.fP_d, self.fQ_d, h, self.lmax, self.lmin, P_weight, 1., buf, interleaved=True)
+        ph = self._lmaj.get(('phase', ncol))
+        if ph is None:
+            ph = plan.phase_buffer(ncol)
+            self._lmaj[('phase', ncol)] = ph
+        if self.kernel_events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.legendre_synthesis_lmajor(buf, ncol, ph)
+            e1.record()
+            self.kernel_events.append((ncol, e0, e1))
+        else:
+            plan.legendre_synthesis_lmajor(buf, ncol, ph)
+        ws = self._lmaj.get(('gnl_parts', nr))
+        if ws is None:
+            ws = torch.empty(plan.ring_gnl_work_bytes(nr), dtype=torch.uint8, device=self.base.device)
+            self._lmaj[('gnl_parts', nr)] = ws
+        U, V = (self.U_d, self.V_d) if c2 != 0 else (None, None)
+        out = plan.ring_gnl(ph, nr, self.w_d['gNL-loc'], c4, U, V, c2, ws=ws)
+        self.base.n_SHTs_reverse += ncol
+        self.timers['map_transforms'] += time.time() - t0
+        return out
+
     # ---------------------------------------------------------- tauNL pieces
     def _A_maps(self, P1, Q1, P2=None, Q2=None, out=None):
         """[P Q]_LM of fields 1 = (P1, Q1) and 2 = (P2, Q2) (tspec_template.py:1294-1303, n = 0):
@@ -776,10 +824,12 @@ class TSpecTemplate:
         if graph_ok and not getattr(self.applySinv, 'capturable', False):
             raise Exception("capture_graph needs an applySinv without host synchronisation "
                             "(Weightings.applySinv_ideal); %r is not marked capturable" % (self.applySinv,))
+        # the maps P, Q are read again only by the per-simulation disconnected terms and tauNL-loc
+        need_maps = (include_disconnected_term and not collapsed) or 'tauNL-loc' in self.templates
         if graph_ok:
             t4_d, P, Q, A_dd = self._numerator_graph(data)
         else:
-            t4_d, P, Q, A_dd = self._numerator_device(data, input_type, collapsed)
+            t4_d, P, Q, A_dd = self._numerator_device(data, input_type, collapsed, need_maps=need_maps)
         t4 = t4_d.cpu().numpy()
 
         if include_disconnected_term:
@@ -830,14 +880,22 @@ class TSpecTemplate:
         self.timers['numerator'] += time.time() - t_start
         return out
 
-    def _numerator_device(self, data, input_type, collapsed):
+    def _numerator_device(self, data, input_type, collapsed, need_maps=True):
         """The data-dependent device work of a numerator: S^-1, radial filtering, syntheses and
         the t4 sums (with the collapsed Monte-Carlo terms when `collapsed`).  Returns the (nT,)
         FP64 tensor of t4 values and the maps P, Q and A_dd (tauNL-loc) that the remaining
-        terms use; no host synchronisation."""
+        terms use (None when `need_maps` is False and the fused gNL-loc sum ran); no host
+        synchronisation."""
         t0 = time.time()
         h = self._h_dev(self.applySinv(data, input_type=input_type, lmax=self.lmax))
         self.timers['Sinv'] += time.time() - t0
+        if not need_maps and self._gnl_fused_ok():
+            t1 = time.time()
+            c4 = 9. / 100. * 4. * self.base.A_pix * FOURPI15
+            c2 = -54. / 100. * self.base.A_pix * FOURPI15 / self.N_it if collapsed else 0.
+            t4 = self._filter_gnl(h, c4, c2)
+            self.timers['gNL_summation'] += time.time() - t1
+            return t4.reshape(1), None, None, None
         maps = self._filter([h])[0]
         P, Q = maps[0], maps[1]
         t4 = []
@@ -873,12 +931,14 @@ class TSpecTemplate:
         key = (tuple(data.shape), data.dtype, self._sims_version, id(self.applySinv))
         if self._graph is None or self._graph_key != key:
             self._graph = None
-            self._numerator_device(data, 'map', True)  # warm-up: plans, workspaces, scratch
+            # warm-up: plans, workspaces, scratch
+            self._numerator_device(data, 'map', True, need_maps='tauNL-loc' in self.templates)
             torch.cuda.synchronize()
             self._g_in = data.detach().clone()
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._g_out = self._numerator_device(self._g_in, 'map', True)
+                self._g_out = self._numerator_device(self._g_in, 'map', True,
+                                                     need_maps='tauNL-loc' in self.templates)
             self._graph, self._graph_key = g, key
         self._g_in.copy_(data)
         self._graph.replay()

@@ -1,0 +1,122 @@
+"""The gNL-loc sum fused into the ring stage of the synthesis (ps_sht_ring_gnl) against the maps
+it replaces: ring synthesis of the same phases, then ps_gnl_loc_fused_sum.
+
+Both paths compute the maps with the same staged ring kernel, up to the compiler's FP32
+contraction choices in the two instantiations (an ulp here and there), and add the pixels in
+different orders: they agree to 1e-8 of the sum of |terms| (the sums cancel strongly; a missing
+or misplaced term would be off by O(1) of it).  The interleaved weighting is a pure column
+permutation of the split one (bit for bit).  At the estimator level the fused numerator equals
+the unfused one to FP32 rounding: there a ring transform pairs P_r with Q_r instead of P_r with
+P_r+1, which changes the FP32 rounding of every map value (the real and imaginary inputs of a
+complex FFT share their rounding errors), so the two agree to 1e-4 of |t4|, half the FP32
+tolerance that both meet against the FP64 oracle in test_tspec_gpu.py (with fuse_gnl left on,
+its gNL-loc-only FP32 cases run the fused path).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec,nr", [(32, 8), (32, 5), (64, 6)])
+def test_interleaved_weighting_is_a_column_permutation(gpu, prec, nr):
+    from polyspec_amd import kernels as K
+    rng = np.random.default_rng(3 + nr)
+    lmax, lmin, nslot = 40, 2, 3
+    nlm = (lmax + 1) * (lmax + 2) // 2
+    rdt = torch.float32 if prec == 32 else torch.float64
+    cdt = torch.complex64 if prec == 32 else torch.complex128
+    fP = torch.from_numpy(rng.standard_normal((lmax + 1, nslot, nr))).to(gpu, rdt)
+    fQ = torch.from_numpy(rng.standard_normal((lmax + 1, nslot, nr))).to(gpu, rdt)
+    h = torch.from_numpy(rng.standard_normal((2, nlm)) + 1j * rng.standard_normal((2, nlm))).to(gpu, cdt)
+    ld = 4 * ((2 * nr + 4 + 3) // 4)
+    a = torch.zeros((nlm, ld), dtype=cdt, device=gpu)
+    b = torch.zeros((nlm, ld), dtype=cdt, device=gpu)
+    K.weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, 0.3, 1.7, a, col_offset=4)
+    K.weight_alm_pq_lmajor(fP, fQ, h, lmax, lmin, 0.3, 1.7, b, col_offset=4, interleaved=True)
+    assert torch.equal(b[:, 4:4 + 2 * nr:2], a[:, 4:4 + nr])
+    assert torch.equal(b[:, 5:5 + 2 * nr:2], a[:, 4 + nr:4 + 2 * nr])
+    assert not torch.any(b[:, :4]) and not torch.any(b[:, 4 + 2 * nr:])
+
+
+@pytest.mark.parametrize("nside,nr,with_uv", [(64, 5, True), (256, 7, False), (512, 40, True)])
+def test_ring_gnl_matches_maps_then_sum(gpu, nside, nr, with_uv):
+    from polyspec_amd import kernels as K
+    from polyspec_amd._abi import ShtPlan
+    print(f"ring_gnl nside={nside} nr={nr} uv={with_uv}", flush=True)
+    lmax = 2 * nside
+    plan = ShtPlan(nside, lmax, 0, 32)
+    assert plan.ring_gnl_work_bytes(nr) > 0
+    ncol = 2 * nr
+    ld = -(-ncol // plan.lmajor_multiple) * plan.lmajor_multiple
+    rng = np.random.default_rng(nside + nr)
+    l = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
+    a = (rng.standard_normal((plan.nlm, ld)) + 1j * rng.standard_normal((plan.nlm, ld))) / np.maximum(l, 1)[:, None]
+    a[:, ncol:] = 0
+    alm_t = torch.from_numpy(a).to(gpu, torch.complex64)
+    phase = plan.phase_buffer(ncol)
+    plan.legendre_synthesis_lmajor(alm_t, ncol, phase)
+    # the fused sum takes the staged kernel of each ring class; the maps are made with the same ones
+    shapes = plan.shapes(ncol)
+    pw = [c for c in plan.shape_candidates(ncol, "ring_pow2") if c & 16]
+    bl = [c for c in plan.shape_candidates(ncol, "ring_bluestein") if c & 16]
+    shapes[1] = shapes[1] if shapes[1] & 16 else max(pw)
+    shapes[2] = shapes[2] if shapes[2] & 16 else min(bl)
+    plan.set_shapes(ncol, shapes)
+    maps = plan.ring_synthesis(phase, ncol).view(nr, 2, plan.npix)
+    P, Q = maps[:, 0].contiguous(), maps[:, 1].contiguous()
+    w = torch.from_numpy(rng.uniform(0.5, 1.5, nr)).to(gpu)
+    c4, c2 = 0.7, -1.3
+    U = V = None
+    if with_uv:
+        U = torch.from_numpy(rng.standard_normal((nr, plan.npix))).to(gpu)
+        V = torch.from_numpy(rng.uniform(0.0, 2.0, (nr, plan.npix))).to(gpu)
+    ref = float(K.gnl_loc_fused_sum(P, Q, w, c4, U, V, c2 if with_uv else 0.0))
+    got = float(plan.ring_gnl(phase, nr, w, c4, U, V, c2 if with_uv else 0.0))
+    # the sums cancel: compare on the scale of sum |term| (FP64 rounding of a reordered sum)
+    p, q = P.double(), Q.double()
+    terms = c4 * p ** 3 * q
+    if with_uv:
+        terms = terms + c2 * (p * p * U + p * q * V)
+    scale = float((w[:, None] * terms).abs().sum())
+    assert abs(got - ref) <= 1e-8 * scale, (got, ref, scale)
+    assert abs(got - float((w[:, None] * terms).sum())) <= 1e-8 * scale
+    # accumulate adds to the output; repeats are bit-identical
+    out = torch.full((), 2.5, dtype=torch.float64, device=gpu)
+    plan.ring_gnl(phase, nr, w, c4, U, V, c2 if with_uv else 0.0, out=out, accumulate=True)
+    assert float(out) == 2.5 + got
+    for _ in range(3):
+        assert float(plan.ring_gnl(phase, nr, w, c4, U, V, c2 if with_uv else 0.0)) == got
+
+
+@pytest.mark.parametrize("pol,include_disc", [(True, True), (False, False)])
+def test_fused_numerator_matches_unfused(gpu, pol, include_disc):
+    import oracle
+    from estimator_setup import grf_alm, radial_nodes, spectra, transfer
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.tspec_template import TSpecTemplate
+    from polyspec_amd.weighting import Weightings
+    nside, lmax, nr = 64, 128, 6
+    raw, tot, beam = spectra(nside, pol)
+    # built-in launch shapes: both paths then compute the same FP32 map values
+    base = PolySpec(nside, dict(tot), beam=beam, pol=pol, precision=32, autotune=False)
+    mask = np.ones(12 * nside ** 2)
+    W = Weightings(base, mask, dict(raw))
+    k, Tl = transfer(lmax, pol, 60)
+    r, wr = radial_nodes(nr)
+    rng = np.random.default_rng(11)
+    L = 3 * nside - 1
+    data = oracle.to_map(grf_alm(rng, tot, L, pol), nside, L)
+    sims = [(grf_alm(rng, tot, lmax, pol), grf_alm(rng, tot, lmax, pol)) for _ in range(3)]
+    res = []
+    for fuse in (True, False):
+        tm = TSpecTemplate(base, mask, W.applySinv_ideal, ['gNL-loc'], k, Tl, 2, lmax, r_values=r,
+                           r_weights={'gNL-loc': wr}, fuse_gnl=fuse)
+        assert tm._gnl_fused_ok() == fuse
+        tm.load_sims(lambda ii: sims[ii], len(sims), input_type='harmonic')
+        res.append(tm.Tl_numerator(data, include_disconnected_term=include_disc))
+    # on the scale of the t4 term, which the disconnected terms largely cancel (as in the
+    # oracle tests)
+    t4 = tm.Tl_numerator(data, include_disconnected_term=False)
+    assert abs(res[0][0] - res[1][0]) <= 1e-4 * abs(t4[0]), (res, t4)
