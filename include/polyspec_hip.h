@@ -403,6 +403,15 @@ int ps_nn_adam(long long n, float *p, const float *g, float *m, float *v, float 
 int ps_nn_mse(long long per_sample, int B, const float *Y, const float *T, const double *scale, float *dY,
               double *work, double *loss, void *stream);
 
+/* Inference form of one stencil layer on bf16 matrix cores: Y = act(im2col9(X) W + b), times gate[p]
+ * per pixel when gate is given, with the gather fused into the operand loads and FP32
+ * accumulation.  X: bf16 (B npix, cin) with cin = 16, 32 or 64 (pad the channels with zeros);
+ * Wp: W in bf16 as (32 ceil(cout / 32), 9 cin) row-major (element [n][k cin + c] = W[k cin + c, n],
+ * zero rows for padded channels); cout <= 64; Y: bf16 (y_bf16 = 1) or FP32 (B npix, cout).  No
+ * counterpart in the reference (the learned preconditioner is new). */
+int ps_nn_conv9_bf16(const void *X, const int *nbr, int B, int npix, int cin, const void *Wp, const float *bias,
+                     int cout, int relu, const float *gate, void *Y, int y_bf16, void *stream);
+
 /* Y[b,p,j] = w[p] X[b,p,j] (per-pixel gate; Y == X allowed). */
 int ps_nn_pixel_scale(int B, int npix, int c, const float *w, const float *X, float *Y, void *stream);
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "ps_nn_adam": (_I, [_LL, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _P]),
     "ps_nn_mse": (_I, [_LL, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ps_nn_pixel_scale": (_I, [_I, _I, _I, _P, _P, _P, _P]),
+    "ps_nn_conv9_bf16": (_I, [_P, _P, _I, _I, _I, _P, _P, _I, _I, _P, _P, _I, _P]),
     "ps_bspec_scratch_bytes": (_LL, [_I]),
     "ps_bspec_triple_sums": (_I, [_I, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),
     "ps_bspec_linear_sums": (_I, [_I, _P, _P, _P, _P, _I, _LL, _P, _I, _D, _P, _LL, _P, _I, _P]),

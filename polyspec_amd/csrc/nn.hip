@@ -198,11 +198,121 @@ __global__ void k_pixel_scale(long n, int npix, int c, const float* __restrict__
 
 static unsigned grid1d(long n) { return (unsigned)std::min<long>((n + 255) / 256, 16384); }
 
+// ------------------------------------------------ inference on bf16 matrix cores
+// A stencil layer for inference, Y[r, n] = act(bias[n] + sum_{k < 9, c < cin} X[b npix + nbr(p, k), c]
+// W[k cin + c, n]) (times gate[p] when given), r = b npix + p, on v_mfma_f32_32x32x16_bf16 with
+// FP32 accumulation and the stencil gather fused into the operand loads (no im2col buffer).  X is
+// bf16 with cin = 16 CPH channels per row (the caller pads with zeros); Y is bf16 (the next
+// layer's input) or FP32.  Wp holds the weights in bf16 as (32 NT, 9 cin) row-major.  One wave
+// computes 32 rows x 32 NT columns; lane l supplies row (A) or column (B) l % 32 and the 8
+// consecutive k values 8 (l / 32) .. + 7 of each 16-wide chunk of k, the same k for both
+// operands, so the products pair up whatever order the matrix core takes the 16 k values of a
+// chunk in.  A neighbour's whole row segment (CPH chunks) is one batch of loads, issued a
+// neighbour ahead of its MFMAs.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f_nn __attribute__((ext_vector_type(16)));
+
+template <int NT, int CPH>
+__global__ __launch_bounds__(256) void k_conv9_bf16(const __bf16* __restrict__ X, const int* __restrict__ nbr, int B,
+                                                    int npix, const __bf16* __restrict__ Wp,
+                                                    const float* __restrict__ bias, int cout, int relu,
+                                                    const float* __restrict__ gate, void* __restrict__ Y, int ybf) {
+    constexpr int CIN = 16 * CPH, K = 9 * CIN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long M = (long)B * npix;
+    const long r0 = ((long)blockIdx.x * 4 + wave) * 32;
+    if (r0 >= M) return;  // a wave past the last row (the kernel has no barriers)
+    const int li = lane & 31, h = lane >> 5;
+    const long r = r0 + li;
+    const bool rv = r < M;
+    const long bb = rv ? r / npix : 0;
+    const int p = rv ? (int)(r - bb * npix) : 0;
+    const long rowbase = bb * npix;
+    v16f_nn acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[nt] = v16f_nn{0};
+    v8bf xa[CPH], xn[CPH], wa[CPH][NT], wn[CPH][NT];
+    auto load = [&](int k, v8bf (&x)[CPH], v8bf (&w)[CPH][NT]) {
+        const int q = rv ? nbr[(long)p * 9 + k] : -1;
+        const v8bf* row = reinterpret_cast<const v8bf*>(X + (rowbase + (q >= 0 ? q : 0)) * (long)CIN);
+#pragma unroll
+        for (int c = 0; c < CPH; c++) x[c] = q >= 0 ? row[2 * c + h] : v8bf{0};
+#pragma unroll
+        for (int c = 0; c < CPH; c++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+                w[c][nt] = *reinterpret_cast<const v8bf*>(Wp + (long)(li + 32 * nt) * K + k * CIN + 16 * c + 8 * h);
+    };
+    load(0, xa, wa);
+    for (int k = 0; k < 9; k++) {
+        if (k + 1 < 9) load(k + 1, xn, wn);
+#pragma unroll
+        for (int c = 0; c < CPH; c++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+                acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[c], wa[c][nt], acc[nt], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CPH; c++) {
+            xa[c] = xn[c];
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) wa[c][nt] = wn[c][nt];
+        }
+    }
+    // accumulator entry i of lane l: row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32 of the tile
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+        const int col = li + 32 * nt;
+        if (col >= cout) continue;
+        const float bc = bias[col];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const long row = r0 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (row < M) {
+                float v = acc[nt][i] + bc;
+                if (relu) v = fmaxf(v, 0.f);
+                if (gate) v *= gate[row % npix];
+                if (ybf)
+                    reinterpret_cast<__bf16*>(Y)[row * cout + col] = (__bf16)v;
+                else
+                    reinterpret_cast<float*>(Y)[row * cout + col] = v;
+            }
+        }
+    }
+}
+
 }  // namespace ps
 
 using namespace ps;
 
 extern "C" {
+
+int ps_nn_conv9_bf16(const void* X, const int* nbr, int B, int npix, int cin, const void* Wp, const float* bias,
+                     int cout, int relu, const float* gate, void* Y, int y_bf16, void* stream) {
+    PS_REQUIRE(X && nbr && Wp && bias && Y, "ps_nn_conv9_bf16: NULL argument");
+    PS_REQUIRE(B >= 1 && npix >= 1 && cout >= 1 && cout <= 64, "ps_nn_conv9_bf16: bad sizes (cout <= 64)");
+    PS_REQUIRE(cin == 16 || cin == 32 || cin == 64, "ps_nn_conv9_bf16: cin must be 16, 32 or 64 (pad with zeros)");
+    PS_REQUIRE((((uintptr_t)Wp | (uintptr_t)X) & 15) == 0, "ps_nn_conv9_bf16: X and Wp must be 16-byte aligned");
+    const long M = (long)B * npix;
+    const unsigned nblk = (unsigned)((M + 127) / 128);
+    const hipStream_t st = (hipStream_t)stream;
+    const __bf16* x = (const __bf16*)X;
+    const __bf16* w = (const __bf16*)Wp;
+#define PS_CONV9(NT, CPH)                                                                                \
+    hipLaunchKernelGGL((k_conv9_bf16<NT, CPH>), dim3(nblk), dim3(256), 0, st, x, nbr, B, npix, w, bias, cout, \
+                       relu, gate, Y, y_bf16)
+    if (cout <= 32) {
+        if (cin == 16) PS_CONV9(1, 1);
+        else if (cin == 32) PS_CONV9(1, 2);
+        else PS_CONV9(1, 4);
+    } else {
+        if (cin == 16) PS_CONV9(2, 1);
+        else if (cin == 32) PS_CONV9(2, 2);
+        else PS_CONV9(2, 4);
+    }
+#undef PS_CONV9
+    PS_LAUNCH_CHECK();
+    return 0;
+}
 
 int ps_nn_im2col9(const float* X, const int* nbr, int B, int npix, int cin, float* Xcol, void* stream) {
     PS_REQUIRE(X && nbr && Xcol && B >= 1 && npix >= 1 && cin >= 1, "ps_nn_im2col9: bad argument");

@@ -53,8 +53,12 @@ class StencilNet:
     Layer i: Y = act(im2col9(X) W_i + b_i), W_i (9 cin, cout) row-major, ReLU on
     every layer but the last."""
 
-    def __init__(self, nbr, rev, channels=(5, 64, 64, 64, 3), seed=0, device=None, gate=None):
+    def __init__(self, nbr, rev, channels=(5, 64, 64, 64, 3), seed=0, device=None, gate=None, bf16_inference=True):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
+        # forward passes that keep nothing for backward (inference) run on bf16 matrix cores with
+        # the stencil gather fused (ps_nn_conv9_bf16); training keeps the FP32 kernels
+        self.bf16_inference = bf16_inference
+        self._packed, self._packed_key, self._set_count = None, None, 0
         # optional per-pixel factor (npix,) applied to the last layer's output
         self.gate = None if gate is None else torch.as_tensor(gate).to(self.device, torch.float32).contiguous()
         self.nbr = torch.from_numpy(np.ascontiguousarray(nbr)).to(self.device)
@@ -80,9 +84,58 @@ class StencilNet:
         for (W, b), (Wn, bn) in zip(self.params, params):
             W.copy_(torch.as_tensor(np.asarray(Wn), dtype=torch.float32))
             b.copy_(torch.as_tensor(np.asarray(bn), dtype=torch.float32))
+        self._set_count += 1
 
-    def get_params(self):
-        return [(W.double().cpu().numpy(), b.double().cpu().numpy()) for W, b in self.params]
+    @staticmethod
+    def _cin_bf16(cin):
+        """Channels of a layer's bf16 input rows: cin rounded up to 16, 32 or 64."""
+        return 16 if cin <= 16 else 32 if cin <= 32 else 64
+
+    def _bf16_weights(self):
+        """Every layer's W in bf16 as ps_nn_conv9_bf16 reads it, (32 ceil(cout/32), 9 cp) with cp the
+        padded input channels (zero rows there); rebuilt whenever a parameter changed (Adam steps,
+        set_params, or an in-place torch operation, seen through the tensors' version counters)."""
+        key = (self.t, self._set_count) + tuple(t._version for pr in self.params for t in pr)
+        if self._packed_key != key:
+            packed = []
+            for i, (W, b) in enumerate(self.params):
+                cin, cout = self.channels[i], self.channels[i + 1]
+                cp = self._cin_bf16(cin)
+                rows = 32 if cout <= 32 else 64
+                Wp = torch.zeros((rows, 9, cp), dtype=torch.float32, device=self.device)
+                Wp[:cout, :, :cin] = W.view(9, cin, cout).permute(2, 0, 1)
+                packed.append(Wp.reshape(rows, 9 * cp).to(torch.bfloat16).contiguous())
+            self._packed, self._packed_key = packed, key
+        return self._packed
+
+    def _bf16_ok(self):
+        """The bf16 kernels take hidden widths of 16, 32 or 64, inputs of at most 64 channels and
+        outputs of at most 64."""
+        ch = self.channels
+        return ch[0] <= 64 and all(c in (16, 32, 64) for c in ch[1:-1]) and ch[-1] <= 64
+
+    def _forward_bf16(self, X):
+        """Inference on bf16 matrix cores: the input padded to 16/32/64 bf16 channels, hidden
+        activations kept in bf16, the last layer's output in FP32."""
+        L = load()
+        B = X.shape[0]
+        c0 = self.channels[0]
+        cp = self._cin_bf16(c0)
+        h = torch.zeros((B, self.npix, cp), dtype=torch.bfloat16, device=self.device)
+        h[..., :c0] = X
+        nl = len(self.params)
+        st = stream_handle()
+        for i, ((W, b), Wp) in enumerate(zip(self.params, self._bf16_weights())):
+            cin, cout = self.channels[i], self.channels[i + 1]
+            last = i == nl - 1
+            cnext = cout if last else self._cin_bf16(cout)
+            assert cnext == cout or last, "hidden widths must be 16, 32 or 64 for the bf16 path"
+            y = torch.empty((B, self.npix, cout), dtype=torch.float32 if last else torch.bfloat16, device=self.device)
+            gate = self.gate if (last and self.gate is not None) else None
+            check(L.ps_nn_conv9_bf16(ptr(h), ptr(self.nbr), B, self.npix, self._cin_bf16(cin), ptr(Wp), ptr(b), cout,
+                                     int(not last), ptr(gate) if gate is not None else None, ptr(y), int(not last), st))
+            h = y
+        return h
 
     def _buf(self, key, n):
         t = self._work.get(key)
@@ -91,11 +144,15 @@ class StencilNet:
             self._work[key] = t
         return t
 
-    def forward(self, X, keep=False):
-        """X (B, npix, C0) float32 -> (B, npix, C_last).  keep=True stores what backward needs."""
+    def forward(self, X, keep=False, bf16=None):
+        """X (B, npix, C0) float32 -> (B, npix, C_last).  keep=True stores what backward needs (FP32
+        kernels); otherwise the bf16 matrix-core form runs unless bf16=False (default:
+        bf16_inference)."""
         L = load()
         assert X.dtype == torch.float32 and X.dim() == 3 and X.shape[1] == self.npix \
             and X.shape[2] == self.channels[0], "StencilNet: bad input shape"
+        if not keep and (self.bf16_inference if bf16 is None else bf16) and self._bf16_ok():
+            return self._forward_bf16(X)
         B = X.shape[0]
         M = B * self.npix
         h = X.contiguous()
@@ -260,18 +317,24 @@ class LearnedPreconditioner:
         return feats.to(torch.float32), ms[1]
 
     def __call__(self, r):
+        return self.apply(r)[0]
+
+    def apply(self, r):
+        """(M r, Sigma r): the preconditioned residual and the harmonic part alone (the judge's
+        Sigma r, so that flexible CG does not apply Sigma twice)."""
         t0 = time.time()
         Sr = K.lmat_apply(r, self.L, mat=self.W.Sigma_l_d)
         if not self.enabled:
-            return Sr
+            return Sr, Sr
         feats, s = self._dyn(r, Sr)
         X = torch.cat([feats, self.static], dim=-1)[None].contiguous()
         Y = self.net.forward(X)[0]
         corr = self.back_project(Y) * s
-        Sr.index_add_(-1, self.sel, corr.to(Sr.dtype))
+        z = Sr.clone()
+        z.index_add_(-1, self.sel, corr.to(Sr.dtype))
         self.n_calls += 1
         self.time += time.time() - t0
-        return Sr
+        return z, Sr
 
     # ----------------------------------------------------------- training
     def collect(self, data_maps, max_pairs=2000, iters=60, verb=False):
@@ -392,22 +455,26 @@ class FlexibleCGD(CGD):
         K.cg_update_xr(self.rz, pAp, self.p, Ap, self.x, self.r)
         del Ap
         self.i += 1
-        judge = K.lmat_apply(self.r, self.L, mat=self.weightings.Sigma_l_d)
-        self.err = float(self.dot(self.r, judge) / self.rz0)
-        rn = float(torch.sqrt(self.dot(self.r, self.r)))
-        self.history.append((self.i, self.err, rn / self.bnorm))
+        # M r and the judge's Sigma r, then every scalar the step decides on in one transfer
+        if hasattr(self.precond, 'apply'):
+            z, judge = self.precond.apply(self.r)
+        else:
+            z, judge = self.precond(self.r), K.lmat_apply(self.r, self.L, mat=self.weightings.Sigma_l_d)
+        rz_new = self.dot(self.r, z)
+        num = rz_new - self.dot(r_old, z)  # Polak-Ribiere (flexible) numerator
+        err, rr, rz_h = torch.stack([self.dot(self.r, judge) / self.rz0, self.dot(self.r, self.r),
+                                     rz_new]).cpu().tolist()
+        self.err = err
+        self.history.append((self.i, self.err, float(np.sqrt(rr)) / self.bnorm))
         if not np.isfinite(self.err):
             raise FloatingPointError("flexible CG: non-finite residual")
         if self.err < self.best:
             self.best, self.nbad = self.err, 0
         else:
             self.nbad += 1
-        z = self.precond(self.r)
-        rz_new = self.dot(self.r, z)
-        if not self._usable(rz_new) or self.nbad >= self.stall:
+        if not self._usable(rz_h) or self.nbad >= self.stall:
             self._fall_back()
             return
-        num = rz_new - self.dot(r_old, z)  # Polak-Ribiere (flexible) numerator
         K.cg_update_p(num, self.rz, z, self.p)
         self.rz = rz_new
         self.z = z

@@ -331,9 +331,10 @@ class CGD:
         self.rz = next_rz
         self.z = z
         self.i += 1
-        self.err = float(self.rz / self.rz0)
-        rn = float(torch.sqrt(self.dot(self.r, self.r)))
-        self.history.append((self.i, self.err, rn / self.bnorm))
+        # the statistic and |r| reach the host in one transfer (one synchronisation per iteration)
+        err, rr = torch.stack([self.rz / self.rz0, self.dot(self.r, self.r)]).cpu().tolist()
+        self.err = err
+        self.history.append((self.i, self.err, float(np.sqrt(rr)) / self.bnorm))
         if self.callback is not None:
             self.callback(self)
 

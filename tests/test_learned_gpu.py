@@ -94,6 +94,37 @@ def test_adam_closed_form(gpu):
     assert np.abs(pd.double().cpu().numpy() - pr).max() < 1e-5
 
 
+@pytest.mark.parametrize("nside,ch,B,gated", [(16, (14, 64, 64, 64, 3), 2, True), (8, (5, 16, 32, 40), 3, False)])
+def test_bf16_inference_matches_fp32_forward(gpu, nside, ch, B, gated):
+    """The inference form on bf16 matrix cores (gather fused into the operand loads) against the
+    FP32 im2col + GEMM forward that the oracle pins: bf16 keeps 8 bits of every input, weight
+    and hidden activation, so after four layers the outputs agree to about 6e-3 rms (measured),
+    checked at 1.5e-2 rms and 5e-2 of the largest output; a wrong operand or accumulator layout
+    would be off by O(1).  Covers inputs padded to 16 channels, hidden widths 16, 32 and 64, one
+    and two 32-column output tiles, missing neighbours (-1 in the stencil) and the gate."""
+    import torch
+    from polyspec_amd.healpix import stencil9
+    from polyspec_amd.learned import StencilNet
+    rng = np.random.default_rng(21 + nside)
+    nbr, rev = stencil9(nside)
+    assert (nbr < 0).any()
+    gate = rng.uniform(0, 1, nbr.shape[0]) if gated else None
+    net = StencilNet(nbr, rev, ch, gate=gate)
+    assert net._bf16_ok()
+    net.set_params(_params(rng, ch))
+    X = torch.from_numpy(rng.standard_normal((B, nbr.shape[0], ch[0]))).float().cuda()
+    y32 = net.forward(X, bf16=False).double()
+    y16 = net.forward(X, bf16=True).double()
+    scale = float(y32.abs().max())
+    assert float((y16 - y32).abs().max()) < 5e-2 * scale
+    assert float(((y16 - y32) ** 2).mean().sqrt()) < 1.5e-2 * float((y32 ** 2).mean().sqrt())
+    # a parameter change (in place) is seen by the packed bf16 weights
+    net.params[-1][0].mul_(-2.0)
+    y32b = net.forward(X, bf16=False).double()
+    y16b = net.forward(X, bf16=True).double()
+    assert float((y16b - y32b).abs().max()) < 5e-2 * float(y32b.abs().max())
+
+
 def test_training_reduces_loss(gpu):
     """A few hundred Adam steps fit a fixed linear stencil map (the target is a 9-point
     smoothing of the first input channel): the relative loss falls by an order of magnitude."""
