@@ -897,8 +897,11 @@ __global__ __launch_bounds__(256, 2) void k_leg_synth_s0_wr(ShtDev<float> d, int
                                                            cplx<float>* __restrict__ phase, int rb0, int nrbs) {
     constexpr int BCM = 16 * NT;
     constexpr int RW = 2 * BCM;
-    __shared__ __align__(16) float sa[2][LCH * RW];
-    __shared__ __align__(16) double sco[2][LCH * 4];  // (alpha, beta, gamma, 0) per l
+    // double buffers as distinct LDS objects: the compiler then sees that the asynchronous copies
+    // into one buffer cannot alias the reads of the other, and does not wait for the copies
+    // before each read (with one [2][...] array it waited at every group of four l)
+    __shared__ __align__(16) float sa0[LCH * RW], sa1[LCH * RW];
+    __shared__ __align__(16) double sco0[LCH * 4], sco1[LCH * 4];  // (alpha, beta, gamma, 0) per l
     __shared__ cplx<float> rot[128];
     const int nchunk = (ncol + BCM - 1) / BCM;
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -929,34 +932,38 @@ __global__ __launch_bounds__(256, 2) void k_leg_synth_s0_wr(ShtDev<float> d, int
         const int nch = (nl + LCH - 1) / LCH;
         static_assert(LCH * RW == 4 * 256 * NT, "NT 16-byte copies per thread per chunk");
         static_assert(LCH * 4 * 8 == 64 * 16, "one wave copies a chunk's coefficients");
-        // chunk c -> buffer buf: thread tid copies the 16-byte pieces q = tid + 256 e of the rows
-        auto stage = [&](int c, int buf) {
+        // chunk c -> buffers (dsa, dsco): thread tid copies the 16-byte pieces q = tid + 256 e of the rows
+        auto stage = [&](int c, float* dsa, double* dsco) {
             const int l0n = m + c * LCH, nn = min(LCH, nl - c * LCH);
 #pragma unroll
             for (int e = 0; e < NT; e++) {
                 const int q = tid + 256 * e, li = min(q / (8 * NT), nn - 1), c4 = q % (8 * NT);
                 const int col = min(chunk * BCM + 2 * c4, ld - 2);  // ld is even
                 const float* src = reinterpret_cast<const float*>(alm_t) + ((size_t)(base + l0n + li) * ld + col) * 2;
-                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&sa[buf][4 * (256 * e + 64 * wave)],
-                                                 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&dsa[4 * (256 * e + 64 * wave)], 16, 0,
+                                                 0);
             }
             if (wave == 0) {
                 const int l = min(l0n + (lane >> 1), d.lmax);
                 const char* src = reinterpret_cast<const char*>(d.coef + 4 * (base + l)) + 16 * (lane & 1);
-                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&sco[buf][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)dsco, 16, 0, 0);
             }
         };
-        stage(0, 0);
+        stage(0, sa0, sco0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         bool fast = __all(r.k == 0);
-        for (int c = 0; c < nch; c++) {
-            const int cur = c & 1;
-            if (c + 1 < nch) stage(c + 1, cur ^ 1);
+        // one chunk from (sab, co); the copies of the next chunk go to the other buffers
+        auto run_chunk = [&](int c, const float* __restrict__ sab, const double* __restrict__ co) {
             const int n = __builtin_amdgcn_readfirstlane(min(LCH, nl - c * LCH));
-            const double* co = sco[cur];
-            const float* sab = sa[cur];
             for (int g = 0; g < n; g += 4) {
+                // the B operands first: their LDS latency then passes under the recursion
+                float bev[NT], bod[NT];
+#pragma unroll
+                for (int cc = 0; cc < NT; cc++) {
+                    bev[cc] = sab[(g + 2 * h) * RW + 32 * cc + li32];
+                    bod[cc] = sab[(g + 1 + 2 * h) * RW + 32 * cc + li32];
+                }
                 double v[4];
                 bool live = true;
                 if (g + 4 <= n) {  // a full group: four steps, no tail tests
@@ -995,14 +1002,20 @@ __global__ __launch_bounds__(256, 2) void k_leg_synth_s0_wr(ShtDev<float> d, int
                 const float aev = (float)(h ? v[2] : v[0]), aod = (float)(h ? v[3] : v[1]);
 #pragma unroll
                 for (int cc = 0; cc < NT; cc++) {
-                    const float bev = sab[(g + 2 * h) * RW + 32 * cc + li32];
-                    const float bod = sab[(g + 1 + 2 * h) * RW + 32 * cc + li32];
-                    Dev[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev, Dev[cc], 0, 0, 0);
-                    Dod[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod, Dod[cc], 0, 0, 0);
+                    Dev[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aev, bev[cc], Dev[cc], 0, 0, 0);
+                    Dod[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(aod, bod[cc], Dod[cc], 0, 0, 0);
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);  // the next chunk's copies have landed in LDS
             __syncthreads();
+        };
+        for (int c = 0; c < nch; c += 2) {
+            if (c + 1 < nch) stage(c + 1, sa1, sco1);
+            run_chunk(c, sa0, sco0);
+            if (c + 1 < nch) {
+                if (c + 2 < nch) stage(c + 2, sa0, sco0);
+                run_chunk(c + 1, sa1, sco1);
+            }
         }
     }
     const bool re = (li32 & 1) == 0;
