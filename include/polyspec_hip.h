@@ -157,16 +157,19 @@ int ps_gnl_loc_fused_sum(int precision, const void *P, const void *Q, const void
 
 /* The same sum without the maps: the ring stage of the numerator's synthesis fused with it.
  * phase holds the Legendre-stage output (ps_sht_legendre_synthesis_lmajor) of 2*nr columns
- * interleaved as (P_r, Q_r) (ps_weight_alm_pq_interleaved_lmajor); *out (added when
- * accumulate) = c4 * gnl_loc_sum(w, P, Q) [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] over the maps
- * that ps_sht_ring_synthesis would produce (tspec_template.py:1391 and 1565-1575 through
- * tauNL_utils.pyx:260-285), each ring block adding its pixels in FP64, fixed order.  FP32 spin-0
- * plans; work: ps_sht_ring_gnl_work_bytes bytes, which is -1 when no staged ring kernel fits the
- * plan (then use ps_sht_ring_synthesis and ps_gnl_loc_fused_sum). */
+ * interleaved as (a_r, b_r) = (s_r P_r, Q_r / s_r) (ps_weight_alm_pq_interleaved_lmajor with the
+ * tables scaled by s_r and 1 / s_r); inv_s2[r] = 1 / s_r^2 (NULL: s_r = 1).  The two columns of a
+ * pair share one complex FFT and so its absolute rounding error: s_r should make their magnitudes
+ * alike.  *out (added when accumulate) = c4 * gnl_loc_sum(w, P, Q)
+ * [+ c2 * sum_r w_r sum_p (P^2 U + P Q V)] over the maps that ps_sht_ring_synthesis would produce
+ * (tspec_template.py:1391 and 1565-1575 through tauNL_utils.pyx:260-285), each ring block adding
+ * its pixels in FP64, fixed order.  FP32 spin-0 plans; work: ps_sht_ring_gnl_work_bytes bytes,
+ * which is -1 when no staged ring kernel fits the plan (then use ps_sht_ring_synthesis and
+ * ps_gnl_loc_fused_sum). */
 long long ps_sht_ring_gnl_work_bytes(const ps_sht_plan *plan, int nr);
 int ps_sht_ring_gnl(const ps_sht_plan *plan, int nr, const void *phase, const double *U, const double *V,
-                    const double *w, double c4, double c2, void *work, long long work_bytes, double *out,
-                    int accumulate, void *stream);
+                    const double *w, const double *inv_s2, double c4, double c2, void *work, long long work_bytes,
+                    double *out, int accumulate, void *stream);
 
 /* tauNL_utils.gnl_loc_disc_sum (tauNL_utils.pyx:275-285). */
 int ps_gnl_loc_disc_sum(int precision, const void *P1, const void *P2, const void *Q1, const void *Q2,

@@ -40,7 +40,7 @@ SIGNATURES = {
     "ps_weight_alm_pq_lmajor": (_I, [_I, _P, _P, _I, _I, _P, _I, _I, _I, _D, _D, _I, _I, _P, _P]),
     "ps_weight_alm_pq_interleaved_lmajor": (_I, [_I, _P, _P, _I, _I, _P, _I, _I, _I, _D, _D, _I, _I, _P, _P]),
     "ps_sht_ring_gnl_work_bytes": (_LL, [_P, _I]),
-    "ps_sht_ring_gnl": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _P, _LL, _P, _I, _P]),
+    "ps_sht_ring_gnl": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _D, _P, _LL, _P, _I, _P]),
     "ps_gnl_loc_fused_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _D, _D, _P, _P, _I, _P]),
     "ps_gnl_loc_disc_sum": (_I, [_I, _P, _P, _P, _P, _P, _I, _LL, _P, _P, _I, _P]),
     "ps_mc_accumulate": (_I, [_I, _P, _P, _I, _LL, _P, _P, _P]),
@@ -311,10 +311,14 @@ class ShtPlan:
         n = int(load().ps_sht_ring_gnl_work_bytes(self._h, int(nr)))
         return n if n >= 0 else -1
 
-    def ring_gnl(self, phase, nr, w, c4, U=None, V=None, c2=0.0, out=None, accumulate=False, ws=None):
-        """Ring stage fused with the gNL-loc sum: phases of 2 nr interleaved (P_r, Q_r) columns ->
-        c4 sum_r w_r sum_p P^3 Q [+ c2 sum_r w_r sum_p (P^2 U + P Q V)] as an FP64 device scalar."""
+    def ring_gnl(self, phase, nr, w, c4, U=None, V=None, c2=0.0, out=None, accumulate=False, ws=None, inv_s2=None):
+        """Ring stage fused with the gNL-loc sum: phases of 2 nr interleaved (s_r P_r, Q_r / s_r)
+        columns (inv_s2 = 1 / s_r^2, None for s_r = 1) -> c4 sum_r w_r sum_p P^3 Q
+        [+ c2 sum_r w_r sum_p (P^2 U + P Q V)] as an FP64 device scalar."""
         assert (U is None) == (V is None), "U and V go together"
+        if inv_s2 is not None:
+            require_device(inv_s2, torch.float64, "inv_s2")
+            assert inv_s2.numel() == nr, "inv_s2 must have nr entries"
         for t in (U, V):
             if t is not None:
                 require_device(t, torch.float64, "U/V")
@@ -331,7 +335,8 @@ class ShtPlan:
         if out is None:
             out = torch.zeros((), dtype=torch.float64, device=self.device)
         check(load().ps_sht_ring_gnl(self._h, int(nr), ptr(phase), ptr(U) if U is not None else None,
-                                     ptr(V) if V is not None else None, ptr(w), float(c4), float(c2), ptr(ws), nb,
+                                     ptr(V) if V is not None else None, ptr(w),
+                                     ptr(inv_s2) if inv_s2 is not None else None, float(c4), float(c2), ptr(ws), nb,
                                      ptr(out), int(accumulate), stream_handle()))
         return out
 

@@ -161,11 +161,11 @@ long long ps_sht_ring_gnl_work_bytes(const ps_sht_plan* plan, int nr) {
 }
 
 int ps_sht_ring_gnl(const ps_sht_plan* plan, int nr, const void* phase, const double* U, const double* V,
-                    const double* w, double c4, double c2, void* work, long long work_bytes, double* out,
-                    int accumulate, void* stream) {
+                    const double* w, const double* inv_s2, double c4, double c2, void* work, long long work_bytes,
+                    double* out, int accumulate, void* stream) {
     PS_REQUIRE(plan && phase && w && work && out && nr >= 1, "ps_sht_ring_gnl: bad argument");
-    return sht_ring_gnl(plan->p, nr, phase, U, V, w, c4, c2, (double*)work, (long)(work_bytes / (long long)sizeof(double)),
-                        out, accumulate, (hipStream_t)stream);
+    return sht_ring_gnl(plan->p, nr, phase, U, V, w, inv_s2, c4, c2, (double*)work,
+                        (long)(work_bytes / (long long)sizeof(double)), out, accumulate, (hipStream_t)stream);
 }
 
 int ps_sht_adjoint(const ps_sht_plan* plan, int nmaps, const void* map, void* alm, double scale, void* work,

@@ -68,13 +68,15 @@ int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, 
 int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
                                   hipStream_t st);
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st);
-// Ring synthesis of 2 nr interleaved columns (P_r, Q_r) fused with the gNL-loc sum
-// sum_r w_r sum_pix [c4 P^3 Q + c2 (P^2 U_r + P Q V_r)] (U, V (nr, npix) FP64 or both null) into
-// *out (FP64, added when accumulate); part: sht_ring_gnl_parts(p, nr) doubles.  FP32 spin-0
-// plans whose ring classes have staged kernels (sht_ring_gnl_parts >= 0).
+// Ring synthesis of 2 nr interleaved columns (a_r, b_r) = (s_r P_r, Q_r / s_r) fused with the
+// gNL-loc sum sum_r w_r sum_pix [c4 P^3 Q + c2 (P^2 U_r + P Q V_r)] (U, V (nr, npix) FP64 or both
+// null; inv_s2 = 1 / s_r^2 or null for s_r = 1) into *out (FP64, added when accumulate); part:
+// sht_ring_gnl_parts(p, nr) doubles.  FP32 spin-0 plans whose ring classes have staged kernels
+// (sht_ring_gnl_parts >= 0).
 long sht_ring_gnl_parts(const ShtPlan& p, int nr);
 int sht_ring_gnl(const ShtPlan& p, int nr, const void* phase, const double* U, const double* V, const double* w,
-                 double c4, double c2, double* part, long nparts, double* out, int accumulate, hipStream_t st);
+                 const double* inv_s2, double c4, double c2, double* part, long nparts, double* out, int accumulate,
+                 hipStream_t st);
 int sht_alm_to_lmajor(const ShtPlan& p, int ncol, const void* alm, long alm_stride, int ld, void* alm_t,
                       hipStream_t st);
 // work: sht_adjoint_work_bytes(p, nmap) bytes (ring phases, then ring-slice partial sums)

@@ -1979,6 +1979,10 @@ struct GnlArgs {
     const double* U = nullptr;  // (nr, npix) Monte-Carlo sums, or null
     const double* V = nullptr;
     const double* w = nullptr;  // (nr,) radial weights
+    // (nr,) or null: the columns hold a = s_r P_r and b = Q_r / s_r (magnitudes balanced, since the
+    // two real columns of one complex FFT share its absolute rounding error), inv_s2 = 1 / s_r^2:
+    // P Q = a b and P^2 = a^2 inv_s2
+    const double* inv_s2 = nullptr;
     double c4 = 0.0, c2 = 0.0;
     double* part = nullptr;     // one partial sum per block of the launch
     long npix = 0;
@@ -2105,7 +2109,7 @@ __global__ __launch_bounds__(256) void k_ring_synth_sg(ShtDev<T> d, const int* _
                 // w_r [c4 P^3 Q + c2 (P^2 U + P Q V)] = P^2 (w c4 P Q + w c2 U) + w c2 P Q V
                 const int r = (c0 >> 1) + q;
                 const double wc4 = g.w[r] * g.c4, wc2 = g.w[r] * g.c2;
-                const double P = (double)a, pq = P * (double)b, p2 = P * P;
+                const double P = (double)a, pq = P * (double)b, p2 = P * P * (g.inv_s2 ? g.inv_s2[r] : 1.0);
                 if (g.U) {
                     const long off = (long)r * g.npix + ri.start;  // uniform across the block
                     const double* Ur = g.U + off;
@@ -3635,7 +3639,8 @@ long sht_ring_gnl_parts(const ShtPlan& p, int nr) {
 }
 
 int sht_ring_gnl(const ShtPlan& p, int nr, const void* phase, const double* U, const double* V, const double* w,
-                 double c4, double c2, double* part, long nparts, double* out, int accumulate, hipStream_t st) {
+                 const double* inv_s2, double c4, double c2, double* part, long nparts, double* out, int accumulate,
+                 hipStream_t st) {
     if (p.prec != 4 || p.spin != 0) {
         set_error("fused gNL sum: needs an FP32 spin-0 plan");
         return -1;
@@ -3658,6 +3663,7 @@ int sht_ring_gnl(const ShtPlan& p, int nr, const void* phase, const double* U, c
     g.U = U;
     g.V = V;
     g.w = w;
+    g.inv_s2 = inv_s2;
     g.c4 = c4;
     g.c2 = c2;
     g.npix = p.npix;

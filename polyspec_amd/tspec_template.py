@@ -171,6 +171,35 @@ class TSpecTemplate:
                            for t in self.templates}
         self.mw_d = torch.from_numpy(1. + (self.base.m_arr[self.base.l_arr <= self.lmax] > 0)).to(dev, rdt)
         self._lmaj = {}
+        self._gnl_tables = None
+
+    def _gnl_balanced_tables(self, P_weight=1. / np.sqrt(4. * np.pi)):
+        """Radial tables for the fused gNL-loc sum: fP s_r, fQ / s_r and 1 / s_r^2 (device).  The
+        fused ring stage takes P_r and Q_r as the two real columns of one complex FFT, whose absolute
+        rounding error they share; Q_r is typically orders of magnitude larger than P_r (p_l carries
+        the primordial power, q_l does not: 700x at Nside 512), which left the FP32 P maps with
+        relative errors of ~1e-4.  s_r^2 = rms(Q_r) / rms(P_r) balances the two: measured on the
+        simulations' maps when they are processed (_collapse_finish), else estimated here from the
+        tables with the power (2l + 1) [C^-1]_XX(l) of S^-1 d for a Gaussian full sky."""
+        if self._gnl_tables is None:
+            npol = 2 if self.pol else 1
+            fP = np.asarray(self.plLXs[self.nmax], dtype=np.float64)[:self.lmax + 1, :npol] * P_weight
+            fQ = np.asarray(self.qlXs, dtype=np.float64)[:self.lmax + 1, :npol]
+            l = np.arange(self.lmax + 1)
+            icl = self.base.inv_Cl_tot_mat
+            pw = np.stack([(2 * l + 1.) * np.abs(icl[X, X, l]) for X in range(npol)], axis=1)  # (l, X)
+            pw[l < self.lmin] = 0.
+            num = np.einsum('lx,lxr->r', pw, fQ ** 2)
+            den = np.einsum('lx,lxr->r', pw, fP ** 2)
+            ok = (num > 0) & (den > 0) & np.isfinite(num) & np.isfinite(den)
+            s2 = np.where(ok, np.sqrt(np.where(ok, num, 1.) / np.where(ok, den, 1.)), 1.)
+            sr = np.sqrt(s2)
+            dev, rdt = self.base.device, self.base.rdtype
+            self._gnl_tables = (
+                torch.from_numpy(np.ascontiguousarray(self.plLXs[self.nmax] * sr)).to(dev, rdt),
+                torch.from_numpy(np.ascontiguousarray(self.qlXs / sr)).to(dev, rdt),
+                torch.from_numpy(1. / s2).to(dev, torch.float64))
+        return self._gnl_tables
 
     # ------------------------------------------------- radial-node optimizer
     def _fisher_deriv_gnl_loc(self, weights, events=None):
@@ -526,7 +555,8 @@ class TSpecTemplate:
         nr = self.N_r
         ncol = 2 * nr
         buf, plan = self._lmajor_buffer(ncol, self.lmax)
-        K.weight_alm_pq_lmajor(self.fP_d, self.fQ_d, h, self.lmax, self.lmin, P_weight, 1., buf, interleaved=True)
+        fPb, fQb, inv_s2 = self._gnl_balanced_tables(P_weight)
+        K.weight_alm_pq_lmajor(fPb, fQb, h, self.lmax, self.lmin, P_weight, 1., buf, interleaved=True)
         ph = self._lmaj.get(('phase', ncol))
         if ph is None:
             ph = plan.phase_buffer(ncol)
@@ -544,7 +574,7 @@ class TSpecTemplate:
             ws = torch.empty(plan.ring_gnl_work_bytes(nr), dtype=torch.uint8, device=self.base.device)
             self._lmaj[('gnl_parts', nr)] = ws
         U, V = (self.U_d, self.V_d) if c2 != 0 else (None, None)
-        out = plan.ring_gnl(ph, nr, self.w_d['gNL-loc'], c4, U, V, c2, ws=ws)
+        out = plan.ring_gnl(ph, nr, self.w_d['gNL-loc'], c4, U, V, c2, ws=ws, inv_s2=inv_s2)
         self.base.n_SHTs_reverse += ncol
         self.timers['map_transforms'] += time.time() - t0
         return out
@@ -622,6 +652,11 @@ class TSpecTemplate:
         self.U_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
         self.V_d = torch.zeros((self.N_r, self.base.Npix), dtype=torch.float64, device=dev)
         self._t0_acc = torch.zeros(len(self.templates), dtype=torch.float64, device=dev)
+        # sum over the simulations of |P_r|^2 and |Q_r|^2 per node: the magnitude balance of the
+        # fused gNL-loc sum's column pairs (_gnl_balanced_tables), measured on maps with the data's
+        # statistics
+        self._pq_pow = torch.zeros((2, self.N_r), dtype=torch.float64, device=dev)
+        self._gnl_tables = None
         # tauNL-loc mean-field shift M (l-major (nlm(nLmax), N_r)): every self-product map A_xx
         # (A_dd, A_aa, A_bb, and sum_s A_ss -> sum_s A_ss - 2 N_it M) enters the tau sums as A_xx - M.
         # For any fixed M the tau(M, .) terms of t4, t2 and t0 cancel exactly (tau is bilinear and
@@ -664,6 +699,8 @@ class TSpecTemplate:
         P = stack[:, 0].contiguous()
         Q = stack[:, 1].contiguous()
         K.mc_accumulate(P, Q, self.U_d, self.V_d)
+        self._pq_pow[0] += torch.linalg.vector_norm(P, dim=(0, 2), dtype=torch.float64) ** 2
+        self._pq_pow[1] += torch.linalg.vector_norm(Q, dim=(0, 2), dtype=torch.float64) ** 2
         for ii, t in enumerate(self.templates):
             if t == 'gNL-loc':
                 for i in range(n):
@@ -696,6 +733,17 @@ class TSpecTemplate:
                 pref[ii] = 1. / 24. * 3 / self.N_it * FOURPI15
         self.t0_num = self._t0_acc.cpu().numpy() * pref
         del self._t0_acc
+        if 'gNL-loc' in self.templates:
+            # balance of the fused sum's (P_r, Q_r) column pairs from the simulations' own maps:
+            # s_r^2 = rms(Q_r) / rms(P_r)
+            pp, qq = self._pq_pow[0], self._pq_pow[1]
+            ok = (pp > 0) & (qq > 0)
+            s2 = torch.where(ok, torch.sqrt(torch.where(ok, qq, 1.) / torch.where(ok, pp, 1.)), torch.ones_like(pp))
+            sr = torch.sqrt(s2)
+            rdt = self.base.rdtype
+            self._gnl_tables = ((self.fP_d.double() * sr).to(rdt).contiguous(),
+                                (self.fQ_d.double() / sr).to(rdt).contiguous(), (1. / s2).contiguous())
+        del self._pq_pow
         if 'tauNL-loc' in self.templates:
             # sum_s A_ss = Y^dag[U], l-major (one column per radial node), shifted: D = sum_s (A_ss - M),
             # formed in FP64 (2 N_it M is exact there for an FP32 M)

@@ -99,10 +99,12 @@ def _cfg2_template(prec, batch, templates, mask, nside=512, lmax=1024, nr=40):
     return base, tm, tot
 
 
-def test_cfg2_numerators_and_fisher_fp32_batched_vs_fp64_reference_loop(gpu):
+@pytest.mark.parametrize("templates", [['gNL-loc', 'tauNL-loc'], ['gNL-loc']])
+def test_cfg2_numerators_and_fisher_fp32_batched_vs_fp64_reference_loop(gpu, templates):
+    """With gNL-loc alone the batched FP32 path takes the fused ring-stage sum (no P/Q maps), whose
+    magnitude balancing of the P and Q columns this pins at the config size."""
     from polyspec_amd.masks import galactic_cut_mask
     nside, lmax, nr, npairs, ndata = 512, 1024, 40, 2, 4
-    templates = ['gNL-loc', 'tauNL-loc']
     print(f"cfg2 numerators nside={nside} lmax={lmax} nr={nr} pairs={npairs}", flush=True)
     mask = galactic_cut_mask(nside)
     base64, tm64, tot = _cfg2_template(64, False, templates, mask)
@@ -120,6 +122,7 @@ def test_cfg2_numerators_and_fisher_fp32_batched_vs_fp64_reference_loop(gpu):
     # in A_dd, A_ss); the batched FP32 path takes its tau sums over A - M, M the simulations' mean
     # field, which leaves the sum unchanged and its terms near sigma
     base32, tm32, _ = _cfg2_template(32, True, templates, mask)
+    assert tm32._gnl_fused_ok() == (templates == ['gNL-loc'])
     tm32.load_sims(lambda ii: (sims[ii][0].float(), sims[ii][1].float()), npairs, input_type='map')
     n32 = np.array([tm32.Tl_numerator(maps[i].float()) for i in range(ndata)])
     f32 = tm32.fisher_contribution_from_grfs(a_maps)

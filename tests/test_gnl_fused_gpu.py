@@ -82,6 +82,15 @@ def test_ring_gnl_matches_maps_then_sum(gpu, nside, nr, with_uv):
     scale = float((w[:, None] * terms).abs().sum())
     assert abs(got - ref) <= 1e-8 * scale, (got, ref, scale)
     assert abs(got - float((w[:, None] * terms).sum())) <= 1e-8 * scale
+    # balanced columns: the transform outputs (a, b) stand for P = a / s, Q = b s, so with
+    # inv_s2 = 1 / s^2 the sum is sum_r w_r [c4 a^3 b / s^2 + c2 (a^2 U / s^2 + a b V)]
+    s_r = torch.from_numpy(rng.uniform(0.1, 10.0, nr)).to(gpu)
+    got_b = float(plan.ring_gnl(phase, nr, w, c4, U, V, c2 if with_uv else 0.0, inv_s2=(1.0 / s_r ** 2).contiguous()))
+    tref = c4 * p ** 3 * q / s_r[:, None] ** 2
+    if with_uv:
+        tref = tref + c2 * (p ** 2 * U / s_r[:, None] ** 2 + p * q * V)
+    scale_b = float((w[:, None] * tref).abs().sum())
+    assert abs(got_b - float((w[:, None] * tref).sum())) <= 1e-8 * scale_b
     # accumulate adds to the output; repeats are bit-identical
     out = torch.full((), 2.5, dtype=torch.float64, device=gpu)
     plan.ring_gnl(phase, nr, w, c4, U, V, c2 if with_uv else 0.0, out=out, accumulate=True)
