@@ -4,17 +4,20 @@
 // (polyspec/base.py:147-151, 172-176, 193, 206, 225, 230-231, 253, 282).
 //
 // Two stages per direction:
-//   Legendre stage  alm <-> ring phases F_m(ring) for m = 0..mmax.  One thread
-//     per north/south ring pair, one block per (m, batch chunk, ring block):
-//     the normalised spin-weighted Legendre recursion runs in registers, the
-//     alm of the chunk are wave-uniform (scalar loads) and the batch is
-//     accumulated in registers.  Seeds are built in FP64 from log2 tables and
-//     carried as scaled value * 2^(64 k) so that sin^m(theta) never underflows.
-//   Ring stage      ring phases <-> pixels: m is folded modulo nphi in LDS, then
-//     an in-LDS radix-2 FFT (nphi a power of two) or a Bluestein chirp-z
-//     transform over a power-of-two length.  Synthesis: one block per (ring,
-//     group of column pairs), two real columns per complex transform; adjoint:
-//     one block per (ring, map).
+//   Legendre stage  alm <-> ring phases F_m(ring) for m = 0..mmax.  The
+//     normalised spin-weighted Legendre recursion runs in FP64 (seeds built from
+//     log2 tables and carried as scaled value * 2^(64 k), so that sin^m(theta)
+//     never underflows); the contraction with the batch columns runs on the
+//     matrix cores (v_mfma_f32_32x32x2_f32) for FP32 batches, with alm rows and
+//     recursion coefficients staged in LDS, or in registers on the vector ALUs
+//     for narrow batches and FP64.
+//   Ring stage      ring phases <-> pixels: m is folded modulo nphi, then a
+//     self-sorting (Stockham) radix-16 FFT runs in LDS with the 16-point DFTs in
+//     registers (nphi a power of two), or a Bluestein chirp-z transform over a
+//     power-of-two length (the polar rings of length 4i).  Two real columns
+//     share one complex transform in both directions.  The synthesis can stage
+//     a block's phase rows in LDS before folding, and can add the gNL-loc pixel
+//     integrand instead of writing the maps.
 //
 // Layouts: synthesis phases in tiles of 16 rings, [ring/16][mmax+1][ring%16][map*ncomp]
 // complex, already rotated by exp(i m phi0(ring)); adjoint phases [map*ncomp][mmax+1][nrings]; maps
