@@ -2726,6 +2726,293 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
     }
 }
 
+// ------------------------------- Legendre adjoints, spin 0 and spin s, FP64 matrix cores
+// The matrix-core adjoints above in FP64, on v_mfma_f64_16x16x4f64 (lane layout as in the
+// Fisher-derivative GEMM: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k, D[i][j] at lane
+// j + 16 (i % 4), register i / 4).  Per tile of 32 values of l each lane writes its ring's FP64
+// recursion values into LDS rows [parity][(l - l0) / 2][ring], the ring index XOR-swizzled by
+// 4 ((l - l0) / 2) so that the row-wise stores and the reads of the A operands (16 rows x 4
+// consecutive rings) spread over the banks; a wave then issues one MFMA per 4 rings and parity,
+// A = lambda (16 values of l of one parity x 4 rings), B = the north/south sums or differences
+// (4 rings x 16 real columns), held in registers for the whole l loop.  The waves' sums meet in
+// LDS; 256-ring-pair slices write partial sums that k_adj_sum_splits adds in a fixed order.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+constexpr int ADJ64_L = 32;  // values of l per tile
+
+// Spin 0: one block = 4 waves = 256 ring pairs (one per lane), one m, 8 complex columns.
+__global__ __launch_bounds__(256) void k_leg_adj_s0_mfma64(ShtDev<double> d, int ncol,
+                                                           const cplx<double>* __restrict__ phase,
+                                                           cplx<double>* __restrict__ out, long out_stride,
+                                                           double osc, int nslice) {
+    // lambda tiles [wave][parity][i][64], then the waves' sums [wave][parity][i][17]
+    __shared__ __align__(16) double T[4 * 2 * 16 * 64];
+    const int nchunk = (ncol + 7) / 8;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = lb % nchunk;
+    const int sl = (lb / nchunk) % nslice;
+    const int m = m_of_q(lb / (nchunk * nslice), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k4 = lane >> 4, j16 = lane & 15;
+    const int wring = sl * ADJM_PAIRS + wave * 64;  // first ring pair of this wave
+    cplx<double>* dst = nslice > 1 ? out + (size_t)sl * ncol * out_stride : out;
+
+    // recursion of ring pair wring + lane
+    const int jr = wring + lane;
+    const int jrc = jr < d.npairs ? jr : d.npairs - 1;
+    const double x = d.cth[jrc];
+    Rec r;
+    rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jrc] + d.seedB[m] * d.l2ch[jrc], d.seedS[m]);
+
+    // B operands: ring pair wring + 4 s + k4, real column j16 (complex column j16 / 2)
+    double bev[16], bod[16];
+    {
+        const int col = chunk * 8 + (j16 >> 1), part = j16 & 1;
+        const double* P = reinterpret_cast<const double*>(phase + (size_t)col * (d.mmax + 1) * d.nrings +
+                                                          (size_t)m * d.nrings);
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const int jb = wring + 4 * s + k4;
+            double pn = 0.0, ps = 0.0;
+            if (jb < d.npairs && col < ncol) {
+                const int rs = d.nrings - 1 - jb;
+                pn = P[2 * jb + part];
+                if (rs != jb) ps = P[2 * rs + part];
+            }
+            bev[s] = pn + ps;
+            bod[s] = pn - ps;
+        }
+    }
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const double* cf4 = d.coef + 4 * base;
+    // a block whose ring pairs are all dead at this m writes zeros without the recursion
+    const bool skip = __syncthreads_and(jr >= d.npairs || d.dead[(size_t)m * d.npairs + jrc]);
+    double* Tw = T + wave * 2048;
+    bool fast = false;
+    for (int l0 = m; l0 <= d.lmax; l0 += ADJ64_L) {
+        const int n = min(ADJ64_L, d.lmax + 1 - l0);
+        v4d_t De = v4d_t{0.0, 0.0, 0.0, 0.0}, Do = v4d_t{0.0, 0.0, 0.0, 0.0};
+        if (!skip) {
+            // lane c holds (alpha, gamma) of l0 + c
+            double pal = 0.0, pga = 0.0;
+            if (lane < n) {
+                pal = cf4[4 * (l0 + lane)];
+                pga = cf4[4 * (l0 + lane) + 2];
+            }
+            if (!fast) fast = __all(r.k == 0);
+            if (fast && n == ADJ64_L) {
+#pragma unroll 4
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = r.f;
+                    rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                }
+            } else if (n == ADJ64_L) {
+                // full tile while some lane still carries a scale exponent
+#pragma unroll 4
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = r.f * r.cf;
+                    rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                    if ((c & 3) == 3) rec_rescale(r);
+                }
+            } else {
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    double v = 0.0;
+                    if (c < n) {
+                        v = fast ? r.f : r.f * r.cf;
+                        rec_fast(r, readlane_f64(pal, c) * x, readlane_f64(pga, c));
+                        if (!fast && (c & 3) == 3) rec_rescale(r);
+                    }
+                    Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = v;
+                }
+            }
+            __syncthreads();  // the wave's tile is complete
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int sw = j16 * 64 + ((4 * s + k4) ^ (4 * j16));
+                De = __builtin_amdgcn_mfma_f64_16x16x4f64(Tw[sw], bev[s], De, 0, 0, 0);
+                Do = __builtin_amdgcn_mfma_f64_16x16x4f64(Tw[1024 + sw], bod[s], Do, 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave has read its tile: reuse the space for the sums
+        // D[i][j] at lane j + 16 (i % 4), register i / 4; i = (l - l0) / 2
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 4 * v + k4;
+            T[((wave * 2) * 16 + i) * 17 + j16] = De[v];
+            T[((wave * 2 + 1) * 16 + i) * 17 + j16] = Do[v];
+        }
+        __syncthreads();
+        {
+            // thread: real column j, l = l0 + 2 (tid % 16) + u
+            const int j = tid >> 4, cc = chunk * 8 + (j >> 1), pp = j & 1;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int ll = (tid & 15) * 2 + u, par = ll & 1, i = ll >> 1;
+                double sum = T[((0 * 2 + par) * 16 + i) * 17 + j];
+                sum += T[((1 * 2 + par) * 16 + i) * 17 + j];
+                sum += T[((2 * 2 + par) * 16 + i) * 17 + j];
+                sum += T[((3 * 2 + par) * 16 + i) * 17 + j];
+                if (ll < n && cc < ncol)
+                    reinterpret_cast<double*>(dst + (size_t)cc * out_stride + base + l0 + ll)[pp] = sum * osc;
+            }
+        }
+        __syncthreads();  // the sums are read before the next tile overwrites them
+    }
+}
+
+// Spin s: as k_leg_adj_sp_mfma (lanes 0-31 of a wave run f+ and lanes 32-63 f- of the wave's
+// 32 ring pairs; D += f+ B+ + f- B-), one block = 8 waves = 256 ring pairs, one m, 4 maps
+// (16 real columns: G re, G im, C re, C im of each map).
+__global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int nmap,
+                                                           const cplx<double>* __restrict__ phase,
+                                                           cplx<double>* __restrict__ out, long out_stride,
+                                                           double osc, int nslice) {
+    // f tiles [wave][sign][parity][i][32], then the waves' sums [wave][parity][i][17]
+    __shared__ __align__(16) double T[8 * 2 * 2 * 16 * 32];
+    const int nchunk = (nmap + 3) / 4;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int chunk = lb % nchunk;
+    const int sl = (lb / nchunk) % nslice;
+    const int m = m_of_q(lb / (nchunk * nslice), d.mmax);
+    const int M1 = d.mmax + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int k4 = lane >> 4, j16 = lane & 15;
+    const int wring = sl * ADJM_SP_PAIRS + wave * 32;  // first ring pair of this wave
+    cplx<double>* dst = nslice > 1 ? out + (size_t)sl * 2 * nmap * out_stride : out;
+
+    // recursion f+ (h = 0) or f- (h = 1) of ring pair wring + li32
+    const int jr = wring + li32;
+    const int jrc = jr < d.npairs ? jr : d.npairs - 1;
+    const double x = d.cth[jrc];
+    const double sgb = h ? -1.0 : 1.0;
+    Rec r;
+    rec_seed(r, d.seedL[h * M1 + m] + d.seedA[h * M1 + m] * d.l2sh[jrc] + d.seedB[h * M1 + m] * d.l2ch[jrc],
+             d.seedS[h * M1 + m]);
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const int lstart = m > d.spin ? m : d.spin;
+    const int p0 = (lstart + m + d.spin) & 1;  // tile rows of parity p0 are the "odd" type
+
+    // B operands of ring pair wring + 4 s + k4, real column j16 = 4 (map) + q
+    double bp0[8], bm0[8], bp1[8], bm1[8];
+    {
+        const int b = chunk * 4 + (j16 >> 2), q = j16 & 3;
+        const size_t pstride = (size_t)M1 * d.nrings;
+        const cplx<double>* PR = phase + (size_t)(2 * b) * pstride + (size_t)m * d.nrings;
+        const cplx<double>* PI = PR + pstride;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int jb = wring + 4 * s + k4;
+            cplx<double> Rn{0.0, 0.0}, Rs{0.0, 0.0}, In{0.0, 0.0}, Is{0.0, 0.0};
+            if (jb < d.npairs && b < nmap) {
+                const int rs = d.nrings - 1 - jb;
+                Rn = PR[jb];
+                In = PI[jb];
+                if (rs != jb) {
+                    Rs = PR[rs];
+                    Is = PI[rs];
+                }
+            }
+            const cplx<double> Ra = cadd(Rn, Rs), Rd = csub(Rn, Rs), Ia = cadd(In, Is), Id = csub(In, Is);
+            double pe, ne, po, no;  // Bp, Bn of the even and the odd type
+            switch (q) {
+                case 0: pe = -Ra.re; ne = Id.im; po = -Rd.re; no = Ia.im; break;
+                case 1: pe = -Ra.im; ne = -Id.re; po = -Rd.im; no = -Ia.re; break;
+                case 2: pe = -Ia.re; ne = -Rd.im; po = -Id.re; no = -Ra.im; break;
+                default: pe = -Ia.im; ne = Rd.re; po = -Id.im; no = Ra.re; break;
+            }
+            const double p_0 = p0 ? po : pe, n_0 = p0 ? no : ne, p_1 = p0 ? pe : po, n_1 = p0 ? ne : no;
+            bp0[s] = 0.5 * (p_0 + n_0);
+            bm0[s] = 0.5 * (p_0 - n_0);
+            bp1[s] = 0.5 * (p_1 + n_1);
+            bm1[s] = 0.5 * (p_1 - n_1);
+        }
+    }
+
+    // coefficients with m <= l < spin vanish identically
+    for (int i = tid; i < (lstart - m) * 8; i += 512) {
+        const int l = m + i / 8, b = chunk * 4 + (i % 8) / 2, g = i % 2;
+        if (l <= d.lmax && b < nmap) dst[(size_t)(2 * b + g) * out_stride + base + l] = cplx<double>{0.0, 0.0};
+    }
+    const double* cf4 = d.coef + 4 * base;
+    const bool skip = __syncthreads_and(jr >= d.npairs || d.dead[(size_t)m * d.npairs + jrc]);
+    double* Tw = T + wave * 2048 + h * 1024;  // this lane's sign
+    const double* Ta = T + wave * 2048;
+    bool fast = false;
+    for (int l0 = lstart; l0 <= d.lmax; l0 += ADJ64_L) {
+        const int n = min(ADJ64_L, d.lmax + 1 - l0);
+        v4d_t D0 = v4d_t{0.0, 0.0, 0.0, 0.0}, D1 = v4d_t{0.0, 0.0, 0.0, 0.0};
+        if (!skip) {
+            double pal = 0.0, pbe = 0.0, pga = 0.0;
+            if (lane < n) {
+                pal = cf4[4 * (l0 + lane)];
+                pbe = cf4[4 * (l0 + lane) + 1];
+                pga = cf4[4 * (l0 + lane) + 2];
+            }
+            if (!fast) fast = __all(r.k == 0);
+            if (fast && n == ADJ64_L) {
+#pragma unroll 4
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = r.f;
+                    rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                }
+            } else if (n == ADJ64_L) {
+#pragma unroll 4
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = r.f * r.cf;
+                    rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                    if ((c & 3) == 3) rec_rescale(r);
+                }
+            } else {
+                for (int c = 0; c < ADJ64_L; c++) {
+                    const int i = c >> 1;
+                    double v = 0.0;
+                    if (c < n) {
+                        v = fast ? r.f : r.f * r.cf;
+                        rec_fast(r, readlane_f64(pal, c) * (x + sgb * readlane_f64(pbe, c)), readlane_f64(pga, c));
+                        if (!fast && (c & 3) == 3) rec_rescale(r);
+                    }
+                    Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = v;
+                }
+            }
+            __syncthreads();  // the wave's tiles are complete
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const int sw = j16 * 32 + ((4 * s + k4) ^ ((4 * j16) & 28));
+                D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[sw], bp0[s], D0, 0, 0, 0);
+                D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[1024 + sw], bm0[s], D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[512 + sw], bp1[s], D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[1024 + 512 + sw], bm1[s], D1, 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave has read its tiles: reuse the space for the sums
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 4 * v + k4;
+            T[((wave * 2) * 16 + i) * 17 + j16] = D0[v];
+            T[((wave * 2 + 1) * 16 + i) * 17 + j16] = D1[v];
+        }
+        __syncthreads();
+        {
+            // thread: real column j = 4 (map) + q, l = l0 + tid % 32
+            const int j = tid >> 5, b = chunk * 4 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
+            const int ll = tid & 31, par = ll & 1, i = ll >> 1;
+            double sum = 0.0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) sum += T[((w * 2 + par) * 16 + i) * 17 + j];
+            if (ll < n && b < nmap)
+                reinterpret_cast<double*>(dst + (size_t)(2 * b + g) * out_stride + base + l0 + ll)[pp] = sum * osc;
+        }
+        __syncthreads();  // the sums are read before the next tile overwrites them
+    }
+}
+
 // One chunk of LC values of l for NRT ring pairs (spin s); p0 = parity of l_start + m + s.
 template <typename T, int NRT, int BC, int LC, bool FAST, bool FULL>
 __device__ __forceinline__ void adj_sp_chunk(cplx<T> (&acc)[LC][BC][2], Rec (&rp)[NRT], Rec (&rm)[NRT],
@@ -3163,10 +3450,8 @@ size_t sht_adjoint_work_bytes(const ShtPlan& p, int nmap) {
     const int ns = sht_adjoint_nsplit(p, nmap);
     // ring-slice partial sums: of the vector kernels' split, or of the matrix-core kernel
     int np = ns > 1 ? ns : 0;
-    if (p.prec == 4) {
-        const int nsm = (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS;
-        if (nsm > 1) np = std::max(np, nsm);
-    }
+    const int nsm = (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS;
+    if (nsm > 1) np = std::max(np, nsm);
     return ph + (size_t)np * nmap * p.ncomp * p.nlm * 2 * p.prec;
 }
 
@@ -3204,11 +3489,11 @@ static constexpr AdjShape ADJ_SHAPES[] = {
     {4, 8, 2, true, true},  {1, 1, 1, false, true}, {2, 1, 1, false, true}, {4, 1, 1, false, true},
     {8, 1, 1, false, true}, {8, 2, 1, false, true}, {8, 4, 1, false, true}};
 static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES[0]));
-// shape id of the matrix-core adjoint (FP32): 256 ring pairs and 16 complex columns (spin 0) or
-// 8 maps (spin s) per block
+// shape id of the matrix-core adjoint: 256 ring pairs per block and 16 complex columns (spin 0) or
+// 8 maps (spin s) in FP32, 8 complex columns or 4 maps in FP64
 static constexpr int ADJ_SHAPE_MFMA = N_ADJ_SHAPES;
 
-static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4; }
+static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4 || p.prec == 8; }
 
 static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
     if (id == ADJ_SHAPE_MFMA) return adj_mfma_ok(p);
@@ -3221,8 +3506,9 @@ static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
 // chunk of 2 (spin 0) or 1 map (spin s) per block.  Throughput (many columns): 4 ring pairs per
 // thread and more waves per block.
 static int adj_shape_rule(const ShtPlan& p, int ncol, bool many, int pps) {
-    // FP32 batches: the matrix-core kernel (as for the synthesis)
+    // batches: the matrix-core kernel (FP32 as for the synthesis; FP64 from 8 real columns)
     if (p.prec == 4 && (p.spin == 0 ? ncol > 8 : ncol / 2 > 2)) return ADJ_SHAPE_MFMA;
+    if (p.prec == 8 && (p.spin == 0 ? ncol > 4 : ncol / 2 > 1)) return ADJ_SHAPE_MFMA;
     const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
     if (many) {
         static constexpr int thr[6] = {2, 2, 2, 6, 7, 8};
@@ -3266,28 +3552,40 @@ static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phas
 static_assert(ADJM_PAIRS == ADJM_SP_PAIRS, "both matrix-core adjoints slice rings alike");
 static int adj_mfma_nslice(const ShtPlan& p) { return (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS; }
 
+template <typename T>
 static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
                            double scale, void* part, hipStream_t st) {
-    ShtDev<float> d = dev_view<float>(p);
+    ShtDev<T> d = dev_view<T>(p);
     const int nslice = adj_mfma_nslice(p);
     void* dst = nslice > 1 ? part : alm;
     const long dstride = nslice > 1 ? p.nlm : alm_stride;
-    const float osc = nslice > 1 ? 1.0f : (float)scale;
-    if (p.spin == 0) {
-        const int nchunk = (ncol + 15) / 16;
-        hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0, st,
-                           d, ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+    const T osc = nslice > 1 ? T(1) : (T)scale;
+    if constexpr (sizeof(T) == 4) {
+        if (p.spin == 0) {
+            const int nchunk = (ncol + 15) / 16;
+            hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0,
+                               st, d, ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+        } else {
+            const int nmap = ncol / 2, nchunk = (nmap + 7) / 8;
+            hipLaunchKernelGGL(k_leg_adj_sp_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0,
+                               st, d, nmap, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+        }
     } else {
-        const int nmap = ncol / 2, nchunk = (nmap + 7) / 8;
-        hipLaunchKernelGGL(k_leg_adj_sp_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0, st,
-                           d, nmap, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+        if (p.spin == 0) {
+            const int nchunk = (ncol + 7) / 8;
+            hipLaunchKernelGGL(k_leg_adj_s0_mfma64, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0,
+                               st, d, ncol, (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
+        } else {
+            const int nmap = ncol / 2, nchunk = (nmap + 3) / 4;
+            hipLaunchKernelGGL(k_leg_adj_sp_mfma64, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0,
+                               st, d, nmap, (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
+        }
     }
     PS_LAUNCH_CHECK();
     if (nslice == 1) return 0;
     const long n = (long)ncol * p.nlm;
-    hipLaunchKernelGGL(k_adj_sum_splits<float>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0,
-                       st, (const cplx<float>*)part, nslice, ncol, p.nlm, (cplx<float>*)alm, alm_stride,
-                       (float)scale);
+    hipLaunchKernelGGL(k_adj_sum_splits<T>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0,
+                       st, (const cplx<T>*)part, nslice, ncol, p.nlm, (cplx<T>*)alm, alm_stride, (T)scale);
     PS_LAUNCH_CHECK();
     return 0;
 }
@@ -3311,7 +3609,7 @@ static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, 
             set_error("adjoint: invalid launch shape");
             return -1;
         }
-        return launch_adj_mfma(p, ncol, phase, alm, alm_stride, scale, part, st);
+        return launch_adj_mfma<T>(p, ncol, phase, alm, alm_stride, scale, part, st);
     }
     int rc = launch_adj_shape<T>(p, shape, ncol, phase, dst, dstride, scale, ns, pps, st);
     if (rc || ns == 1) return rc;

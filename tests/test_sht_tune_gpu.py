@@ -36,7 +36,9 @@ def _case(nside, lmax, spin, nmaps, seed):
 @pytest.mark.parametrize("nside,lmax,prec,spin,nmaps", [
     (32, 64, 32, 0, 1), (32, 64, 32, 0, 20), (32, 64, 64, 0, 3), (32, 64, 32, 2, 5), (32, 64, 64, 2, 1),
     (16, 47, 32, 1, 3), (64, 128, 32, 0, 2), (64, 128, 64, 2, 2), (256, 512, 32, 0, 17), (256, 512, 32, 2, 9),
-    (64, 128, 32, 0, 80), (32, 95, 32, 0, 100), (128, 200, 32, 0, 40)])
+    (64, 128, 32, 0, 80), (32, 95, 32, 0, 100), (128, 200, 32, 0, 40),
+    # FP64 batches (the FP64 matrix-core adjoint; ring slices and partial sums from Nside 256)
+    (32, 64, 64, 0, 20), (64, 128, 64, 2, 9), (256, 512, 64, 0, 12), (256, 512, 64, 2, 5)])
 def test_every_candidate_shape_matches_oracle(gpu, nside, lmax, prec, spin, nmaps):
     from polyspec_amd._abi import ShtPlan
     a, m, ref_map, ref_alm = _case(nside, lmax, spin, nmaps, 4000 + nside + lmax + nmaps + 100 * spin)
