@@ -538,6 +538,105 @@ def bench_bspec(nside, nsims, nbins=20, plain=True):
     return out
 
 
+def bench_bspec_cfg4(spec, nbins=20, fp64_sims=2, sim_memory_gb=None):
+    """Config 4 at its size: one BSpecBin analysis per (Nside, simulations) of spec (lmax =
+    min(2 Nside, 2000), nbins log-spaced bins, TTT/TTE/TEE/EEE, both parities, the synthetic mask),
+    every simulation generated on the device and streamed (the H maps of one Nside-1024 simulation
+    take 12 GB, so 256 of them cannot all be resident), and every pixel sum in
+    ps_bspec_sums_multi launches over rounds of simulations of all resolutions
+    (Bl_numerator_multi).  Timed end to end: S^-1 and H maps of the data and of every simulation,
+    the sums.  Against the plain per-simulation GPU path (FP64, one transform per (bin, field),
+    per-simulation launches) on the first fp64_sims simulations of each analysis: the FP32
+    numerators of those simulations against the FP64 ones, and the FP64 time per simulation of
+    each resolution, from which the plain path's time for the whole set is extrapolated (every
+    simulation costs the same)."""
+    import torch
+    from polyspec_amd.base import PolySpec
+    from polyspec_amd.bspec_bin import BSpecBin, Bl_numerator_multi
+    from polyspec_amd.weighting import Weightings
+    fields = ['TTT', 'TTE', 'TEE', 'EEE']
+
+    def analysis(nside, nsims, prec, batch, sim=None):
+        lmax = min(2 * nside, 2000)
+        raw, tot, beam = spectra(nside)
+        mask = synthetic_mask(nside)
+        base = PolySpec(nside, dict(tot), beam=beam, pol=True, precision=prec)
+        base.grf_device = True
+        W = Weightings(base, mask, dict(raw))
+        bs = BSpecBin(base, mask, W.applySinv_ideal, log_bins(lmax, nbins), fields=fields, parity='both',
+                      batch_maps=batch)
+        mask_d = torch.from_numpy(mask).to(base.device, base.rdtype)
+        seed0 = int(7e6) + 1000 * nside
+        if sim is None:
+            def sim(i):
+                return (base.generate_data(seed=seed0 + 1 + i) * mask_d).contiguous()
+        bs.load_sims(sim, nsims, preload=False)
+        return bs, sim
+
+    info, analyses, datas, sims = [], [], [], []
+    for nside, nsims in spec:
+        bs, sim = analysis(nside, nsims, 32, True)
+        analyses.append(bs)
+        datas.append(sim(-1))
+        sims.append(sim)
+        info.append({"nside": nside, "lmax": min(2 * nside, 2000), "n_sims": nsims,
+                     "n_triplets": int(bs._trip.shape[0])})
+    # warm-up on a one-simulation copy of the analyses: plans, workspaces, tile autotuning
+    for bs in analyses:
+        bs.N_it = 1
+    Bl_numerator_multi(analyses, datas, sim_memory_gb=sim_memory_gb)
+    for bs, (nside, nsims) in zip(analyses, spec):
+        bs.N_it = nsims
+    torch.cuda.synchronize()
+    analyses[0].kernel_events = []
+    t0 = time.perf_counter()
+    full = Bl_numerator_multi(analyses, datas, sim_memory_gb=sim_memory_gb)
+    torch.cuda.synchronize()
+    t_full = time.perf_counter() - t0
+    ev = analyses[0].kernel_events
+    analyses[0].kernel_events = None
+    launches = [e0.elapsed_time(e1) for n, e0, e1 in ev if n == 'multi']
+    nsim_total = int(sum(a["n_sims"] for a in info))
+    flops = sum((42. + 108. * a["n_sims"]) * a["n_triplets"] * 12 * a["nside"] ** 2 for a in info)
+    out = {"analyses": info, "n_sims_total": nsim_total, "s_total": t_full, "sims_per_s": nsim_total / t_full,
+           "sum_launches": len(launches), "s_sum_launches": sum(launches) / 1e3,
+           "sum_tflops": flops / (sum(launches) * 1e-3) / 1e12,
+           "numerator_max_abs": [float(np.abs(b).max()) for b in full]}
+    log("bspec cfg4 %s: %d sims in %.1f s (%.2f sims/s), %d sum launches %.1f s (%.1f TFLOP/s)" %
+        (spec, nsim_total, t_full, nsim_total / t_full, len(launches), sum(launches) / 1e3, out["sum_tflops"]))
+    # the plain per-simulation FP64 path on the first fp64_sims simulations of each analysis
+    sub = []
+    for bs in analyses:
+        bs.N_it = fp64_sims
+    sub32 = Bl_numerator_multi(analyses, datas, sim_memory_gb=sim_memory_gb)
+    per_sim64, diffs = [], []
+    for (nside, nsims), sim32, d32, b32 in zip(spec, sims, datas, sub32):
+        bs64, _ = analysis(nside, fp64_sims, 64, False, sim=lambda i, f=sim32: f(i).double())
+        d64 = d32.double()
+        bs64.Bl_numerator(d64, include_linear_term=False)  # warm-up (plans)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b64 = bs64.Bl_numerator(d64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        per_sim64.append(t64 / fp64_sims)
+        diffs.append(float(np.abs(b32 - b64).max() / np.abs(b64).max()))
+        sub.append({"nside": nside, "s_fp64_per_sim": t64 / fp64_sims, "max_abs_diff_over_max_abs": diffs[-1]})
+        del bs64
+        torch.cuda.empty_cache()
+    t_plain = sum(ps * a["n_sims"] for ps, a in zip(per_sim64, info))
+    out["plain_fp64"] = {"sample": "%d simulations of each analysis (FP64 data and simulation maps: the FP32 ones "
+                                   "widened)" % fp64_sims, "per_analysis": sub,
+                         "s_total_extrapolated": t_plain, "sims_per_s_extrapolated": nsim_total / t_plain}
+    out["speedup_vs_plain_fp64"] = t_plain / t_full
+    out["max_abs_diff_over_max_abs"] = max(diffs)
+    log("bspec cfg4: plain fp64 %.1f s extrapolated (%.1fx), max|diff|/max|B| %.2e on the sample" %
+        (t_plain, t_plain / t_full, max(diffs)))
+    del analyses, datas
+    torch.cuda.empty_cache()
+    return out
+
+
 def _bspec_shape_report():
     """Launch shapes the binned-bispectrum sum autotuner kept: {segment kinds: (shape id, ms per shape)}."""
     from polyspec_amd import bspec_bin as B
@@ -758,6 +857,11 @@ def main():
     ap.add_argument("--bspec-nside", type=str, default="256",
                     help="comma-separated Nside values of the binned-bispectrum section (lmax = min(2 Nside, 2000))")
     ap.add_argument("--bspec-sims", type=int, default=16)
+    ap.add_argument("--bspec-cfg4", type=str, default="",
+                    help="config 4 at its size, e.g. 256:86,512:85,1024:85 (Nside:simulations; streamed "
+                         "simulations, rounds of one-launch sums; off by default: about two minutes)")
+    ap.add_argument("--bspec-cfg4-fp64-sims", type=int, default=2,
+                    help="simulations per analysis of the plain FP64 comparison of --bspec-cfg4")
     ap.add_argument("--bspec-mixed", type=str, default="256:8,512:4,1024:2",
                     help="Nside:simulations list of the mixed-resolution binned-bispectrum section ('' = skip)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -916,6 +1020,10 @@ def main():
             log("binned bispectrum, mixed resolutions in one launch ...")
             spec = [tuple(int(v) for v in x.split(":")) for x in args.bspec_mixed.split(",")]
             result["bspec_mixed"] = bench_bspec_mixed(spec)
+    if args.bspec_cfg4:
+        log("binned bispectrum, config 4 at its size (streamed simulations) ...")
+        spec = [tuple(int(v) for v in x.split(":")) for x in args.bspec_cfg4.split(",")]
+        result["bspec_cfg4"] = bench_bspec_cfg4(spec, fp64_sims=args.bspec_cfg4_fp64_sims)
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")

@@ -74,10 +74,11 @@ def multi_autotune(segments, ngroup, reps=2):
     return best.value, {i: ms[i] for i in range(n)}
 
 
-def multi_sums(segments, outs, shape=-1):
+def multi_sums(segments, outs, shape=-1, accumulate=False):
     """One ps_bspec_sums_multi launch.  segments: [(stacks, triplets, group)], stacks being
     (A, B, C) (triple), (A, C) (cubic, A = B) or (P, Q, M, N) (linear term); outs: one
-    (ntrip, 2) FP64 tensor per group, overwritten with the sum of its segments."""
+    (ntrip, 2) FP64 tensor per group, overwritten with the sum of its segments (accumulate:
+    the sum is added to it)."""
     lib = load()
     prec = 32 if segments[0][0][0].dtype == torch.complex64 else 64
     ng = len(outs)
@@ -85,7 +86,7 @@ def multi_sums(segments, outs, shape=-1):
     grp = (BspecGroup * ng)()
     for g, o in zip(grp, outs):
         assert o.dtype == torch.float64 and o.is_contiguous()
-        g.out, g.scale, g.accumulate = o.data_ptr(), 1.0, 0
+        g.out, g.scale, g.accumulate = o.data_ptr(), 1.0, int(bool(accumulate))
     nb = int(lib.ps_bspec_multi_scratch_bytes(prec, len(segments), ctypes.cast(arr, ctypes.c_void_p), ng))
     check(min(nb, 0))
     scr = _scratch(segments[0][0][0].device, nb)
@@ -112,44 +113,92 @@ def _tuned_shape(segments, ngroup, enabled):
 
 
 def _numerator_segments(bs, data, include_linear_term, g_cub, g_lin):
-    """S^-1, H maps and the sum segments of one analysis (groups g_cub: cubic, g_lin: linear)."""
+    """S^-1, H maps and the sum segments of one analysis (groups g_cub: cubic, g_lin: linear):
+    the cubic term and the linear term of every preloaded simulation.  Returns (segments,
+    (p1, m2)), the data's H maps being what streamed simulations pair with."""
     h = bs.applySinv(data, input_type='map')
     p1, m2 = bs.H_maps(h)
     segs = [((p1, m2), bs._trip, g_cub)]
-    if include_linear_term:
+    if include_linear_term and bs.preload:
         for ii in range(bs.N_it):
             segs.append(((p1, bs.p1_H_maps[ii], m2, bs.m2_H_maps[ii]), bs._trip, g_lin))
-    return segs
+    return segs, (p1, m2)
 
 
-def Bl_numerator_multi(analyses, datas, include_linear_term=True):
-    """Numerators of several BSpecBin analyses (e.g. one per resolution, each with its
-    preloaded simulations) with every pixel reduction -- the cubic term of each data map and the
-    linear term of every simulation of every analysis -- in one ps_bspec_sums_multi launch.
-    Returns the list of numerators, each as BSpecBin.Bl_numerator returns it."""
+def _stream_rounds(streamed, budget):
+    """Segments of the streamed simulations in rounds: each round makes the H maps of the next
+    simulation of every analysis in turn (round-robin over resolutions) until the round's maps
+    would exceed `budget` bytes (at least one simulation per round)."""
+    nxt = [0] * len(streamed)
+    while any(nxt[k] < st[0].N_it for k, st in enumerate(streamed)):
+        segs, used, added = [], 0, True
+        while added:
+            added = False
+            for k, (bs, p1, m2, g) in enumerate(streamed):
+                per = 2 * p1.numel() * p1.element_size()
+                if nxt[k] >= bs.N_it or (segs and used + per > budget):
+                    continue
+                sp1, sm2 = bs.load_sim_data(nxt[k])
+                nxt[k] += 1
+                used += per
+                segs.append(((p1, sp1, m2, sm2), bs._trip, g))
+                added = True
+        yield segs
+
+
+def Bl_numerator_multi(analyses, datas, include_linear_term=True, sim_memory_gb=None):
+    """Numerators of several BSpecBin analyses (e.g. one per resolution) with every pixel
+    reduction -- the cubic term of each data map and the linear term of every simulation of
+    every analysis -- in ps_bspec_sums_multi launches: one launch for the cubic terms and the
+    preloaded simulations, and, for analyses whose simulations are streamed (load_sims /
+    generate_sims with preload=False, as a simulation set too large for device memory needs),
+    one launch per round of simulations whose H maps fit `sim_memory_gb` (default: half the
+    free device memory), each adding to the previous ones.  Returns the list of numerators,
+    each as BSpecBin.Bl_numerator returns it."""
     assert len(analyses) == len(datas) and len(analyses) >= 1
-    segs, outs, gids = [], [], []
+    segs, outs, gids, streamed = [], [], [], []
     for bs, data in zip(analyses, datas):
         if not hasattr(bs, 'sym_factor'):
             bs._compute_symmetry_factor()
-        if include_linear_term and not getattr(bs, 'preload', False):
-            raise Exception("Bl_numerator_multi needs preloaded bias simulations (load_sims(..., preload=True))")
+        if include_linear_term and not hasattr(bs, 'preload'):
+            raise Exception("Need to generate or specify bias simulations!")
         ntrip = bs._trip.shape[0]
         g_cub = len(outs)
         g_lin = g_cub + 1 if include_linear_term else -1
         outs += [torch.zeros((ntrip, 2), dtype=torch.float64, device=bs.base.device)
                  for _ in range(2 if include_linear_term else 1)]
         gids.append((g_cub, g_lin))
-        segs += _numerator_segments(bs, data, include_linear_term, g_cub, g_lin)
-    shape = _tuned_shape(segs, len(outs), all(bs.base.autotune for bs in analyses))
+        s_, (p1, m2) = _numerator_segments(bs, data, include_linear_term, g_cub, g_lin)
+        segs += s_
+        if include_linear_term and not bs.preload:
+            streamed.append((bs, p1, m2, g_lin))
+    tune = all(bs.base.autotune for bs in analyses)
     ev = analyses[0].kernel_events
-    if ev is not None:
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-    multi_sums(segs, outs, shape)
-    if ev is not None:
-        e1.record()
-        ev.append(('multi', e0, e1))
+
+    def launch(sg, acc):
+        # the launch's groups: those its segments add to (a group needs at least one segment)
+        used = sorted({g for _, _, g in sg})
+        remap = {g: i for i, g in enumerate(used)}
+        sg = [(st, tr, remap[g]) for st, tr, g in sg]
+        shape = _tuned_shape(sg, len(used), tune)
+        if ev is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        multi_sums(sg, [outs[g] for g in used], shape, accumulate=acc)
+        if ev is not None:
+            e1.record()
+            ev.append(('multi', e0, e1))
+
+    launch(segs, False)
+    if streamed:
+        if sim_memory_gb is None:
+            free, _ = torch.cuda.mem_get_info(analyses[0].base.device)
+            budget = 0.5 * free
+        else:
+            budget = float(sim_memory_gb) * 2 ** 30
+        for rs in _stream_rounds(streamed, budget):
+            launch(rs, True)
+            del rs
     res = []
     for bs, (g_cub, g_lin) in zip(analyses, gids):
         b3 = bs.base.A_pix * bs._assemble(outs[g_cub]) / 3.

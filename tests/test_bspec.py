@@ -245,3 +245,22 @@ def test_gpu_bspec_fused_and_multi_resolution(gpu, prec):
     for (bs, d, fused), c in zip(runs, cub):
         ref = bs.Bl_numerator(d, include_linear_term=False)
         assert np.abs(c - ref).max() < tol * np.abs(ref).max()
+    # streamed simulations (preload=False): rounds of simulations whose H maps fit the budget,
+    # each round one launch adding to the earlier ones.  A budget of one Nside-32 simulation's maps
+    # gives one round of the three Nside-16 simulations (a quarter of that each), then one round
+    # per Nside-32 simulation; a large budget, one round of all six.  The Nside-16 analysis streamed
+    # beside a preloaded Nside-32 one covers the mixed case.
+    keep = [(r[0].p1_H_maps, r[0].m2_H_maps) for r in runs]
+    for r, (p1s, m2s) in zip(runs, keep):
+        r[0].load_sims(lambda i, p1s=p1s, m2s=m2s: None, 3, preload=False)
+        r[0].load_sim_data = lambda i, p1s=p1s, m2s=m2s: (p1s[i], m2s[i])
+    per32 = 2 * keep[1][0][0].numel() * keep[1][0][0].element_size()
+    for budget in (per32 / 2 ** 30, 1.0):
+        streamed = Bl_numerator_multi([r[0] for r in runs], [r[1] for r in runs], sim_memory_gb=budget)
+        for (bs, d, fused), m in zip(runs, streamed):
+            assert np.abs(m - fused).max() < tol * np.abs(fused).max(), budget
+    runs[1][0].preload = True
+    runs[1][0].p1_H_maps, runs[1][0].m2_H_maps = keep[1]
+    mixed = Bl_numerator_multi([r[0] for r in runs], [r[1] for r in runs], sim_memory_gb=per32 / 2 ** 30)
+    for (bs, d, fused), m in zip(runs, mixed):
+        assert np.abs(m - fused).max() < tol * np.abs(fused).max()
