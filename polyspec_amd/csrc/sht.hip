@@ -1239,6 +1239,294 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int 
     }
 }
 
+// ------------------------------- Legendre syntheses, spin 0 and spin s, FP64 matrix cores
+// F_m(ring, col) = sum_l lambda_l(ring) a_lm(col) on v_mfma_f64_16x16x4f64 (A[i][k] at lane
+// i + 16 k, B[k][j] at lane j + 16 k, D[i][j] at lane j + 16 (i % 4), register i / 4).  The
+// recursion values are staged in LDS as in the FP64 adjoints: per tile of 32 values of l each
+// lane writes its ring's values into rows [parity][(l - l0) / 2][ring ^ swizzle(row)], and a
+// wave issues one MFMA per 16 rings, parity and 4 values of l of that parity, A = lambda
+// (16 rings x 4 l), B = the tile's alm rows (4 l x 16 real columns), which the block stages in
+// LDS (the next tile's rows are loaded into registers while the current one runs).  Even and
+// odd l - l_start accumulate separately: north = even + odd, south = even - odd.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+constexpr int SYN64_L = 32;  // values of l per tile
+
+// Spin 0: one block = 4 waves = 256 ring pairs (one per lane), one m, 8 complex columns.
+__global__ __launch_bounds__(256) void k_leg_synth_s0_mfma64(ShtDev<double> d, int ncol, int ld,
+                                                             const cplx<double>* __restrict__ alm_t,
+                                                             cplx<double>* __restrict__ phase) {
+    __shared__ __align__(16) double T[4 * 2 * 16 * 64];  // lambda tiles [wave][parity][row][64]
+    __shared__ __align__(16) double sa[SYN64_L * 16];   // alm rows [l - l0][real column]
+    __shared__ double sco[SYN64_L * 2];                 // alpha, gamma
+    __shared__ cplx<double> rot[256];
+    const int nrb = (d.npairs + 255) / 256;
+    const int nchunk = (ncol + 7) / 8;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = lb % nrb;
+    const int chunk = (lb / nrb) % nchunk;
+    const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int k4 = lane >> 4, j16 = lane & 15;
+    const int jp = rb * 256 + tid;  // this lane's ring pair
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    {
+        const RingInfo ri = d.rings[jpc];
+        rot[tid] = expi_pi<double>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    }
+    const double x = d.cth[jpc];
+    Rec r;
+    rec_seed(r, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
+    v4d_t D[4][2];
+#pragma unroll
+    for (int rg = 0; rg < 4; rg++) D[rg][0] = D[rg][1] = v4d_t{0.0, 0.0, 0.0, 0.0};
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const double* cf4 = d.coef + 4 * base;
+    // staging: thread t holds complex column t % 8 of tile row t / 8 (columns past ld read as zero)
+    const int srow = tid >> 3, scol = tid & 7;
+    const bool scol_ok = chunk * 8 + scol < ld;
+    cplx<double> pa{0.0, 0.0};
+    double pal = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        const int nn = min(SYN64_L, d.lmax + 1 - l0n);
+        pa = (srow < nn && scol_ok) ? alm_t[(size_t)(base + l0n + srow) * ld + chunk * 8 + scol]
+                                    : cplx<double>{0.0, 0.0};
+        if (tid < nn) {
+            pal = cf4[4 * (l0n + tid)];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    // a block whose ring pairs are all dead at this m writes zero phases without the l loop
+    const bool skip = __syncthreads_and(jp >= d.npairs || d.dead[(size_t)m * d.npairs + jpc]);
+    double* Tw = T + wave * 2048;
+    bool fast = false;
+    if (!skip) fetch(m);
+    for (int l0 = m; l0 <= d.lmax && !skip; l0 += SYN64_L) {
+        const int n = min(SYN64_L, d.lmax + 1 - l0);
+        __syncthreads();  // every read of the previous tile is done
+        sa[srow * 16 + 2 * scol] = pa.re;
+        sa[srow * 16 + 2 * scol + 1] = pa.im;
+        if (tid < SYN64_L) {
+            sco[2 * tid] = pal;
+            sco[2 * tid + 1] = pga;
+        }
+        __syncthreads();
+        if (l0 + SYN64_L <= d.lmax) fetch(l0 + SYN64_L);
+        if (!fast) fast = __all(r.k == 0);
+        if (fast && n == SYN64_L) {
+#pragma unroll 4
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = r.f;
+                rec_fast(r, sco[2 * c] * x, sco[2 * c + 1]);
+            }
+        } else if (n == SYN64_L) {
+            // full tile while some lane still carries a scale exponent
+#pragma unroll 4
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = r.f * r.cf;
+                rec_fast(r, sco[2 * c] * x, sco[2 * c + 1]);
+                if ((c & 3) == 3) rec_rescale(r);
+            }
+        } else {
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                double v = 0.0;
+                if (c < n) {
+                    v = fast ? r.f : r.f * r.cf;
+                    rec_fast(r, sco[2 * c] * x, sco[2 * c + 1]);
+                    if (!fast && (c & 3) == 3) rec_rescale(r);
+                }
+                Tw[(c & 1) * 1024 + i * 64 + (lane ^ (4 * i))] = v;
+            }
+        }
+        __syncthreads();  // the tiles are complete
+#pragma unroll
+        for (int par = 0; par < 2; par++)
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int row = 4 * kk + k4;  // (l - l0) / 2 of this lane's k
+                const double b = sa[(2 * row + par) * 16 + j16];
+#pragma unroll
+                for (int rg = 0; rg < 4; rg++) {
+                    const double a = Tw[par * 1024 + row * 64 + ((16 * rg + j16) ^ (4 * row))];
+                    D[rg][par] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, D[rg][par], 0, 0, 0);
+                }
+            }
+    }
+    // D[rg][par][v] at lane (j16, k4): ring pair 16 rg + 4 v + k4 of the wave, real column j16
+    const bool re = (j16 & 1) == 0;
+    const int col = chunk * 8 + (j16 >> 1);
+#pragma unroll
+    for (int rg = 0; rg < 4; rg++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int ir = wave * 64 + 16 * rg + 4 * v + k4;
+            const int jr = rb * 256 + ir;
+            const double e = D[rg][0][v], o = D[rg][1][v];
+            const double nv = e + o, sv = e - o;
+            const double nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
+            const cplx<double> w = rot[ir];
+            const double no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+            const double so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+            if (jr < d.npairs && col < ncol) {
+                const int rn = jr, rs = d.nrings - 1 - jr;
+                reinterpret_cast<double*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+                if (rs != rn)
+                    reinterpret_cast<double*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+            }
+        }
+}
+
+// Spin s: lanes 0-31 of a wave run f+ and lanes 32-63 f- of the wave's 32 ring pairs; the
+// accumulators hold F+- = sum_l f+- a_lm, and lam+- = (f+ +- f-) / 2 are formed at the end
+// (D+- = (F+ +- F-) / 2), which then combine as in k_leg_synth_sp_mfma.  One block = 4 waves =
+// 128 ring pairs, one m, 4 maps (8 complex columns: G and C of each map).
+__global__ __launch_bounds__(256) void k_leg_synth_sp_mfma64(ShtDev<double> d, int ncol, int ld,
+                                                             const cplx<double>* __restrict__ alm_t,
+                                                             cplx<double>* __restrict__ phase) {
+    __shared__ __align__(16) double T[4 * 2 * 2 * 16 * 32];  // f tiles [wave][sign][parity][row][32]
+    __shared__ __align__(16) double sa[SYN64_L * 16];
+    __shared__ double sco[SYN64_L * 3];
+    __shared__ cplx<double> rot[128];
+    const int nrb = (d.npairs + 127) / 128;
+    const int nchunk = (ncol + 7) / 8;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = lb % nrb;
+    const int chunk = (lb / nrb) % nchunk;
+    const int m = m_of_q(lb / (nrb * nchunk), d.mmax);
+    const int M1 = d.mmax + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int k4 = lane >> 4, j16 = lane & 15;
+    const int jp = rb * 128 + wave * 32 + li32;
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    if (tid < 128) {
+        const int jr = min(rb * 128 + tid, d.npairs - 1);
+        const RingInfo ri = d.rings[jr];
+        rot[tid] = expi_pi<double>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    }
+    const double x = d.cth[jpc];
+    const double sgb = h ? -1.0 : 1.0;
+    Rec r;
+    rec_seed(r, d.seedL[h * M1 + m] + d.seedA[h * M1 + m] * d.l2sh[jpc] + d.seedB[h * M1 + m] * d.l2ch[jpc],
+             d.seedS[h * M1 + m]);
+    v4d_t F[2][2][2];  // [sign][parity][ring group]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) F[a][b][0] = F[a][b][1] = v4d_t{0.0, 0.0, 0.0, 0.0};
+
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const double* cf4 = d.coef + 4 * base;
+    const int lstart = m > d.spin ? m : d.spin;
+    const int srow = tid >> 3, scol = tid & 7;
+    const bool scol_ok = chunk * 8 + scol < ld;
+    cplx<double> pa{0.0, 0.0};
+    double pal = 0.0, pbe = 0.0, pga = 0.0;
+    auto fetch = [&](int l0n) {
+        const int nn = min(SYN64_L, d.lmax + 1 - l0n);
+        pa = (srow < nn && scol_ok) ? alm_t[(size_t)(base + l0n + srow) * ld + chunk * 8 + scol]
+                                    : cplx<double>{0.0, 0.0};
+        if (tid < nn) {
+            pal = cf4[4 * (l0n + tid)];
+            pbe = cf4[4 * (l0n + tid) + 1];
+            pga = cf4[4 * (l0n + tid) + 2];
+        }
+    };
+    const bool skip = __syncthreads_and(jp >= d.npairs || d.dead[(size_t)m * d.npairs + jpc]);
+    double* Tw = T + wave * 2048 + h * 1024;  // this lane's sign: [parity][row][32]
+    const double* Ta = T + wave * 2048;
+    bool fast = false;
+    if (!skip) fetch(lstart);
+    for (int l0 = lstart; l0 <= d.lmax && !skip; l0 += SYN64_L) {
+        const int n = min(SYN64_L, d.lmax + 1 - l0);
+        __syncthreads();  // every read of the previous tile is done
+        sa[srow * 16 + 2 * scol] = pa.re;
+        sa[srow * 16 + 2 * scol + 1] = pa.im;
+        if (tid < SYN64_L) {
+            sco[3 * tid] = pal;
+            sco[3 * tid + 1] = pbe;
+            sco[3 * tid + 2] = pga;
+        }
+        __syncthreads();
+        if (l0 + SYN64_L <= d.lmax) fetch(l0 + SYN64_L);
+        if (!fast) fast = __all(r.k == 0);
+        if (fast && n == SYN64_L) {
+#pragma unroll 4
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = r.f;
+                rec_fast(r, sco[3 * c] * (x + sgb * sco[3 * c + 1]), sco[3 * c + 2]);
+            }
+        } else if (n == SYN64_L) {
+#pragma unroll 4
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = r.f * r.cf;
+                rec_fast(r, sco[3 * c] * (x + sgb * sco[3 * c + 1]), sco[3 * c + 2]);
+                if ((c & 3) == 3) rec_rescale(r);
+            }
+        } else {
+            for (int c = 0; c < SYN64_L; c++) {
+                const int i = c >> 1;
+                double v = 0.0;
+                if (c < n) {
+                    v = fast ? r.f : r.f * r.cf;
+                    rec_fast(r, sco[3 * c] * (x + sgb * sco[3 * c + 1]), sco[3 * c + 2]);
+                    if (!fast && (c & 3) == 3) rec_rescale(r);
+                }
+                Tw[(c & 1) * 512 + i * 32 + (li32 ^ ((4 * i) & 28))] = v;
+            }
+        }
+        __syncthreads();  // the tiles are complete
+#pragma unroll
+        for (int par = 0; par < 2; par++)
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int row = 4 * kk + k4;
+                const double b = sa[(2 * row + par) * 16 + j16];
+#pragma unroll
+                for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+                    for (int rg = 0; rg < 2; rg++) {
+                        const double a = Ta[sg * 1024 + par * 512 + row * 32 + ((16 * rg + j16) ^ ((4 * row) & 28))];
+                        F[sg][par][rg] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, F[sg][par][rg], 0, 0, 0);
+                    }
+            }
+    }
+    // even accumulators hold l - l_start even, i.e. l + m + s even unless l_start + m + s is odd
+    const bool swp = ((lstart + m + d.spin) & 1) != 0;
+    const int col = chunk * 8 + (j16 >> 1);
+    const bool re = (j16 & 1) == 0;
+    const int q = j16 & 3;
+    const double sgn = (q == 0 || q == 3) ? -1.0 : 1.0;
+#pragma unroll
+    for (int rg = 0; rg < 2; rg++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int ir = wave * 32 + 16 * rg + 4 * v + k4;
+            const int jr = rb * 128 + ir;
+            const double Pe = 0.5 * (F[0][0][rg][v] + F[1][0][rg][v]), Po = 0.5 * (F[0][1][rg][v] + F[1][1][rg][v]);
+            const double Me = 0.5 * (F[0][0][rg][v] - F[1][0][rg][v]), Mo = 0.5 * (F[0][1][rg][v] - F[1][1][rg][v]);
+            const double pe = swp ? Po : Pe, po = swp ? Pe : Po;
+            const double me = swp ? Mo : Me, mo = swp ? Me : Mo;
+            const double mn3 = __shfl_xor(me + mo, 3), ms3 = __shfl_xor(mo - me, 3);
+            const double nv = -((pe + po) + sgn * mn3), sv = -((pe - po) + sgn * ms3);
+            const double nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);  // the other part
+            const cplx<double> w = rot[ir];
+            const double no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+            const double so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+            if (jr < d.npairs && col < ncol) {
+                const int rn = jr, rs = d.nrings - 1 - jr;
+                reinterpret_cast<double*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+                if (rs != rn)
+                    reinterpret_cast<double*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+            }
+        }
+}
+
 // -------------------------------------------------- Legendre synthesis, spin s
 // alm_t: [nlm][ld] complex with columns (map b, comp c) at 2 b + c (G, C).
 template <typename T, int BC>
@@ -2736,7 +3024,6 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
 // A = lambda (16 values of l of one parity x 4 rings), B = the north/south sums or differences
 // (4 rings x 16 real columns), held in registers for the whole l loop.  The waves' sums meet in
 // LDS; 256-ring-pair slices write partial sums that k_adj_sum_splits adds in a fixed order.
-typedef double v4d_t __attribute__((ext_vector_type(4)));
 constexpr int ADJ64_L = 32;  // values of l per tile
 
 // Spin 0: one block = 4 waves = 256 ring pairs (one per lane), one m, 8 complex columns.
@@ -3323,6 +3610,10 @@ static int launch_ring_synth_sg(const ShtPlan& p, const int* list, int nlist, lo
             if constexpr (NB == 1)
                 return launch_ring_synth_sgq<T, 1, 4, BLUE>(p, list, nlist, ncol, phase, map, map_stride, st, sh);
             break;
+        case 8:
+            if constexpr (NB == 1 && BLUE)
+                return launch_ring_synth_sgq<T, 1, 8, BLUE>(p, list, nlist, ncol, phase, map, map_stride, st, sh);
+            break;
         default: break;
     }
     set_error("ring synthesis: staged shape not valid for this ring class");
@@ -3352,10 +3643,13 @@ static int ring_nb_rule(const ShtPlan& p, long maxM, int ncol, bool blue) {
 
 static bool ring_nb_valid(const ShtPlan& p, long maxM, int ncol, bool blue, int id) {
     if (id >= 0 && (id & RING_SG)) {
+        // two buffers within 64 KB of LDS; one buffer up to the CU's 160 KB (the Bluestein rings
+        // of Nside 1024, transforms of 8192 points, take the dynamic-LDS attribute)
         const int nb = id & ~RING_SG, q = ring_q_of(maxM);
         const int items = (blue ? 2 : 4) * nb * q;
-        return (nb == 1 || (nb == 2 && ncol > 2)) && items <= 16 &&
-               ring_sg_lds(p, maxM, nb) + 256 * 2 * (size_t)p.prec <= 65536;
+        const size_t lds = ring_sg_lds(p, maxM, nb) + 256 * 2 * (size_t)p.prec;
+        return (nb == 1 || (nb == 2 && ncol > 2)) && items <= 16 && (q < 8 || blue) &&
+               lds <= (nb == 1 ? (size_t)160 * 1024 : (size_t)65536);
     }
     const int prec = p.prec;
     const int nb = id & ~RING_STK;
@@ -3683,7 +3977,7 @@ static int launch_synth_s0_wr(const ShtPlan& p, int ncol, int ld, const void* al
     PS_LAUNCH_CHECK();
     return 0;
 }
-// Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32 only), SYN_SHAPE_WR = its wide form
+// Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32, or FP64), SYN_SHAPE_WR = its wide form
 // (FP32 spin 0), otherwise the batch columns (spin 0) or maps (spin s) per block of the vector
 // kernel.  Measured at Nside 256 and 512: the wide form is fastest from 24 columns up (1.2-1.4x),
 // the 16-column one from 5 to 16 columns, the vector kernel up to 4.
@@ -3692,14 +3986,16 @@ static int synth_shape_rule(const ShtPlan& p, int ncol) {
     if (p.spin == 0) {
         constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_S0_F : SYN_BC_S0_D;
         if (sizeof(T) == 4 && ncol > 16) return SYN_SHAPE_WR;
-        if (sizeof(T) == 4 && ncol > 4) return 0;
+        if (ncol > 4) return 0;  // the matrix-core kernel (FP32, or its FP64 form)
         return ncol <= 1 ? 1 : ncol <= 2 ? 2 : ncol <= 4 ? 4 : (BCMAX > 8 && ncol <= 8) ? 8 : BCMAX;
     }
     constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
     const int nmap = ncol / p.ncomp;
-    if (sizeof(T) == 4 && nmap > 2) return 0;
+    if (nmap > 2) return 0;
     return nmap <= 1 ? 1 : (BCMAX > 2 && nmap <= 2) ? 2 : BCMAX;
 }
+
+static int launch_synth_mfma64(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st);
 
 template <typename T>
 static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st,
@@ -3708,7 +4004,7 @@ static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* a
     switch (shape) {
         case 0:
             if constexpr (sizeof(T) == 4) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
-            break;
+            else return launch_synth_mfma64(p, ncol, ld, alm_t, phase, st);
         case SYN_SHAPE_WR:
             if constexpr (sizeof(T) == 4) return launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st);
             break;
@@ -3722,6 +4018,23 @@ static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* a
     }
     set_error("synthesis: invalid launch shape");
     return -1;
+}
+
+// FP64 matrix-core syntheses: 256 (spin 0) or 128 (spin s) ring pairs and 8 complex columns per block
+static int launch_synth_mfma64(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
+                               hipStream_t st) {
+    ShtDev<double> d = dev_view<double>(p);
+    const int pairs = p.spin == 0 ? 256 : 128;
+    const int nrb = (p.npairs + pairs - 1) / pairs, nchunk = (ncol + 7) / 8;
+    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    if (p.spin == 0)
+        hipLaunchKernelGGL(k_leg_synth_s0_mfma64, dim3(nb), dim3(256), 0, st, d, ncol, ld,
+                           (const cplx<double>*)alm_t, (cplx<double>*)phase);
+    else
+        hipLaunchKernelGGL(k_leg_synth_sp_mfma64, dim3(nb), dim3(256), 0, st, d, ncol, ld,
+                           (const cplx<double>*)alm_t, (cplx<double>*)phase);
+    PS_LAUNCH_CHECK();
+    return 0;
 }
 
 static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase,
@@ -3742,7 +4055,7 @@ static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* a
     switch (shape) {
         case 0:
             if constexpr (sizeof(T) == 4) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
-            break;
+            else return launch_synth_mfma64(p, 2 * nmap, ld, alm_t, phase, st);
         case 1: return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
         case 2: return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
         case 4:
@@ -3878,6 +4191,10 @@ static int launch_ring_gnl_class(const ShtPlan& p, const int* list, int nlist, l
         return launch_ring_synth_sgq<float, 1, 2, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
     if (nb == 1 && q == 4)
         return launch_ring_synth_sgq<float, 1, 4, BLUE, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    if constexpr (BLUE) {
+        if (nb == 1 && q == 8)
+            return launch_ring_synth_sgq<float, 1, 8, true, true>(p, list, nlist, ncol, phase, nullptr, 0, st, sh, g);
+    }
     set_error("fused gNL sum: no staged ring kernel for this ring class");
     return -1;
 }
@@ -4055,7 +4372,7 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
                                           : (p.prec == 4 ? SYN_BC_SP_F : SYN_BC_SP_D);
             int top = 1;
             while (top < w && top < bcmax) top *= 2;
-            if (p.prec == 4) add(0);
+            add(0);  // the matrix-core kernel (FP32 or FP64)
             if (p.prec == 4 && p.spin == 0) add(SYN_SHAPE_WR);
             for (int bc = std::max(1, top / 2); bc <= top; bc *= 2) add(bc);
             break;

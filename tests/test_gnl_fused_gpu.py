@@ -40,7 +40,7 @@ def test_interleaved_weighting_is_a_column_permutation(gpu, prec, nr):
     assert not torch.any(b[:, :4]) and not torch.any(b[:, 4 + 2 * nr:])
 
 
-@pytest.mark.parametrize("nside,nr,with_uv", [(64, 5, True), (256, 7, False), (512, 40, True)])
+@pytest.mark.parametrize("nside,nr,with_uv", [(64, 5, True), (256, 7, False), (512, 40, True), (1024, 6, True)])
 def test_ring_gnl_matches_maps_then_sum(gpu, nside, nr, with_uv):
     from polyspec_amd import kernels as K
     from polyspec_amd._abi import ShtPlan
