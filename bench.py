@@ -19,6 +19,7 @@ amplitude difference between the two in units of the numerator scatter, the
 Legendre-kernel roofline, and the FP64 CPU oracle on a bounded sample.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -34,7 +35,7 @@ PEAK_FP32 = 157.3e12
 # HBM bytes of one Legendre-synthesis launch (k_leg_synth_s0_wr<5>, 80 maps, Nside 512, lmax 1024):
 # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 in KiB (factors from a 1 GiB copy),
 # profiles/leg_synth_s0_wr_nside512_lmax1024_80maps_pmc.md
-LEG_TRAFFIC = {(512, 1024, 80): (206249.7 * 2 + 1311359.4) * 1024}
+LEG_TRAFFIC = {(512, 1024, 80): (200190.5 * 2 + 1311359.4) * 1024}
 PEAK_HBM = 8.0e12
 PEAK_FP64 = 78.6e12
 
@@ -53,6 +54,8 @@ def leg_kernel_name(base, lmax, ncol):
             if tiles > 5 and -(-tiles // c) * c < -(-tiles // nt) * nt:
                 nt = c
         return "k_leg_synth_s0_wr<%d>" % nt
+    if shape in (82, 83, 84):  # the wide form with an explicit tile count (80 + tiles)
+        return "k_leg_synth_s0_wr<%d> (%d column chunks)" % (shape - 80, -(-ncol // (16 * (shape - 80))))
     return "k_leg_synth_s0_mfma (shape %s)" % shape
 
 
@@ -118,7 +121,7 @@ def make_template(base, raw, mask, tables, lmax, batch, tables_cache=None, graph
 
 def bench_tau(base, raw, mask, tables, lmax, nsims, datas, steps=3, fisher_steps=2):
     """tauNL-loc at the same workload: numerators/s of the batched FP32 path (t2's A_sd terms over
-    every simulation, in batches of 256 // N_r, simulation maps resident up to half the free
+    every simulation, in batches of 256 // N_r, simulation maps resident up to 70 % of the free
     memory, re-synthesised beyond), then gNL-loc + tauNL-loc Fisher contributions/s (masked sky)."""
     import torch
     tm, _ = make_template(base, raw, mask, tables, lmax, True, templates=('tauNL-loc',))
@@ -142,8 +145,13 @@ def bench_tau(base, raw, mask, tables, lmax, nsims, datas, steps=3, fisher_steps
            "path": "batched FP32: sum_s A_ss = Y^dag[sum_s Q_s P_s]; A_sd = Y^dag[Q_s P + Q P_s] in batches; "
                    "tau sums as per-L Gram products on FP64 matrix cores"}
     log("tauNL-loc: %.3f s/numerator (%d of %d simulation maps resident)" % (dt, n_res, 2 * nsims))
+    # the simulation store holds most of the HBM: release it before the next sections (the template
+    # object can sit in a reference cycle until the collector runs)
+    tm._sim_P = tm._sim_Q = None
     del tm
+    gc.collect()
     torch.cuda.empty_cache()
+    log("device memory after the tauNL-loc numerators: %.1f GB allocated" % (torch.cuda.memory_allocated() / 1e9))
     tmf, _ = make_template(base, raw, mask, tables, lmax, True, templates=('gNL-loc', 'tauNL-loc'))
     tmf.compute_fisher_contribution(0)  # warm-up
     torch.cuda.synchronize()
@@ -157,6 +165,7 @@ def bench_tau(base, raw, mask, tables, lmax, nsims, datas, steps=3, fisher_steps
                              "Q derivatives through the masked S^-1; FP32 transforms, FP64 outer product"}
     log("fisher (gNL+tauNL): %.3f s/contribution" % dtf)
     del tmf
+    gc.collect()
     torch.cuda.empty_cache()
     return out
 

@@ -3951,6 +3951,12 @@ static int launch_synth_s0_mfma(const ShtPlan& p, int ncol, int ld, const void* 
 // The wide matrix-core synthesis (shape SYN_SHAPE_WR, k_leg_synth_s0_wr): NT = the column tiles
 // of one block, up to 5 (80 columns); wider batches take several column chunks.
 constexpr int SYN_SHAPE_WR = 65;
+// the wide form with an explicit tile count: SYN_SHAPE_WR_NT + nt, nt = 2 .. 4 (fewer registers and
+// more waves per SIMD, the recursion repeated for each column chunk); autotuner candidates
+constexpr int SYN_SHAPE_WR_NT = 80;
+static bool synth_is_wr(int shape) {
+    return shape == SYN_SHAPE_WR || (shape >= SYN_SHAPE_WR_NT + 2 && shape <= SYN_SHAPE_WR_NT + 4);
+}
 static int synth_lt_nt(int ncol) {
     const int tiles = (ncol + 15) / 16;
     int nt = tiles <= 5 ? tiles : 5;  // wider batches: the tile count of least padding, largest first
@@ -3959,9 +3965,9 @@ static int synth_lt_nt(int ncol) {
     return nt;
 }
 static int launch_synth_s0_wr(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st,
-                              int rb0 = 0, int rb1 = -1) {
+                              int rb0 = 0, int rb1 = -1, int shape = SYN_SHAPE_WR) {
     ShtDev<float> d = dev_view<float>(p);
-    const int nt = synth_lt_nt(ncol);
+    const int nt = shape == SYN_SHAPE_WR ? synth_lt_nt(ncol) : shape - SYN_SHAPE_WR_NT;
     const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * nt - 1) / (16 * nt);
     if (rb1 < 0) rb1 = nrb;
     const unsigned nb = (unsigned)((rb1 - rb0) * nchunk * (p.mmax + 1));
@@ -4006,7 +4012,10 @@ static int launch_synth_s0_any(const ShtPlan& p, int ncol, int ld, const void* a
             if constexpr (sizeof(T) == 4) return launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st);
             else return launch_synth_mfma64(p, ncol, ld, alm_t, phase, st);
         case SYN_SHAPE_WR:
-            if constexpr (sizeof(T) == 4) return launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st);
+        case SYN_SHAPE_WR_NT + 2:
+        case SYN_SHAPE_WR_NT + 3:
+        case SYN_SHAPE_WR_NT + 4:
+            if constexpr (sizeof(T) == 4) return launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st, 0, -1, shape);
             break;
         case 1: return launch_synth_s0<T, 1>(p, ncol, ld, alm_t, phase, st);
         case 2: return launch_synth_s0<T, 2>(p, ncol, ld, alm_t, phase, st);
@@ -4116,10 +4125,10 @@ static int synthesis_overlapped(const ShtPlan& p, int ncol, const void* alm_t, i
     int shape = shape_of(p, ncol, SHT_STAGE_SYNTH);
     if (shape < 0) shape = synth_shape_rule<float>(p, ncol);  // no tuned or set shape: the built-in rule
     if (p.prec != 4 || p.spin != 0 || !pow2_nside || p.nrings == p.n_pow2 || rbA < 1 || rbA >= nrb ||
-        (shape != 0 && shape != SYN_SHAPE_WR))
+        (shape != 0 && !synth_is_wr(shape)))
         return 1;
     auto legendre = [&](int r0, int r1) {
-        return shape == SYN_SHAPE_WR ? launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st, r0, r1)
+        return synth_is_wr(shape) ? launch_synth_s0_wr(p, ncol, ld, alm_t, phase, st, r0, r1, shape)
                                      : launch_synth_s0_mfma(p, ncol, ld, alm_t, phase, st, r0, r1);
     };
     if (!p.side) {
@@ -4374,6 +4383,8 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
             while (top < w && top < bcmax) top *= 2;
             add(0);  // the matrix-core kernel (FP32 or FP64)
             if (p.prec == 4 && p.spin == 0) add(SYN_SHAPE_WR);
+            if (p.prec == 4 && p.spin == 0 && ncol > 16)
+                for (int nt = 2; nt < synth_lt_nt(ncol) && nt <= 4; nt++) add(SYN_SHAPE_WR_NT + nt);
             for (int bc = std::max(1, top / 2); bc <= top; bc *= 2) add(bc);
             break;
         }

@@ -63,7 +63,7 @@ class TSpecTemplate:
         # applySinv must then be capturable (no host synchronisation, e.g. applySinv_ideal)
         self.capture_graph = capture_graph
         # sim_memory_gb: device memory for the filtered simulation maps that the batched tauNL-loc
-        # terms revisit on every numerator (default: half of the free memory when they are made);
+        # terms revisit on every numerator (default: 70 % of the free memory when they are made);
         # simulations beyond it keep only their S^-1-filtered harmonics and are re-synthesised
         self.sim_memory_gb = sim_memory_gb
         # fuse_gnl: when only the gNL-loc sum of the data's P and Q maps is needed (gNL-loc alone,
@@ -73,7 +73,6 @@ class TSpecTemplate:
         self._graph = None
         self._sims_version = 0
         self.kernel_events = None  # set to a list to time the Legendre stage of every filter call
-        self.Pzeta = lambda k: _Pzeta(k, self.As, self.ns, self.k_pivot)
 
         assert self.lmax <= base.lmax, "Maximum l can't be larger than HEALPix resolution!"
         assert self.lmin >= 2, "Minimum l can't be less than 2"
@@ -132,7 +131,13 @@ class TSpecTemplate:
             self.r_weights = {k: np.asarray(v, dtype=np.float64) for k, v in r_weights.items()}
             self._prepare_templates()
 
+    def Pzeta(self, k):
+        """Primordial curvature power spectrum of the template (a method, not a stored closure,
+        so that a template holding device buffers is freed as soon as it is dropped)."""
+        return _Pzeta(k, self.As, self.ns, self.k_pivot)
+
     # ------------------------------------------------------------ precompute
+
     def _prepare_templates(self):
         """p_l^X(r), q_l^X(r) and F_L(r, r') (tspec_template.py:520-708, n = 0 terms)."""
         t0 = time.time()
@@ -762,7 +767,7 @@ class TSpecTemplate:
         per = 2 * self.N_r * self.base.Npix * torch.tensor([], dtype=self.base.rdtype).element_size()
         if self.sim_memory_gb is None:
             free, _ = torch.cuda.mem_get_info(self.base.device)
-            budget = 0.5 * free
+            budget = 0.7 * free
         else:
             budget = self.sim_memory_gb * 2 ** 30
         n_res = int(min(n_sims, budget // per))
