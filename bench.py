@@ -649,7 +649,7 @@ def bench_bspec_cfg4(spec, nbins=20, fp64_sims=2, sim_memory_gb=None):
 def _bspec_shape_report():
     """Launch shapes the binned-bispectrum sum autotuner kept: {segment kinds: (shape id, ms per shape)}."""
     from polyspec_amd import bspec_bin as B
-    return {str(k[0]) + " x%d" % k[1]: {"shape": v[0], "ms_per_shape": {i: round(t, 4) for i, t in v[1].items()}}
+    return {str(k): {"shape": v[0], "ms_per_shape": {i: round(t, 4) for i, t in v[1].items()}}
             for k, v in B._multi_shapes.items()}
 
 

@@ -96,11 +96,13 @@ def multi_sums(segments, outs, shape=-1, accumulate=False):
 
 def _tuned_shape(segments, ngroup, enabled):
     """Launch shape for these segments: autotuned once per set of segment kinds (stack count,
-    stack shape and dtype, triplet count) and segment count, on one segment of each kind."""
+    stack shape and dtype, triplet count), on one segment of each kind -- the tuning run does not
+    depend on how many segments of each kind a launch holds, so neither does the key (rounds of
+    streamed simulations of different sizes share one tuning)."""
     if not enabled:
         return -1
     kinds = sorted(set((len(st), tuple(st[0].shape), str(st[0].dtype), int(tr.shape[0])) for st, tr, _ in segments))
-    key = (tuple(kinds), len(segments))
+    key = tuple(kinds)
     if key not in _multi_shapes:
         sub, seen = [], set()
         for st, tr, g in segments:

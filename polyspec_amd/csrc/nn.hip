@@ -52,19 +52,25 @@ __global__ void k_col2im9(const float* __restrict__ G, const int* __restrict__ r
     }
 }
 
-// C[z] (M x N) = op(A) (M x K-range z) * op(B) (K-range z x N); 64x64x16 tiles, 4x4 per thread.
-// With gridDim.z == 1 the epilogue adds bias[n] and applies ReLU if requested.
+// C[z] (M x N) = op(A) (M x K-range z) * op(B) (K-range z x N); 64x64x16 tiles staged in LDS, each
+// wave one 32 x 32 quarter of the tile on the matrix cores (v_mfma_f32_32x32x2f32: A[i][k] at lane
+// i + 32 k, B[k][j] at lane j + 32 k, D[i][j] at lane j + 32 ((i / 4) % 2), register
+// 4 (i / 8) + i % 4).  With gridDim.z == 1 the epilogue adds bias[n] and applies ReLU if requested.
+typedef float v16f_nn __attribute__((ext_vector_type(16)));
+
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, long lda,
                                               const float* __restrict__ Bm, long ldb, float* __restrict__ C, long ldc,
                                               const float* __restrict__ bias, int relu, int klen) {
     __shared__ float As[16][68];
     __shared__ float Bs[16][68];
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, li32 = lane & 31;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // this wave's quarter of the tile
     const int bm = blockIdx.x * 64, bn = blockIdx.y * 64;
     const int k0 = blockIdx.z * klen;
     const int k1 = min(K, k0 + klen);
-    float acc[4][4] = {{0.f}};
+    v16f_nn acc = v16f_nn{0};
     for (int kk = k0; kk < k1; kk += 16) {
         for (int e = tid; e < 64 * 16; e += 256) {
             // A tile (64 x 16): the contiguous index of the stored matrix varies fastest
@@ -82,35 +88,22 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            float a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = Bs[k][tx * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
-        }
+        for (int k = 0; k < 16; k += 2)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(As[k + h][wm + li32], Bs[k + h][wn + li32], acc, 0, 0, 0);
         __syncthreads();
     }
     float* Cz = C + (long)blockIdx.z * M * ldc;
+    const int n = bn + wn + li32;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int m = bm + ty * 4 + i;
-        if (m >= M) continue;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int n = bn + tx * 4 + j;
-            if (n >= N) continue;
-            float v = acc[i][j];
-            if (gridDim.z == 1) {
-                if (bias) v += bias[n];
-                if (relu) v = fmaxf(v, 0.f);
-            }
-            Cz[(long)m * ldc + n] = v;
+    for (int v = 0; v < 16; v++) {
+        const int m = bm + wm + 8 * (v >> 2) + 4 * h + (v & 3);
+        if (m >= M || n >= N) continue;
+        float val = acc[v];
+        if (gridDim.z == 1) {
+            if (bias) val += bias[n];
+            if (relu) val = fmaxf(val, 0.f);
         }
+        Cz[(long)m * ldc + n] = val;
     }
 }
 
@@ -210,7 +203,6 @@ static unsigned grid1d(long n) { return (unsigned)std::min<long>((n + 255) / 256
 // chunk in.  A neighbour's whole row segment (CPH chunks) is one batch of loads, issued a
 // neighbour ahead of its MFMAs.
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef float v16f_nn __attribute__((ext_vector_type(16)));
 
 template <int NT, int CPH>
 __global__ __launch_bounds__(256) void k_conv9_bf16(const __bf16* __restrict__ X, const int* __restrict__ nbr, int B,
