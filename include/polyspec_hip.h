@@ -62,7 +62,15 @@ int ps_sht_lmajor_multiple(const ps_sht_plan *plan);
 int ps_sht_synthesis(const ps_sht_plan *plan, int nmaps, const void *alm, void *map, void *work,
                      long long work_bytes, void *stream);
 
-/* Same, with alm given l-major: alm_t[idx(l,m) * ld + col], col = map*ncomp + comp,
+/* Spin-s synthesis of (G, C = 0) from G alone: G (nmaps, nlm) complex m-major -> maps
+ * (nmaps, 2, npix), i.e. ps_sht_synthesis of (G, 0) without the C half of the Legendre work.
+ * It replaces compute_spin_transform_map's synthesis of the bin-filtered h (bspec_bin.py:216-228,
+ * 335-337 through base.py:260-288), whose curl part is zero.  FP32 plans of spin >= 1; work:
+ * ps_sht_workspace_bytes(plan, nmaps, 0) bytes. */
+int ps_sht_synthesis_grad(const ps_sht_plan *plan, int nmaps, const void *G, void *map, void *work,
+                          long long work_bytes, void *stream);
+
+/* ps_sht_synthesis with alm given l-major: alm_t[idx(l,m) * ld + col], col = map*ncomp + comp,
  * ld >= nmaps*ncomp a multiple of ps_sht_lmajor_multiple(); padding columns must be zero.
  * FP32 spin-0 batches on the matrix-core shape (Nside a power of two) fork the Bluestein-ring
  * FFTs onto the plan's own side stream and join it back into `stream` before returning (event

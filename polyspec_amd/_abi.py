@@ -25,6 +25,7 @@ SIGNATURES = {
     "ps_sht_workspace_bytes": (_LL, [_P, _I, _I]),
     "ps_sht_lmajor_multiple": (_I, [_P]),
     "ps_sht_synthesis": (_I, [_P, _I, _P, _P, _P, _LL, _P]),
+    "ps_sht_synthesis_grad": (_I, [_P, _I, _P, _P, _P, _LL, _P]),
     "ps_sht_synthesis_lmajor": (_I, [_P, _I, _P, _I, _P, _P, _LL, _P]),
     "ps_sht_adjoint": (_I, [_P, _I, _P, _P, _D, _P, _LL, _P]),
     "ps_sht_phase_bytes": (_I, [_P, _I, ctypes.POINTER(_LL)]),
@@ -252,6 +253,28 @@ class ShtPlan:
             ws = workspace(wb, alm.device)
         assert ws.numel() >= wb, "workspace too small"
         check(load().ps_sht_synthesis(self._h, nm, ptr(alm), ptr(out), ptr(ws), wb, stream_handle()))
+        return out
+
+    def synthesis_grad(self, G, out=None, ws=None):
+        """Spin-s synthesis of (G, C = 0): G (nmaps, nlm) complex -> maps (nmaps, 2, npix) real, the
+        synthesis of (G, 0) without the curl half of the Legendre work (FP32 spin-s plans; other
+        plans run the full synthesis of (G, 0))."""
+        require_device(G, self.cdtype, "G")
+        assert G.dim() == 2 and G.shape[1] == self.nlm, "Wrong G shape"
+        assert self.ncomp == 2, "gradient-only synthesis needs a spin-s plan"
+        nm = G.shape[0]
+        if self.precision != 32:
+            return self.synthesis(torch.stack([G, torch.zeros_like(G)], dim=1), out=out, ws=ws)
+        self._maybe_tune(nm)
+        if out is None:
+            out = torch.empty((nm, 2, self.npix), dtype=self.rdtype, device=G.device)
+        require_device(out, self.rdtype, "map")
+        assert out.shape == (nm, 2, self.npix) and out.is_contiguous()
+        wb = self.work_bytes(nm, 0)
+        if ws is None:
+            ws = workspace(wb, G.device)
+        assert ws.numel() >= wb, "workspace too small"
+        check(load().ps_sht_synthesis_grad(self._h, nm, ptr(G), ptr(out), ptr(ws), wb, stream_handle()))
         return out
 
     def synthesis_lmajor(self, alm_t, nmaps, out=None):

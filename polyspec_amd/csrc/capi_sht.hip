@@ -113,6 +113,26 @@ int ps_sht_synthesis(const ps_sht_plan* plan, int nmaps, const void* alm, void* 
     return sht_synthesis_lmajor(p, nmaps, alm_t, ld, map, p.npix, phase, st);
 }
 
+int ps_sht_synthesis_grad(const ps_sht_plan* plan, int nmaps, const void* G, void* map, void* work,
+                          long long work_bytes, void* stream) {
+    PS_REQUIRE(plan && G && map && work, "ps_sht_synthesis_grad: NULL argument");
+    PS_REQUIRE(nmaps >= 1, "ps_sht_synthesis_grad: nmaps must be >= 1");
+    const ShtPlan& p = plan->p;
+    PS_REQUIRE(p.prec == 4 && p.spin >= 1, "ps_sht_synthesis_grad: FP32 plans of spin >= 1 only");
+    PS_REQUIRE(work_bytes >= ps_sht_workspace_bytes(plan, nmaps, 0), "ps_sht_synthesis_grad: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t phb = (sht_phase_bytes(p, nmaps) + 255) & ~(size_t)255;
+    void* phase = work;
+    void* alm_t = (char*)work + phb;
+    // G of map b in column b, rows of even length (16-byte copies); the kernel reads columns past
+    // ld as zero, and the (G, C) workspace holds 2 nmaps >= ld columns
+    const int ld = (nmaps + 1) / 2 * 2;
+    int rc = sht_alm_to_lmajor(p, nmaps, G, p.nlm, ld, alm_t, st);
+    if (rc) return rc;
+    if ((rc = sht_legendre_synthesis_grad(p, nmaps, alm_t, ld, phase, st))) return rc;
+    return sht_ring_synthesis(p, nmaps, phase, map, p.npix, st);
+}
+
 int ps_sht_alm_to_lmajor(const ps_sht_plan* plan, int ncol, const void* alm, int ld, void* alm_t, void* stream) {
     PS_REQUIRE(plan && alm && alm_t, "ps_sht_alm_to_lmajor: NULL argument");
     PS_REQUIRE(ncol >= 1 && ld >= ncol && ld <= 65535 * 16, "ps_sht_alm_to_lmajor: need 1 <= ncol <= ld <= 1048560");

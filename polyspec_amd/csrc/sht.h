@@ -68,6 +68,10 @@ int sht_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, 
 int sht_legendre_synthesis_lmajor(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
                                   hipStream_t st);
 int sht_ring_synthesis(const ShtPlan& p, int nmap, const void* phase, void* map, long map_stride, hipStream_t st);
+// Legendre stage of the spin-s synthesis of (G, C = 0) from G alone (alm_t column b = G of map b);
+// the phases of the R and I components of nmap maps.  FP32 plans of spin >= 1.
+int sht_legendre_synthesis_grad(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
+                                hipStream_t st);
 // Ring synthesis of 2 nr interleaved columns (a_r, b_r) = (s_r P_r, Q_r / s_r) fused with the
 // gNL-loc sum sum_r w_r sum_pix [c4 P^3 Q + c2 (P^2 U_r + P Q V_r)] (U, V (nr, npix) FP64 or both
 // null; inv_s2 = 1 / s_r^2 or null for s_r = 1) into *out (FP64, added when accumulate); part:

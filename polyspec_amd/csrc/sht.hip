@@ -1056,6 +1056,7 @@ __global__ __launch_bounds__(256, 2) void k_leg_synth_s0_wr(ShtDev<float> d, int
 // i.e. -(D+[j] - D-[j ^ 3]) for q = 0, 3 and -(D+[j] + D-[j ^ 3]) for q = 1, 2; the south
 // ring takes lam+ -> (-1)^(l+m+s) lam+, lam- -> -(-1)^(l+m+s) lam-.  R and I land in the
 // complex columns 2 b and 2 b + 1 of the phase buffer, the layout of the FP64 / few-map kernel.
+template <bool GONLY>
 __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int ncol, int ld,
                                                            const cplx<float>* __restrict__ alm_t,
                                                            cplx<float>* __restrict__ phase) {
@@ -1215,8 +1216,44 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int 
     }
     // ev accumulators hold l - l_start even, i.e. l + m + s even unless l_start + m + s is odd
     const bool swp = ((lstart + m + d.spin) & 1) != 0;
-    const int col = chunk * BCM + (li32 >> 1);
     const bool re = (li32 & 1) == 0;
+    if constexpr (GONLY) {
+        // C = 0 and the tile's 32 real columns are the G of 16 maps (map chunk*16 + li32/2, part
+        // li32 % 2): Re R = -D+[G re], Im R = -D+[G im], Re I = -D-[G im], Im I = +D-[G re],
+        // the lam- terms taken from the partner lane (the other part of the same map)
+        const int map = chunk * BCM + (li32 >> 1);
+        const int ncp = 2 * ncol;  // phase columns: R and I of every map
+        const float sgi = re ? -1.f : 1.f;
+#pragma unroll
+        for (int vv = 0; vv < 16; vv++) {
+            const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+            const int jr = rb * 128 + wave * 32 + i;
+            const float pe = swp ? Pod[vv] : Pev[vv], po = swp ? Pev[vv] : Pod[vv];
+            const float me = swp ? Mod[vv] : Mev[vv], mo = swp ? Mev[vv] : Mod[vv];
+            const float nR = -(pe + po), sR = -(pe - po);
+            const float nI = sgi * __shfl_xor(me + mo, 1), sI = sgi * __shfl_xor(mo - me, 1);
+            const float nR2 = __shfl_xor(nR, 1), sR2 = __shfl_xor(sR, 1);  // the other part
+            const float nI2 = __shfl_xor(nI, 1), sI2 = __shfl_xor(sI, 1);
+            const cplx<float> w = rot[wave * 32 + i];
+            const float noR = re ? nR * w.re - nR2 * w.im : nR2 * w.im + nR * w.re;
+            const float soR = re ? sR * w.re - sR2 * w.im : sR2 * w.im + sR * w.re;
+            const float noI = re ? nI * w.re - nI2 * w.im : nI2 * w.im + nI * w.re;
+            const float soI = re ? sI * w.re - sI2 * w.im : sI2 * w.im + sI * w.re;
+            if (jr < d.npairs && map < ncol) {
+                const int rn = jr, rs = d.nrings - 1 - jr;
+                float* pn = reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncp) + 2 * map);
+                pn[re ? 0 : 1] = noR;
+                pn[re ? 2 : 3] = noI;
+                if (rs != rn) {
+                    float* ps = reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncp) + 2 * map);
+                    ps[re ? 0 : 1] = soR;
+                    ps[re ? 2 : 3] = soI;
+                }
+            }
+        }
+        return;
+    }
+    const int col = chunk * BCM + (li32 >> 1);
     const int q = li32 & 3;
     const float sg = (q == 0 || q == 3) ? -1.f : 1.f;
 #pragma unroll
@@ -4051,8 +4088,25 @@ static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     ShtDev<float> d = dev_view<float>(p);
     const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 15) / 16;
     const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
-    hipLaunchKernelGGL(k_leg_synth_sp_mfma, dim3(nb), dim3(256), 0, st, d, ncol, ld, (const cplx<float>*)alm_t,
-                       (cplx<float>*)phase);
+    hipLaunchKernelGGL(k_leg_synth_sp_mfma<false>, dim3(nb), dim3(256), 0, st, d, ncol, ld,
+                       (const cplx<float>*)alm_t, (cplx<float>*)phase);
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
+// Spin-s synthesis of (G, C = 0) from G alone: alm_t (nlm x ld, column b = G of map b) -> ring
+// phases of the R and I components (2 nmap columns); FP32 plans of spin >= 1.
+int sht_legendre_synthesis_grad(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
+                                hipStream_t st) {
+    if (p.prec != 4 || p.spin < 1) {
+        set_error("gradient-only synthesis: FP32 spin-s plans only");
+        return -1;
+    }
+    ShtDev<float> d = dev_view<float>(p);
+    const int nrb = (p.npairs + 127) / 128, nchunk = (nmap + 15) / 16;
+    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    hipLaunchKernelGGL(k_leg_synth_sp_mfma<true>, dim3(nb), dim3(256), 0, st, d, nmap, ld,
+                       (const cplx<float>*)alm_t, (cplx<float>*)phase);
     PS_LAUNCH_CHECK();
     return 0;
 }
