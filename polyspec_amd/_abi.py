@@ -233,6 +233,12 @@ class ShtPlan:
         self._tuned[int(nmaps)] = res
         return res
 
+    def skip_tuning(self, nmaps):
+        """Keep the built-in shapes for nmaps maps if no shapes were tuned or set for that width
+        (a transform of a new width then runs without the tuning pass of autotune=True)."""
+        if self.autotune_on_first_use and int(nmaps) not in self._tuned:
+            self.set_shapes(nmaps, [-1] * 4)
+
     def _maybe_tune(self, nmaps):
         if self.autotune_on_first_use and int(nmaps) not in self._tuned and \
                 not torch.cuda.is_current_stream_capturing():

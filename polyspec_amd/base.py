@@ -122,6 +122,7 @@ class PolySpec:
         # draw Gaussian random fields with the device generator (fast, large maps)
         # instead of numpy on the host
         self.grf_device = False
+        self._grf_cache = {}  # synalm's per-l factors and index arrays by (lmax, spectra)
 
     # ------------------------------------------------------------ plumbing
     def timer_func(func):
@@ -247,9 +248,10 @@ class PolySpec:
             p0.synthesis(a[0:1].reshape(1, 1, -1), out=out[0:1].view(1, 1, -1))
         return self._ret(out, npy)
 
-    def to_lm_batch(self, maps, lmax=None):
+    def to_lm_batch(self, maps, lmax=None, tune=True):
         """to_lm of a stack of map sets, (B, nf, Npix) -> (B, nf, nlm), nf = 1 or 3: one batched
-        adjoint per spin over all B sets (the reference transforms one set per call)."""
+        adjoint per spin over all B sets (the reference transforms one set per call).  tune=False
+        keeps the built-in launch shapes for a batch width the plans have not tuned."""
         B, nf = maps.shape[0], maps.shape[1]
         assert nf in (1, 3), "Wrong input shape supplied!"
         lmax = self.lmax if lmax is None else lmax
@@ -257,11 +259,15 @@ class PolySpec:
         m = self._to_dev(maps, False).reshape(B, nf, self.Npix)
         sc = np.pi / (3. * self.Nside ** 2)
         p0 = self.plan(lmax, 0)
+        plans = [p0] if nf == 1 else [p0, self.plan(lmax, 2)]
+        if not tune:
+            for p in plans:
+                p.skip_tuning(B)
         if nf == 1:
             return p0.adjoint(m, sc)
         out = torch.empty((B, 3, p0.nlm), dtype=self.cdtype, device=self.device)
         out[:, 0:1] = p0.adjoint(m[:, 0:1].contiguous(), sc)
-        out[:, 1:3] = self.plan(lmax, 2).adjoint(m[:, 1:3].contiguous(), sc)
+        out[:, 1:3] = plans[1].adjoint(m[:, 1:3].contiguous(), sc)
         return out
 
     def to_map_batch(self, alms, lmax=None):
@@ -382,15 +388,13 @@ class PolySpec:
         return out
 
     # ---------------------------------------------------------- utilities
-    def synalm(self, seed, Cl_input, lmax=None):
-        """Seeded Gaussian a_lm with covariance C_l (the role of healpy.synalm at
-        base.py:332-334; this generator's random stream is its own).
-
-        Per l, a Cholesky factor of the (1x1 or 3x3) C_l draws correlated
-        T[,E,B]; m = 0 coefficients are real with variance C_l, m > 0 have
-        independent real/imaginary parts of variance C_l/2."""
-        lmax = self.lmax if lmax is None else lmax
-        ls, ms = getlm(lmax)
+    def _grf_factors(self, Cl_input, lmax):
+        """Per-l factors L_l with L_l L_l^T = C_l for the spectra Cl_input (an eigen-decomposition
+        square root, also valid for singular C_l), with the l and m of every (l, m): numpy arrays,
+        and with grf_device the factors, the l index and the m = 0 mask on the device.  Cached by
+        lmax and the spectra's values, so that a run of draws does no per-draw host work (a
+        per-l decomposition loop and host-to-device copies of the index arrays stalled the device
+        queue between draws)."""
         nf = 3 if self.pol else 1
         keys = [['TT', 'TE', 'TB'], ['TE', 'EE', 'EB'], ['TB', 'EB', 'BB']]
         C = np.zeros((lmax + 1, nf, nf))
@@ -401,24 +405,44 @@ class PolySpec:
                     C[:, a, b] = np.asarray(Cl_input[k])[:lmax + 1]
                 elif k[::-1] in Cl_input:
                     C[:, a, b] = np.asarray(Cl_input[k[::-1]])[:lmax + 1]  # TB/EB may be absent: zero
-        Lc = np.zeros_like(C)
-        for l in range(lmax + 1):
-            # eigen-decomposition square root: also valid for singular C_l
-            w, v = np.linalg.eigh(C[l])
-            Lc[l] = v * np.sqrt(np.clip(w, 0, None))[None, :]
+        key = (lmax, C.tobytes())
+        ent = self._grf_cache.get(key)
+        if ent is None:
+            w, v = np.linalg.eigh(C)  # every l at once
+            ls, ms = getlm(lmax)
+            ent = {'Lc': v * np.sqrt(np.clip(w, 0, None))[:, None, :], 'ls': ls, 'ms': ms}
+            if len(self._grf_cache) >= 4:
+                self._grf_cache.clear()
+            self._grf_cache[key] = ent
+        if self.grf_device and 'Lt' not in ent:
+            ent['Lt'] = torch.from_numpy(ent['Lc']).to(self.device)
+            ent['li'] = torch.from_numpy(ent['ls']).to(self.device)
+            ent['m0'] = torch.from_numpy(ent['ms'] == 0).to(self.device)
+        return ent
+
+    def synalm(self, seed, Cl_input, lmax=None):
+        """Seeded Gaussian a_lm with covariance C_l (the role of healpy.synalm at
+        base.py:332-334; this generator's random stream is its own).
+
+        Per l, a square-root factor of the (1x1 or 3x3) C_l draws correlated
+        T[,E,B]; m = 0 coefficients are real with variance C_l, m > 0 have
+        independent real/imaginary parts of variance C_l/2."""
+        lmax = self.lmax if lmax is None else lmax
+        nf = 3 if self.pol else 1
+        f = self._grf_factors(Cl_input, lmax)
+        ls, ms = f['ls'], f['ms']
         if self.grf_device:
             g = torch.Generator(device=self.device)
             g.manual_seed(int(seed) if seed is not None else 0)
             zr = torch.randn((nf, len(ls)), dtype=torch.float64, device=self.device, generator=g)
             zi = torch.randn((nf, len(ls)), dtype=torch.float64, device=self.device, generator=g)
-            m0 = torch.from_numpy(ms == 0).to(self.device)
+            m0 = f['m0']
             zr = torch.where(m0, zr * np.sqrt(2.), zr)
             zi = torch.where(m0, torch.zeros_like(zi), zi)
             z = torch.complex(zr, zi) / np.sqrt(2.)
             # out[a] = sum_b Lc[l, a, b] z[b]: elementwise over (l, m) (a batched 3 x 3 product per
             # (l, m) as a library GEMM took 16 ms per Nside-1024 draw)
-            Lt = torch.from_numpy(Lc).to(self.device)
-            li = torch.from_numpy(ls).to(self.device)
+            Lt, li = f['Lt'], f['li']
             out = torch.empty((nf, len(ls)), dtype=torch.complex128, device=self.device)
             for a in range(nf):
                 acc = z[0] * Lt[:, a, 0][li]
@@ -430,7 +454,7 @@ class PolySpec:
         z = rng.standard_normal((nf, len(ls))) + 1j * rng.standard_normal((nf, len(ls)))
         z[:, ms == 0] = z[:, ms == 0].real * np.sqrt(2.)
         z /= np.sqrt(2.)
-        return np.einsum('lab,bl->al', Lc[ls], z)
+        return np.einsum('lab,bl->al', f['Lc'][ls], z)
 
     def generate_data(self, seed=None, Cl_input={}, output_type='map', deconvolve_beam=False, b_input=None,
                       add_B=False, remove_mean=True, sum_ells='even', lmax=None):
@@ -451,13 +475,15 @@ class PolySpec:
             if len(Cl_input.keys()) == 4:
                 Cl_input['TB'] = Cl_input['EB'] = 0. * Cl_input['TT']
         initial_lm = self.synalm(seed, Cl_input, self.lmax)
-        sel = self.l_arr <= lmax
-        if isinstance(initial_lm, torch.Tensor):
+        sel = self.l_arr <= lmax if lmax < self.lmax else None  # None: every (l, m) kept
+        if sel is None:
+            pass
+        elif isinstance(initial_lm, torch.Tensor):
             initial_lm = initial_lm[:, torch.from_numpy(sel).to(self.device)].contiguous()
         else:
             initial_lm = initial_lm[:, sel]
         if deconvolve_beam:
-            this_beam = self.beam_lm[:, sel]
+            this_beam = self.beam_lm if sel is None else self.beam_lm[:, sel]
             inv = np.where(this_beam != 0, 1. / np.where(this_beam != 0, this_beam, 1.), 0.)
             if isinstance(initial_lm, torch.Tensor):
                 initial_lm = initial_lm * torch.from_numpy(inv).to(self.device)

@@ -133,23 +133,28 @@ def _numerator_segments(bs, data, include_linear_term, g_cub, g_lin):
 
 
 def _stream_rounds(streamed, budget):
-    """Segments of the streamed simulations in rounds: each round makes the H maps of the next
-    simulation of every analysis in turn (round-robin over resolutions) until the round's maps
-    would exceed `budget` bytes (at least one simulation per round)."""
+    """Segments of the streamed simulations in rounds: each round takes the next simulation of
+    every analysis in turn (round-robin over resolutions) until the round's H maps would exceed
+    `budget` bytes (at least one simulation per round), then makes the H maps of each analysis's
+    share of the round (BSpecBin._load_sims_data: one batched S^-1 for them where S^-1 allows)."""
     nxt = [0] * len(streamed)
     while any(nxt[k] < st[0].N_it for k, st in enumerate(streamed)):
-        segs, used, added = [], 0, True
+        take, used, added = [0] * len(streamed), 0, True
         while added:
             added = False
             for k, (bs, p1, m2, g) in enumerate(streamed):
                 per = 2 * p1.numel() * p1.element_size()
-                if nxt[k] >= bs.N_it or (segs and used + per > budget):
+                if nxt[k] + take[k] >= bs.N_it or (used and used + per > budget):
                     continue
-                sp1, sm2 = bs.load_sim_data(nxt[k])
-                nxt[k] += 1
+                take[k] += 1
                 used += per
-                segs.append(((p1, sp1, m2, sm2), bs._trip, g))
                 added = True
+        segs = []
+        for k, (bs, p1, m2, g) in enumerate(streamed):
+            idx = list(range(nxt[k], nxt[k] + take[k]))
+            nxt[k] += take[k]
+            for sp1, sm2 in (bs._load_sims_data(idx) if idx else []):
+                segs.append(((p1, sp1, m2, sm2), bs._trip, g))
         yield segs
 
 
@@ -236,6 +241,9 @@ class BSpecBin:
         # 16 simulations); Bl_numerator_multi, which joins analyses of several resolutions into
         # one launch, always uses it.
         self.fused_sums = False
+        # streamed simulations filtered together by a batched S^-1 (Bl_numerator_multi's rounds):
+        # at most this many per batch, which bounds the batch's maps and transform workspace
+        self.sinv_batch = 16
         self.mask = mask
         self.applySinv = applySinv
         self.l_bins = np.asarray(l_bins)
@@ -409,6 +417,46 @@ class BSpecBin:
         """S^-1 then H maps of one simulation (bspec_bin.py:216-228)."""
         return self.H_maps(self.applySinv(sim, input_type=input_type))
 
+    def _raw_sim(self, ii):
+        """Simulation ii of the streamed set and its input type ('map' or 'harmonic')."""
+        fn, kind = self._sim_loader
+        return fn(ii) if kind is None else (fn(ii), kind)
+
+    def load_sim_data(self, ii):
+        """H maps (p1, m2) of streamed simulation ii (load_sims / generate_sims, preload=False)."""
+        return self._process_sim(*self._raw_sim(ii))
+
+    def _sinv_batch(self):
+        """The batched form of applySinv, when it is Weightings.applySinv_ideal; else None."""
+        W = getattr(self.applySinv, '__self__', None)
+        fn = getattr(type(W), 'applySinv_ideal', None)
+        if fn is None or getattr(self.applySinv, '__func__', None) is not fn:
+            return None
+        return W.applySinv_ideal_batch
+
+    def _load_sims_data(self, idx):
+        """load_sim_data of the streamed simulations idx.  With S^-1 = Weightings.applySinv_ideal
+        and pixel-space simulations, the filter of up to sinv_batch of them runs as one batched
+        adjoint per spin (matrix-core kernels) rather than one single-map adjoint pair each, with the
+        built-in launch shapes (no tuning pass for a new batch width inside a streamed run); the H
+        maps are made per simulation.  A load_sim_data set on the instance is called per index."""
+        batch = self._sinv_batch() if len(idx) > 1 and 'load_sim_data' not in self.__dict__ else None
+        if batch is None:
+            return [self.load_sim_data(i) for i in idx]
+        out = []
+        for c in range(0, len(idx), self.sinv_batch):
+            raw = [self._raw_sim(i) for i in idx[c:c + self.sinv_batch]]
+            if len(raw) == 1 or any(kind != 'map' for _, kind in raw):
+                out += [self._process_sim(m, input_type=kind) for m, kind in raw]
+                continue
+            maps = torch.stack([torch.as_tensor(m).to(self.base.device, self.base.rdtype).reshape(-1, self.base.Npix)
+                                for m, _ in raw])
+            del raw
+            h = batch(maps, tune=False)
+            del maps
+            out += [self.H_maps(h[j]) for j in range(h.shape[0])]
+        return out
+
     def _sums(self, stacks, scale, out, accumulate):
         """ps_bspec_triple_sums on (A, B, C) = stacks, or ps_bspec_linear_sums on (P, Q, M, N)."""
         ntrip = self._trip.shape[0]
@@ -455,7 +503,8 @@ class BSpecBin:
         else:
             if verb:
                 print("No preloading; simulations will be loaded and accessed at runtime.")
-            self.load_sim_data = lambda ii: self._process_sim(load_sim(ii), input_type=input_type)
+            self.__dict__.pop('load_sim_data', None)
+            self._sim_loader = (load_sim, input_type)
 
     def generate_sims(self, N_sim, Cl_input=[], b_input=None, add_B=False, remove_mean=True, verb=False,
                       preload=True):
@@ -465,14 +514,15 @@ class BSpecBin:
         if len(Cl_input) == 0:
             Cl_input = self.base.Cl_tot
         mask_d = torch.as_tensor(np.asarray(self.mask)).to(self.base.device, self.base.rdtype)
+        base, ones_mask = self.base, self.ones_mask  # not self: the loader is stored on self
 
         def sim(ii):
-            if self.ones_mask:
-                return self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
-                                               remove_mean=remove_mean, output_type='harmonic'), 'harmonic'
-            m = self.base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
-                                        remove_mean=remove_mean)
-            m = torch.as_tensor(m).to(self.base.device, self.base.rdtype)
+            if ones_mask:
+                return base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
+                                          remove_mean=remove_mean, output_type='harmonic'), 'harmonic'
+            m = base.generate_data(int(1e5) + ii, Cl_input=Cl_input, deconvolve_beam=False,
+                                   remove_mean=remove_mean)
+            m = torch.as_tensor(m).to(base.device, base.rdtype)
             return (mask_d * m).contiguous(), 'map'
 
         self.N_it = N_sim
@@ -488,10 +538,8 @@ class BSpecBin:
                 self.p1_H_maps.append(p1)
                 self.m2_H_maps.append(m2)
         else:
-            def load_sim_data(ii):
-                s, kind = sim(ii)
-                return self._process_sim(s, input_type=kind)
-            self.load_sim_data = load_sim_data
+            self.__dict__.pop('load_sim_data', None)
+            self._sim_loader = (sim, None)
 
     # -------------------------------------------------------- estimator
     def Bl_numerator(self, data, include_linear_term=True, verb=False):

@@ -143,6 +143,20 @@ class Weightings:
             raise AssertionError("input_type must be 'map' or 'harmonic'")
         return self.base._ret(out, npy)
 
+    def applySinv_ideal_batch(self, maps, lmax=None, tune=True):
+        """applySinv_ideal(input_type='map') of a stack of map sets, (B, nf, Npix) -> (B, nf, nlm(lmax))
+        on the device: one batched adjoint per spin for all B sets (matrix-core kernels from a few
+        sets on), then the per-l filter B C_tot^-1 of every set (weighting.py:186-214 on each set).
+        tune=False keeps the built-in launch shapes for a batch width not tuned yet (no tuning pass)."""
+        if lmax is None:
+            lmax = self.base.lmax
+        Ltot = self.invCtot_d[:, :, :lmax + 1].contiguous()
+        bl = self.beam_l_d[:, :lmax + 1].contiguous()
+        m = maps if isinstance(maps, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(maps))
+        m = m.to(self.base.device, self.base.rdtype).reshape(m.shape[0], self.nf, -1).contiguous()
+        alm = self.base.to_lm_batch(m, lmax=lmax, tune=tune)
+        return K.lmat_apply_batch(alm, lmax, mat=Ltot, post=bl)
+
     def apply_A(self, x):
         """The CG system matrix A x = C^-1 x + B Y^dag W N^-1 W Y B x at base_lmax
         (weighting.py:297-299); x (nf, nlm(base_lmax)) on the device."""

@@ -264,3 +264,68 @@ def test_gpu_bspec_fused_and_multi_resolution(gpu, prec):
     mixed = Bl_numerator_multi([r[0] for r in runs], [r[1] for r in runs], sim_memory_gb=per32 / 2 ** 30)
     for (bs, d, fused), m in zip(runs, mixed):
         assert np.abs(m - fused).max() < tol * np.abs(fused).max()
+
+
+def test_stream_rounds_plan():
+    """Rounds of streamed simulations: round-robin over analyses while the round's H maps fit the
+    budget (at least one simulation per round), each analysis's share of a round loaded in one
+    call, in simulation order."""
+    import torch
+    from polyspec_amd.bspec_bin import _stream_rounds
+
+    class Fake:
+        def __init__(self, n, tag):
+            self.N_it, self.tag, self._trip, self.calls = n, tag, tag + '_trip', []
+
+        def _load_sims_data(self, idx):
+            self.calls.append(list(idx))
+            return [((self.tag, i), (self.tag, -i)) for i in idx]
+
+    a, b = Fake(5, 'a'), Fake(2, 'b')
+    pa = torch.zeros(10, dtype=torch.float32)  # 80 bytes of H maps per simulation (two stacks)
+    pb = torch.zeros(20, dtype=torch.float32)  # 160 bytes
+    rounds = list(_stream_rounds([(a, pa, pa, 3), (b, pb, pb, 7)], 400))
+    # round 1: a0 (80), b0 (240), a1 (320), b1 would be 480, a2 (400); round 2: a3, b1, a4
+    assert a.calls == [[0, 1, 2], [3, 4]] and b.calls == [[0], [1]]
+    assert [[(s[0][1], s[2]) for s in r] for r in rounds] == [
+        [(('a', 0), 3), (('a', 1), 3), (('a', 2), 3), (('b', 0), 7)],
+        [(('a', 3), 3), (('a', 4), 3), (('b', 1), 7)]]
+    for r in rounds:
+        for st, trip, g in r:
+            tag = st[1][0]
+            assert st[0] is (pa if tag == 'a' else pb) and st[3] == (tag, -st[1][1]) and trip == tag + '_trip'
+    # a budget below one simulation: one simulation per round
+    c = Fake(3, 'c')
+    assert [len(r) for r in _stream_rounds([(c, pa, pa, 0)], 10)] == [1, 1, 1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [64, 32])
+def test_gpu_streamed_sims_batched_sinv(gpu, prec):
+    """Streamed simulations filtered by the batched ideal S^-1 (Weightings.applySinv_ideal_batch,
+    chunks of sinv_batch simulations in a round) against the same simulations preloaded, and the
+    batched filter against one applySinv_ideal call per map set."""
+    import torch
+    from polyspec_amd.bspec_bin import Bl_numerator_multi
+    print(f"bspec streamed batched S^-1 prec={prec}", flush=True)
+    base, W, bs, bins, fields, datas = _setup(32, True, prec, 5)
+    sims = [torch.as_tensor(d).to(base.device, base.rdtype).contiguous() for d in datas[1:]]
+    # FP32: the batched and the single-map adjoints are different kernels (matrix-core and vector)
+    # whose rounding differs by up to about 1e-5 of the largest filtered coefficient (C_tot^-1
+    # raises the high-l ones); a wrong simulation, field or index differs at order one
+    tol = 1e-12 if prec == 64 else 5e-5
+    hb = W.applySinv_ideal_batch(torch.stack(sims))
+    assert hb.shape == (5, 3, base.plan(base.lmax, 0).nlm)
+    for j, m in enumerate(sims):
+        h1 = W.applySinv_ideal(m)
+        assert float((hb[j] - h1).abs().max()) <= tol * float(h1.abs().max()), j
+    bs.load_sims(lambda i: sims[i], 5, preload=True)
+    ref = Bl_numerator_multi([bs], [datas[0]])[0]
+    bs.load_sims(lambda i: sims[i], 5, preload=False)
+    assert bs._sinv_batch() is not None
+    bs.sinv_batch = 2  # one round of five: batches of 2, 2 and a single simulation
+    got = Bl_numerator_multi([bs], [datas[0]], sim_memory_gb=1.0)[0]
+    assert np.abs(got - ref).max() < tol * np.abs(ref).max()
+    # a budget below one simulation's maps: one simulation per round, the single-simulation path
+    one = Bl_numerator_multi([bs], [datas[0]], sim_memory_gb=1e-9)[0]
+    assert np.abs(one - ref).max() < tol * np.abs(ref).max()
