@@ -1276,6 +1276,222 @@ __global__ __launch_bounds__(256) void k_leg_synth_sp_mfma(ShtDev<float> d, int 
     }
 }
 
+// ----------------------- Legendre synthesis, spin s, FP32 matrix cores, wide form
+// k_leg_synth_sp_mfma's tiling and output combination with the structure of k_leg_synth_s0_wr:
+// one block covers NT tiles of 16 complex columns (8 maps of (G, C), or with GONLY the G of 16
+// maps), so the two FP64 recursions of a ring pair (lam+ and lam-) feed 4 NT MFMAs per group of
+// four l instead of 4; the alm rows and recursion coefficients of the next chunk are copied
+// global -> LDS by global_load_lds_dwordx4 while the current chunk runs (one barrier per chunk).
+// Rows past the end of the transform and columns past ld read clamped, finite entries: such rows
+// only meet zero recursion values and such columns land in outputs that are not stored.
+template <int NT, bool GONLY>
+__global__ __launch_bounds__(256) void k_leg_synth_sp_wr(ShtDev<float> d, int ncol, int ld,
+                                                         const cplx<float>* __restrict__ alm_t,
+                                                         cplx<float>* __restrict__ phase) {
+    constexpr int BCM = 16 * NT;
+    constexpr int RW = 2 * BCM;
+    __shared__ __align__(16) float sa0[LCH * RW], sa1[LCH * RW];
+    __shared__ __align__(16) double sco0[LCH * 4], sco1[LCH * 4];  // (alpha, beta, gamma, 0) per l
+    __shared__ cplx<float> rot[128];
+    const int nrb = (d.npairs + 127) / 128;
+    const int nchunk = (ncol + BCM - 1) / BCM;
+    unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = (int)(lb % nrb);
+    const int chunk = (lb / nrb) % nchunk;
+    const int m = __builtin_amdgcn_readfirstlane(m_of_q(lb / (nrb * nchunk), d.mmax));
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int h = lane >> 5, li32 = lane & 31;
+    const int jp = rb * 128 + wave * 32 + li32;
+    const int jpc = jp < d.npairs ? jp : d.npairs - 1;
+    const int M1 = d.mmax + 1;
+    if (tid < 128) {
+        const int jr = min(rb * 128 + tid, d.npairs - 1);
+        const RingInfo ri = d.rings[jr];
+        rot[tid] = expi_pi<float>((long)m * ri.phi_p, ri.phi_q);  // exp(i m phi0)
+    }
+    v16f Pev[NT], Pod[NT], Mev[NT], Mod[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) Pev[c] = Pod[c] = Mev[c] = Mod[c] = v16f{0};
+    const long base = (long)m * (2 * d.lmax + 1 - m) / 2;
+    const int lstart = m > d.spin ? m : d.spin;
+    const int skip = __builtin_amdgcn_readfirstlane(__syncthreads_and(d.dead[(size_t)m * d.npairs + jpc]));
+    if (!skip) {
+        const double x = d.cth[jpc];
+        Rec rp, rm;
+        rec_seed(rp, d.seedL[m] + d.seedA[m] * d.l2sh[jpc] + d.seedB[m] * d.l2ch[jpc], d.seedS[m]);
+        rec_seed(rm, d.seedL[M1 + m] + d.seedA[M1 + m] * d.l2sh[jpc] + d.seedB[M1 + m] * d.l2ch[jpc],
+                 d.seedS[M1 + m]);
+        const int nl = d.lmax + 1 - lstart;
+        const int nch = (nl + LCH - 1) / LCH;
+        static_assert(LCH * RW == 4 * 256 * NT, "NT 16-byte copies per thread per chunk");
+        static_assert(LCH * 4 * 8 == 64 * 16, "one wave copies a chunk's coefficients");
+        auto stage = [&](int c, float* dsa, double* dsco) {
+            const int l0n = lstart + c * LCH, nn = min(LCH, nl - c * LCH);
+#pragma unroll
+            for (int e = 0; e < NT; e++) {
+                const int q = tid + 256 * e, li = min(q / (8 * NT), nn - 1), c4 = q % (8 * NT);
+                const int col = min(chunk * BCM + 2 * c4, ld - 2);  // ld is even
+                const float* src = reinterpret_cast<const float*>(alm_t) + ((size_t)(base + l0n + li) * ld + col) * 2;
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)&dsa[4 * (256 * e + 64 * wave)], 16, 0,
+                                                 0);
+            }
+            if (wave == 0) {
+                const int l = min(l0n + (lane >> 1), d.lmax);
+                const char* src = reinterpret_cast<const char*>(d.coef + 4 * (base + l)) + 16 * (lane & 1);
+                __builtin_amdgcn_global_load_lds((const void*)src, (ps_lds_void*)dsco, 16, 0, 0);
+            }
+        };
+        stage(0, sa0, sco0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        bool fast = __all(rp.k == 0 && rm.k == 0);
+        auto run_chunk = [&](int c, const float* __restrict__ sab, const double* __restrict__ co) {
+            const int n = __builtin_amdgcn_readfirstlane(min(LCH, nl - c * LCH));
+            for (int g = 0; g < n; g += 4) {
+                float bev[NT], bod[NT];
+#pragma unroll
+                for (int cc = 0; cc < NT; cc++) {
+                    bev[cc] = sab[(g + 2 * h) * RW + 32 * cc + li32];
+                    bod[cc] = sab[(g + 1 + 2 * h) * RW + 32 * cc + li32];
+                }
+                double vp[4], vm[4];
+                bool live = true;
+                if (g + 4 <= n) {  // a full group: four steps of each recursion, no tail tests
+                    double cp[4], cm[4], cg[4];
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        const double al = co[4 * (g + cc)], be = co[4 * (g + cc) + 1];
+                        cp[cc] = al * (x + be);
+                        cm[cc] = al * (x - be);
+                        cg[cc] = co[4 * (g + cc) + 2];
+                    }
+                    if (fast) {
+#pragma unroll
+                        for (int cc = 0; cc < 4; cc++) {
+                            vp[cc] = 0.5 * (rp.f + rm.f);
+                            vm[cc] = 0.5 * (rp.f - rm.f);
+                            rec_fast(rp, cp[cc], cg[cc]);
+                            rec_fast(rm, cm[cc], cg[cc]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int cc = 0; cc < 4; cc++) {
+                            const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                            vp[cc] = 0.5 * (fp + fm);
+                            vm[cc] = 0.5 * (fp - fm);
+                            rec_step(rp, cp[cc], cg[cc]);
+                            rec_step(rm, cm[cc], cg[cc]);
+                        }
+                    }
+                } else {  // the last, partial group of the transform
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        if (g + cc < n) {
+                            const double fp = rp.f * rp.cf, fm = rm.f * rm.cf;
+                            vp[cc] = 0.5 * (fp + fm);
+                            vm[cc] = 0.5 * (fp - fm);
+                            const double al = co[4 * (g + cc)], be = co[4 * (g + cc) + 1], ga = co[4 * (g + cc) + 2];
+                            rec_step(rp, al * (x + be), ga);
+                            rec_step(rm, al * (x - be), ga);
+                        } else {
+                            vp[cc] = vm[cc] = 0.0;
+                        }
+                    }
+                }
+                if (!fast) {
+                    rec_rescale(rp);
+                    rec_rescale(rm);
+                    fast = __all(rp.k == 0 && rm.k == 0);
+                    // all values below FP32 range: skip the products
+                    live = __any((rp.k >= scale_consts<float>::KMIN - 1) || (rm.k >= scale_consts<float>::KMIN - 1));
+                }
+                if (!live) continue;
+                const float ape = (float)(h ? vp[2] : vp[0]), apo = (float)(h ? vp[3] : vp[1]);
+                const float ame = (float)(h ? vm[2] : vm[0]), amo = (float)(h ? vm[3] : vm[1]);
+#pragma unroll
+                for (int cc = 0; cc < NT; cc++) {
+                    Pev[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(ape, bev[cc], Pev[cc], 0, 0, 0);
+                    Pod[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(apo, bod[cc], Pod[cc], 0, 0, 0);
+                    Mev[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(ame, bev[cc], Mev[cc], 0, 0, 0);
+                    Mod[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(amo, bod[cc], Mod[cc], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // the next chunk's copies have landed in LDS
+            __syncthreads();
+        };
+        for (int c = 0; c < nch; c += 2) {
+            if (c + 1 < nch) stage(c + 1, sa1, sco1);
+            run_chunk(c, sa0, sco0);
+            if (c + 1 < nch) {
+                if (c + 2 < nch) stage(c + 2, sa0, sco0);
+                run_chunk(c + 1, sa1, sco1);
+            }
+        }
+    }
+    // the output combination of k_leg_synth_sp_mfma, tile by tile
+    const bool swp = ((lstart + m + d.spin) & 1) != 0;
+    const bool re = (li32 & 1) == 0;
+#pragma unroll
+    for (int cc = 0; cc < NT; cc++) {
+        if constexpr (GONLY) {
+            const int map = chunk * BCM + 16 * cc + (li32 >> 1);
+            const int ncp = 2 * ncol;
+            const float sgi = re ? -1.f : 1.f;
+#pragma unroll
+            for (int vv = 0; vv < 16; vv++) {
+                const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+                const int jr = rb * 128 + wave * 32 + i;
+                const float pe = swp ? Pod[cc][vv] : Pev[cc][vv], po = swp ? Pev[cc][vv] : Pod[cc][vv];
+                const float me = swp ? Mod[cc][vv] : Mev[cc][vv], mo = swp ? Mev[cc][vv] : Mod[cc][vv];
+                const float nR = -(pe + po), sR = -(pe - po);
+                const float nI = sgi * __shfl_xor(me + mo, 1), sI = sgi * __shfl_xor(mo - me, 1);
+                const float nR2 = __shfl_xor(nR, 1), sR2 = __shfl_xor(sR, 1);
+                const float nI2 = __shfl_xor(nI, 1), sI2 = __shfl_xor(sI, 1);
+                const cplx<float> w = rot[wave * 32 + i];
+                const float noR = re ? nR * w.re - nR2 * w.im : nR2 * w.im + nR * w.re;
+                const float soR = re ? sR * w.re - sR2 * w.im : sR2 * w.im + sR * w.re;
+                const float noI = re ? nI * w.re - nI2 * w.im : nI2 * w.im + nI * w.re;
+                const float soI = re ? sI * w.re - sI2 * w.im : sI2 * w.im + sI * w.re;
+                if (jr < d.npairs && map < ncol) {
+                    const int rn = jr, rs = d.nrings - 1 - jr;
+                    float* pn = reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncp) + 2 * map);
+                    pn[re ? 0 : 1] = noR;
+                    pn[re ? 2 : 3] = noI;
+                    if (rs != rn) {
+                        float* ps = reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncp) + 2 * map);
+                        ps[re ? 0 : 1] = soR;
+                        ps[re ? 2 : 3] = soI;
+                    }
+                }
+            }
+        } else {
+            const int col = chunk * BCM + 16 * cc + (li32 >> 1);
+            const int q = li32 & 3;
+            const float sg = (q == 0 || q == 3) ? -1.f : 1.f;
+#pragma unroll
+            for (int vv = 0; vv < 16; vv++) {
+                const int i = 8 * (vv >> 2) + 4 * h + (vv & 3);
+                const int jr = rb * 128 + wave * 32 + i;
+                const float pe = swp ? Pod[cc][vv] : Pev[cc][vv], po = swp ? Pev[cc][vv] : Pod[cc][vv];
+                const float me = swp ? Mod[cc][vv] : Mev[cc][vv], mo = swp ? Mev[cc][vv] : Mod[cc][vv];
+                const float mn3 = __shfl_xor(me + mo, 3), ms3 = __shfl_xor(mo - me, 3);
+                const float nv = -((pe + po) + sg * mn3), sv = -((pe - po) + sg * ms3);
+                const float nv2 = __shfl_xor(nv, 1), sv2 = __shfl_xor(sv, 1);
+                const cplx<float> w = rot[wave * 32 + i];
+                const float no = re ? nv * w.re - nv2 * w.im : nv2 * w.im + nv * w.re;
+                const float so = re ? sv * w.re - sv2 * w.im : sv2 * w.im + sv * w.re;
+                if (jr < d.npairs && col < ncol) {
+                    const int rn = jr, rs = d.nrings - 1 - jr;
+                    reinterpret_cast<float*>(phase + syn_phase_off(rn, m, d.mmax, ncol) + col)[re ? 0 : 1] = no;
+                    if (rs != rn)
+                        reinterpret_cast<float*>(phase + syn_phase_off(rs, m, d.mmax, ncol) + col)[re ? 0 : 1] = so;
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------- Legendre syntheses, spin 0 and spin s, FP64 matrix cores
 // F_m(ring, col) = sum_l lambda_l(ring) a_lm(col) on v_mfma_f64_16x16x4f64 (A[i][k] at lane
 // i + 16 k, B[k][j] at lane j + 16 k, D[i][j] at lane j + 16 (i % 4), register i / 4).  The
@@ -4021,9 +4237,10 @@ static int launch_synth_s0_wr(const ShtPlan& p, int ncol, int ld, const void* al
     return 0;
 }
 // Legendre-synthesis shapes: 0 = the matrix-core kernel (FP32, or FP64), SYN_SHAPE_WR = its wide form
-// (FP32 spin 0), otherwise the batch columns (spin 0) or maps (spin s) per block of the vector
-// kernel.  Measured at Nside 256 and 512: the wide form is fastest from 24 columns up (1.2-1.4x),
-// the 16-column one from 5 to 16 columns, the vector kernel up to 4.
+// (FP32), otherwise the batch columns (spin 0) or maps (spin s) per block of the vector
+// kernel.  Measured at Nside 256 and 512 (spin 0): the wide form is fastest from 24 columns up
+// (1.2-1.4x), the 16-column one from 5 to 16 columns, the vector kernel up to 4.  Spin s: the wide
+// form from two tiles of 16 complex columns (synth_sp_nt).
 template <typename T>
 static int synth_shape_rule(const ShtPlan& p, int ncol) {
     if (p.spin == 0) {
@@ -4034,6 +4251,7 @@ static int synth_shape_rule(const ShtPlan& p, int ncol) {
     }
     constexpr int BCMAX = sizeof(T) == 4 ? SYN_BC_SP_F : SYN_BC_SP_D;
     const int nmap = ncol / p.ncomp;
+    if (sizeof(T) == 4 && ncol > 16) return SYN_SHAPE_WR;  // the wide matrix-core form
     if (nmap > 2) return 0;
     return nmap <= 1 ? 1 : (BCMAX > 2 && nmap <= 2) ? 2 : BCMAX;
 }
@@ -4094,14 +4312,51 @@ static int launch_synth_sp_mfma(const ShtPlan& p, int ncol, int ld, const void* 
     return 0;
 }
 
+static int shape_of(const ShtPlan& p, int ncol, int stage);
+
+// The wide spin-s form (k_leg_synth_sp_wr): nt = 2 .. 4 tiles of 16 complex columns per block;
+// ncol counts complex columns, or maps with GONLY.  synth_sp_nt: the built-in tile count, 2 from
+// two tiles of columns up (1: the one-tile kernel).  Measured (FP32, the one-tile kernel against
+// nt = 2 / 3): the gradient-only syntheses of 60 maps at Nside 1024, lmax 2000, 49.0 against
+// 41.1 / 59.2 ms, 40 maps at Nside 256 1.12 against 1.02 ms, the (G, C) synthesis of 64 maps at
+// Nside 1024, lmax 2048, 88.0 against 71.3 / 86.5 ms.  At nt = 2 the kernel holds 215 VGPRs,
+// two waves per SIMD as the one-tile kernel, and runs each recursion step for twice the MFMAs;
+// at nt = 3 and 4 it needs more than 256 registers and one wave per SIMD no longer hides the
+// recursion's latency (tuner candidates only).
+static int synth_sp_nt(int ncol) { return (ncol + 15) / 16 >= 2 ? 2 : 1; }
+template <bool GONLY>
+static int launch_synth_sp_wr(const ShtPlan& p, int ncol, int ld, const void* alm_t, void* phase, hipStream_t st,
+                              int nt) {
+    ShtDev<float> d = dev_view<float>(p);
+    const int nrb = (p.npairs + 127) / 128, nchunk = (ncol + 16 * nt - 1) / (16 * nt);
+    const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
+    const cplx<float>* a = (const cplx<float>*)alm_t;
+    cplx<float>* ph = (cplx<float>*)phase;
+    switch (nt) {
+        case 2: hipLaunchKernelGGL((k_leg_synth_sp_wr<2, GONLY>), dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph); break;
+        case 3: hipLaunchKernelGGL((k_leg_synth_sp_wr<3, GONLY>), dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph); break;
+        case 4: hipLaunchKernelGGL((k_leg_synth_sp_wr<4, GONLY>), dim3(nb), dim3(256), 0, st, d, ncol, ld, a, ph); break;
+        default: set_error("synthesis: invalid wide tile count"); return -1;
+    }
+    PS_LAUNCH_CHECK();
+    return 0;
+}
+
 // Spin-s synthesis of (G, C = 0) from G alone: alm_t (nlm x ld, column b = G of map b) -> ring
-// phases of the R and I components (2 nmap columns); FP32 plans of spin >= 1.
+// phases of the R and I components (2 nmap columns); FP32 plans of spin >= 1.  The Legendre shape
+// of the plan's shape table for this batch (ps_sht_set_shapes) picks the one-tile kernel (0) or
+// the wide form (SYN_SHAPE_WR, SYN_SHAPE_WR_NT + nt); otherwise the built-in rule.
 int sht_legendre_synthesis_grad(const ShtPlan& p, int nmap, const void* alm_t, int ld, void* phase,
                                 hipStream_t st) {
     if (p.prec != 4 || p.spin < 1) {
         set_error("gradient-only synthesis: FP32 spin-s plans only");
         return -1;
     }
+    const int shape = shape_of(p, nmap * p.ncomp, SHT_STAGE_SYNTH);
+    int nt = synth_sp_nt(nmap);
+    if (shape == 0) nt = 1;
+    else if (shape >= SYN_SHAPE_WR_NT + 2 && shape <= SYN_SHAPE_WR_NT + 4) nt = shape - SYN_SHAPE_WR_NT;
+    if (nt >= 2) return launch_synth_sp_wr<true>(p, nmap, ld, alm_t, phase, st, nt);
     ShtDev<float> d = dev_view<float>(p);
     const int nrb = (p.npairs + 127) / 128, nchunk = (nmap + 15) / 16;
     const unsigned nb = (unsigned)(nrb * nchunk * (p.mmax + 1));
@@ -4119,6 +4374,16 @@ static int launch_synth_sp_any(const ShtPlan& p, int nmap, int ld, const void* a
         case 0:
             if constexpr (sizeof(T) == 4) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
             else return launch_synth_mfma64(p, 2 * nmap, ld, alm_t, phase, st);
+        case SYN_SHAPE_WR:
+        case SYN_SHAPE_WR_NT + 2:
+        case SYN_SHAPE_WR_NT + 3:
+        case SYN_SHAPE_WR_NT + 4:
+            if constexpr (sizeof(T) == 4) {
+                const int nt = shape == SYN_SHAPE_WR ? synth_sp_nt(2 * nmap) : shape - SYN_SHAPE_WR_NT;
+                if (nt < 2) return launch_synth_sp_mfma(p, 2 * nmap, ld, alm_t, phase, st);
+                return launch_synth_sp_wr<false>(p, 2 * nmap, ld, alm_t, phase, st, nt);
+            }
+            break;
         case 1: return launch_synth_sp<T, 1>(p, nmap, ld, alm_t, phase, st);
         case 2: return launch_synth_sp<T, 2>(p, nmap, ld, alm_t, phase, st);
         case 4:
@@ -4439,6 +4704,10 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
             if (p.prec == 4 && p.spin == 0) add(SYN_SHAPE_WR);
             if (p.prec == 4 && p.spin == 0 && ncol > 16)
                 for (int nt = 2; nt < synth_lt_nt(ncol) && nt <= 4; nt++) add(SYN_SHAPE_WR_NT + nt);
+            if (p.prec == 4 && p.spin > 0 && ncol > 16) {  // ncol = 2 nmap complex columns
+                add(SYN_SHAPE_WR);
+                for (int nt = 3; nt <= 4 && 16 * (nt - 1) < ncol; nt++) add(SYN_SHAPE_WR_NT + nt);
+            }
             for (int bc = std::max(1, top / 2); bc <= top; bc *= 2) add(bc);
             break;
         }

@@ -39,6 +39,13 @@ def test_gradient_only_synthesis(gpu, nside, lmax, spin, nmaps):
     # repeated calls are bit-identical; an FP64 plan falls back to the full synthesis of (G, 0)
     again = plan.synthesis_grad(Gd).double().cpu().numpy()
     assert np.array_equal(again, got)
+    # every Legendre shape of the batch (the one-tile kernel, the wide form at 2, 3, 4 tiles):
+    # the same products in the same order per output, so the same maps
+    for sh in plan.shape_candidates(nmaps, 'synthesis'):
+        if sh == 0 or sh >= 65:
+            plan.set_shapes(nmaps, [sh, -1, -1, -1])
+            alt = plan.synthesis_grad(Gd).double().cpu().numpy()
+            assert np.abs(alt - got).max() <= 2e-6 * scale, sh
     if nside <= 32:
         p64 = ShtPlan(nside, lmax, spin, 64)
         g64 = p64.synthesis_grad(torch.from_numpy(G).to(gpu)).cpu().numpy()
