@@ -66,7 +66,9 @@ int ps_sht_synthesis(const ps_sht_plan *plan, int nmaps, const void *alm, void *
  * (nmaps, 2, npix), i.e. ps_sht_synthesis of (G, 0) without the C half of the Legendre work.
  * It replaces compute_spin_transform_map's synthesis of the bin-filtered h (bspec_bin.py:216-228,
  * 335-337 through base.py:260-288), whose curl part is zero.  FP32 plans of spin >= 1; work:
- * ps_sht_workspace_bytes(plan, nmaps, 0) bytes. */
+ * ps_sht_workspace_bytes(plan, nmaps, 0) bytes.  The Legendre shape set for nmaps by
+ * ps_sht_set_shapes (0: one tile of 16 maps per block; the wide shapes: 2-4 tiles sharing the
+ * recursions) applies; by default the wide form from 17 maps on. */
 int ps_sht_synthesis_grad(const ps_sht_plan *plan, int nmaps, const void *G, void *map, void *work,
                           long long work_bytes, void *stream);
 

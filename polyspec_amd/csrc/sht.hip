@@ -4053,7 +4053,12 @@ static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
 // chunk of 2 (spin 0) or 1 map (spin s) per block.  Throughput (many columns): 4 ring pairs per
 // thread and more waves per block.
 static int adj_shape_rule(const ShtPlan& p, int ncol, bool many, int pps) {
-    // batches: the matrix-core kernel (FP32 as for the synthesis; FP64 from 8 real columns)
+    // batches: the matrix-core kernel (FP32 as for the synthesis; FP64 from 8 real columns).
+    // FP32 spin 0 from 3 columns on plans of 1024 ring pairs and more (Nside >= 512; measured,
+    // the whole adjoint of 4 / 8 maps: Nside 1024, lmax 2000, 8.3 / 10.0 ms on the vector kernel
+    // against 4.9 / 5.3 ms; Nside 512, lmax 1024, 0.93 / 1.13 against 0.77 / 0.88 ms; at Nside 256
+    // the vector kernel stays ahead, 0.18 / 0.22 against 0.23 / 0.25 ms)
+    if (p.prec == 4 && p.spin == 0 && ncol > 2 && p.npairs >= 1024) return ADJ_SHAPE_MFMA;
     if (p.prec == 4 && (p.spin == 0 ? ncol > 8 : ncol / 2 > 2)) return ADJ_SHAPE_MFMA;
     if (p.prec == 8 && (p.spin == 0 ? ncol > 4 : ncol / 2 > 1)) return ADJ_SHAPE_MFMA;
     const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
