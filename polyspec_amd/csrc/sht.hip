@@ -3112,15 +3112,19 @@ __global__ __launch_bounds__(256) void k_leg_adj_s0_mfma(ShtDev<float> d, int nc
 // 32-63 f- of the wave's 32 ring pairs, and per 64 values of l a wave issues 64
 // v_mfma_f32_32x32x2f32 (A = f+- of 32 values of l of one parity x 2 rings, B = B+- of 2 rings
 // x 32 real columns = 8 maps).  One block = 8 waves = 256 ring pairs, one m, 8 maps.
+// NC > 1: NC chunks of 8 maps per block share the recursion tiles (their MFMAs run from the same
+// A operands, each chunk's sums pass through LDS in turn); per chunk the products accumulate in
+// the order of NC = 1, so the results are the same.
 constexpr int ADJM_SP_PAIRS = 256;
 
+template <int NC>
 __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nmap,
                                                          const cplx<float>* __restrict__ phase,
                                                          cplx<float>* __restrict__ out, long out_stride, float osc,
                                                          int nslice) {
     // f tiles [wave][sign][parity][l / 2][ring ^ (l / 2)], then the waves' sums [w][par][i][33]
     __shared__ __align__(16) float T[8 * 4 * 32 * 32];
-    const int nchunk = (nmap + 7) / 8;
+    const int nchunk = (nmap + 8 * NC - 1) / (8 * NC);
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = lb % nchunk;
     const int sl = (lb / nchunk) % nslice;
@@ -3146,9 +3150,10 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
 
     // B operands of ring pair wring + 2 s + h, real column li32 = 4 (map) + q:
     // bp0 / bm0 for rows of even l - lstart, bp1 / bm1 for odd
-    float bp0[16], bm0[16], bp1[16], bm1[16];
-    {
-        const int b = chunk * 8 + (li32 >> 2), q = li32 & 3;
+    float bp0[NC][16], bm0[NC][16], bp1[NC][16], bm1[NC][16];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) {
+        const int b = (chunk * NC + cc) * 8 + (li32 >> 2), q = li32 & 3;
         const size_t pstride = (size_t)M1 * d.nrings;
         const cplx<float>* PR = phase + (size_t)(2 * b) * pstride + (size_t)m * d.nrings;
         const cplx<float>* PI = PR + pstride;
@@ -3175,16 +3180,16 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
             }
             // row parity 0 (l = lstart + even) is the even type when p0 == 0
             const float p_0 = p0 ? po : pe, n_0 = p0 ? no : ne, p_1 = p0 ? pe : po, n_1 = p0 ? ne : no;
-            bp0[s2] = 0.5f * (p_0 + n_0);
-            bm0[s2] = 0.5f * (p_0 - n_0);
-            bp1[s2] = 0.5f * (p_1 + n_1);
-            bm1[s2] = 0.5f * (p_1 - n_1);
+            bp0[cc][s2] = 0.5f * (p_0 + n_0);
+            bm0[cc][s2] = 0.5f * (p_0 - n_0);
+            bp1[cc][s2] = 0.5f * (p_1 + n_1);
+            bm1[cc][s2] = 0.5f * (p_1 - n_1);
         }
     }
 
     // coefficients with m <= l < spin vanish identically
-    for (int i = tid; i < (lstart - m) * 16; i += 512) {
-        const int l = m + i / 16, b = chunk * 8 + (i % 16) / 2, g = i % 2;
+    for (int i = tid; i < (lstart - m) * 16 * NC; i += 512) {
+        const int l = m + i / (16 * NC), b = chunk * 8 * NC + (i % (16 * NC)) / 2, g = i % 2;
         if (l <= d.lmax && b < nmap) dst[(size_t)(2 * b + g) * out_stride + base + l] = cplx<float>{0.f, 0.f};
     }
     const double* cf4 = d.coef + 4 * base;
@@ -3194,7 +3199,9 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
     bool fast = false;
     for (int l0 = lstart; l0 <= d.lmax; l0 += 64) {
         const int n = min(64, d.lmax + 1 - l0);
-        v16f D0 = v16f{0}, D1 = v16f{0};
+        v16f D0[NC], D1[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; cc++) D0[cc] = D1[cc] = v16f{0};
         if (!skip) {
             double pal = 0.0, pbe = 0.0, pga = 0.0;
             if (lane < n) {
@@ -3236,23 +3243,29 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma(ShtDev<float> d, int nm
 #pragma unroll
             for (int s2 = 0; s2 < 16; s2++) {
                 const int rl = 2 * s2 + h, sw = li32 * 32 + (rl ^ li32);
-                D0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[sw], bp0[s2], D0, 0, 0, 0);
-                D0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[2048 + sw], bm0[s2], D0, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[1024 + sw], bp1[s2], D1, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ta[2048 + 1024 + sw], bm1[s2], D1, 0, 0, 0);
+                const float a0 = Ta[sw], a1 = Ta[2048 + sw], a2 = Ta[1024 + sw], a3 = Ta[2048 + 1024 + sw];
+#pragma unroll
+                for (int cc = 0; cc < NC; cc++) {
+                    D0[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bp0[cc][s2], D0[cc], 0, 0, 0);
+                    D0[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bm0[cc][s2], D0[cc], 0, 0, 0);
+                    D1[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, bp1[cc][s2], D1[cc], 0, 0, 0);
+                    D1[cc] = __builtin_amdgcn_mfma_f32_32x32x2f32(a3, bm1[cc][s2], D1[cc], 0, 0, 0);
+                }
             }
         }
-        __syncthreads();  // every wave has read its tiles: reuse the space for the sums
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int i = 8 * (v >> 2) + 4 * h + (v & 3);
-            T[((wave * 2) * 32 + i) * 33 + li32] = D0[v];
-            T[((wave * 2 + 1) * 32 + i) * 33 + li32] = D1[v];
-        }
-        __syncthreads();
-        {
+        for (int cc = 0; cc < NC; cc++) {
+            // every wave has read its tiles (cc = 0) or the previous chunk's sums: reuse the space
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = 8 * (v >> 2) + 4 * h + (v & 3);
+                T[((wave * 2) * 32 + i) * 33 + li32] = D0[cc][v];
+                T[((wave * 2 + 1) * 32 + i) * 33 + li32] = D1[cc][v];
+            }
+            __syncthreads();
             // thread: real column j = 4 (map) + q, l = l0 + 4 (tid % 16) + u
-            const int j = tid >> 4, b = chunk * 8 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
+            const int j = tid >> 4, b = (chunk * NC + cc) * 8 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int ll = (tid & 15) * 4 + u, par = ll & 1, i = ll >> 1;
@@ -4039,11 +4052,14 @@ static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES
 // shape id of the matrix-core adjoint: 256 ring pairs per block and 16 complex columns (spin 0) or
 // 8 maps (spin s) in FP32, 8 complex columns or 4 maps in FP64
 static constexpr int ADJ_SHAPE_MFMA = N_ADJ_SHAPES;
+// the FP32 spin-s matrix-core kernel with two chunks of 8 maps per block
+static constexpr int ADJ_SHAPE_MFMA2 = N_ADJ_SHAPES + 1;
 
 static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4 || p.prec == 8; }
 
 static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
     if (id == ADJ_SHAPE_MFMA) return adj_mfma_ok(p);
+    if (id == ADJ_SHAPE_MFMA2) return p.prec == 4 && p.spin > 0;
     if (id < 0 || id >= N_ADJ_SHAPES) return false;
     const AdjShape& a = ADJ_SHAPES[id];
     return (p.spin == 0 ? a.s0 : a.sp) && 64 * a.nrt * a.nw >= pps;
@@ -4059,6 +4075,12 @@ static int adj_shape_rule(const ShtPlan& p, int ncol, bool many, int pps) {
     // against 4.9 / 5.3 ms; Nside 512, lmax 1024, 0.93 / 1.13 against 0.77 / 0.88 ms; at Nside 256
     // the vector kernel stays ahead, 0.18 / 0.22 against 0.23 / 0.25 ms)
     if (p.prec == 4 && p.spin == 0 && ncol > 2 && p.npairs >= 1024) return ADJ_SHAPE_MFMA;
+    // FP32 spin s: two chunks of 8 maps per block from 9 maps on plans of 1024 ring pairs and
+    // more, from 16 maps on smaller ones (measured, whole adjoint: 64 maps at Nside 1024, lmax 2048,
+    // 104.1 against 80.9 ms; 16 at Nside 512, lmax 1535, 7.84 against 6.10 ms; 13 at Nside 512,
+    // lmax 1024, 3.78 against 2.96 ms; but 9 at Nside 256, 0.63 against 0.76 ms: the second chunk
+    // nearly empty and half the blocks)
+    if (p.prec == 4 && p.spin > 0 && ncol / 2 > 8 && (p.npairs >= 1024 || ncol / 2 >= 16)) return ADJ_SHAPE_MFMA2;
     if (p.prec == 4 && (p.spin == 0 ? ncol > 8 : ncol / 2 > 2)) return ADJ_SHAPE_MFMA;
     if (p.prec == 8 && (p.spin == 0 ? ncol > 4 : ncol / 2 > 1)) return ADJ_SHAPE_MFMA;
     const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
@@ -4104,9 +4126,10 @@ static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phas
 static_assert(ADJM_PAIRS == ADJM_SP_PAIRS, "both matrix-core adjoints slice rings alike");
 static int adj_mfma_nslice(const ShtPlan& p) { return (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS; }
 
+// nc: chunks of 8 maps per block of the FP32 spin-s kernel (1 or 2)
 template <typename T>
 static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
-                           double scale, void* part, hipStream_t st) {
+                           double scale, void* part, hipStream_t st, int nc = 1) {
     ShtDev<T> d = dev_view<T>(p);
     const int nslice = adj_mfma_nslice(p);
     void* dst = nslice > 1 ? part : alm;
@@ -4118,9 +4141,14 @@ static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* 
             hipLaunchKernelGGL(k_leg_adj_s0_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0,
                                st, d, ncol, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
         } else {
-            const int nmap = ncol / 2, nchunk = (nmap + 7) / 8;
-            hipLaunchKernelGGL(k_leg_adj_sp_mfma, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0,
-                               st, d, nmap, (const cplx<float>*)phase, (cplx<float>*)dst, dstride, osc, nslice);
+            const int nmap = ncol / 2, nchunk = (nmap + 8 * nc - 1) / (8 * nc);
+            const unsigned nb = (unsigned)(nchunk * nslice * (p.mmax + 1));
+            if (nc == 2)
+                hipLaunchKernelGGL(k_leg_adj_sp_mfma<2>, dim3(nb), dim3(512), 0, st, d, nmap, (const cplx<float>*)phase,
+                                   (cplx<float>*)dst, dstride, osc, nslice);
+            else
+                hipLaunchKernelGGL(k_leg_adj_sp_mfma<1>, dim3(nb), dim3(512), 0, st, d, nmap, (const cplx<float>*)phase,
+                                   (cplx<float>*)dst, dstride, osc, nslice);
         }
     } else {
         if (p.spin == 0) {
@@ -4156,12 +4184,12 @@ static int launch_adj(const ShtPlan& p, int ncol, const void* phase, void* alm, 
     const long dstride = ns > 1 ? p.nlm : alm_stride;
     const bool many = p.spin == 0 ? ncol > 2 : nmap > 1;
     if (shape < 0) shape = adj_shape_rule(p, ncol, many, pps);
-    if (shape == ADJ_SHAPE_MFMA) {
-        if (!adj_mfma_ok(p)) {
+    if (shape == ADJ_SHAPE_MFMA || shape == ADJ_SHAPE_MFMA2) {
+        if (!adj_shape_valid(p, shape, pps)) {
             set_error("adjoint: invalid launch shape");
             return -1;
         }
-        return launch_adj_mfma<T>(p, ncol, phase, alm, alm_stride, scale, part, st);
+        return launch_adj_mfma<T>(p, ncol, phase, alm, alm_stride, scale, part, st, shape == ADJ_SHAPE_MFMA2 ? 2 : 1);
     }
     int rc = launch_adj_shape<T>(p, shape, ncol, phase, dst, dstride, scale, ns, pps, st);
     if (rc || ns == 1) return rc;
@@ -4737,6 +4765,7 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
                 if (adj_shape_valid(p, id, pps) && (cover <= 4 * pps || cover == 64)) add(id);
             }
             if (adj_mfma_ok(p)) add(ADJ_SHAPE_MFMA);
+            if (p.prec == 4 && p.spin > 0 && nmap > 8) add(ADJ_SHAPE_MFMA2);
             break;
         }
         default: set_error("launch shapes: stage must be 0..3"); return -1;

@@ -39,6 +39,8 @@ def _case(nside, lmax, spin, nmaps, seed):
     (64, 128, 32, 0, 80), (32, 95, 32, 0, 100), (128, 200, 32, 0, 40),
     # FP32 spin-s batches of two tiles and more (the wide spin-s synthesis at 2, 3 and 4 tiles)
     (32, 64, 32, 1, 9), (64, 128, 32, 2, 40),
+    # the spin-s matrix-core adjoint with two chunks of 8 maps per block, the last one partial
+    (64, 128, 32, 2, 21), (256, 512, 32, 2, 17),
     # FP64 batches (the FP64 matrix-core adjoint; ring slices and partial sums from Nside 256)
     (32, 64, 64, 0, 20), (64, 128, 64, 2, 9), (256, 512, 64, 0, 12), (256, 512, 64, 2, 5)])
 def test_every_candidate_shape_matches_oracle(gpu, nside, lmax, prec, spin, nmaps):
