@@ -3419,13 +3419,16 @@ __global__ __launch_bounds__(256) void k_leg_adj_s0_mfma64(ShtDev<double> d, int
 // Spin s: as k_leg_adj_sp_mfma (lanes 0-31 of a wave run f+ and lanes 32-63 f- of the wave's
 // 32 ring pairs; D += f+ B+ + f- B-), one block = 8 waves = 256 ring pairs, one m, 4 maps
 // (16 real columns: G re, G im, C re, C im of each map).
+// NC > 1: NC chunks of 4 maps per block share the recursion tiles; per chunk the products
+// accumulate in the order of NC = 1.
+template <int NC>
 __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int nmap,
                                                            const cplx<double>* __restrict__ phase,
                                                            cplx<double>* __restrict__ out, long out_stride,
                                                            double osc, int nslice) {
     // f tiles [wave][sign][parity][i][32], then the waves' sums [wave][parity][i][17]
     __shared__ __align__(16) double T[8 * 2 * 2 * 16 * 32];
-    const int nchunk = (nmap + 3) / 4;
+    const int nchunk = (nmap + 4 * NC - 1) / (4 * NC);
     unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = lb % nchunk;
     const int sl = (lb / nchunk) % nslice;
@@ -3451,9 +3454,10 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int
     const int p0 = (lstart + m + d.spin) & 1;  // tile rows of parity p0 are the "odd" type
 
     // B operands of ring pair wring + 4 s + k4, real column j16 = 4 (map) + q
-    double bp0[8], bm0[8], bp1[8], bm1[8];
-    {
-        const int b = chunk * 4 + (j16 >> 2), q = j16 & 3;
+    double bp0[NC][8], bm0[NC][8], bp1[NC][8], bm1[NC][8];
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) {
+        const int b = (chunk * NC + cc) * 4 + (j16 >> 2), q = j16 & 3;
         const size_t pstride = (size_t)M1 * d.nrings;
         const cplx<double>* PR = phase + (size_t)(2 * b) * pstride + (size_t)m * d.nrings;
         const cplx<double>* PI = PR + pstride;
@@ -3479,16 +3483,16 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int
                 default: pe = -Ia.im; ne = Rd.re; po = -Id.im; no = Ra.re; break;
             }
             const double p_0 = p0 ? po : pe, n_0 = p0 ? no : ne, p_1 = p0 ? pe : po, n_1 = p0 ? ne : no;
-            bp0[s] = 0.5 * (p_0 + n_0);
-            bm0[s] = 0.5 * (p_0 - n_0);
-            bp1[s] = 0.5 * (p_1 + n_1);
-            bm1[s] = 0.5 * (p_1 - n_1);
+            bp0[cc][s] = 0.5 * (p_0 + n_0);
+            bm0[cc][s] = 0.5 * (p_0 - n_0);
+            bp1[cc][s] = 0.5 * (p_1 + n_1);
+            bm1[cc][s] = 0.5 * (p_1 - n_1);
         }
     }
 
     // coefficients with m <= l < spin vanish identically
-    for (int i = tid; i < (lstart - m) * 8; i += 512) {
-        const int l = m + i / 8, b = chunk * 4 + (i % 8) / 2, g = i % 2;
+    for (int i = tid; i < (lstart - m) * 8 * NC; i += 512) {
+        const int l = m + i / (8 * NC), b = chunk * 4 * NC + (i % (8 * NC)) / 2, g = i % 2;
         if (l <= d.lmax && b < nmap) dst[(size_t)(2 * b + g) * out_stride + base + l] = cplx<double>{0.0, 0.0};
     }
     const double* cf4 = d.coef + 4 * base;
@@ -3498,7 +3502,9 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int
     bool fast = false;
     for (int l0 = lstart; l0 <= d.lmax; l0 += ADJ64_L) {
         const int n = min(ADJ64_L, d.lmax + 1 - l0);
-        v4d_t D0 = v4d_t{0.0, 0.0, 0.0, 0.0}, D1 = v4d_t{0.0, 0.0, 0.0, 0.0};
+        v4d_t D0[NC], D1[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; cc++) D0[cc] = D1[cc] = v4d_t{0.0, 0.0, 0.0, 0.0};
         if (!skip) {
             double pal = 0.0, pbe = 0.0, pga = 0.0;
             if (lane < n) {
@@ -3538,23 +3544,29 @@ __global__ __launch_bounds__(512) void k_leg_adj_sp_mfma64(ShtDev<double> d, int
 #pragma unroll
             for (int s = 0; s < 8; s++) {
                 const int sw = j16 * 32 + ((4 * s + k4) ^ ((4 * j16) & 28));
-                D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[sw], bp0[s], D0, 0, 0, 0);
-                D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[1024 + sw], bm0[s], D0, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[512 + sw], bp1[s], D1, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ta[1024 + 512 + sw], bm1[s], D1, 0, 0, 0);
+                const double a0 = Ta[sw], a1 = Ta[1024 + sw], a2 = Ta[512 + sw], a3 = Ta[1024 + 512 + sw];
+#pragma unroll
+                for (int cc = 0; cc < NC; cc++) {
+                    D0[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bp0[cc][s], D0[cc], 0, 0, 0);
+                    D0[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bm0[cc][s], D0[cc], 0, 0, 0);
+                    D1[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bp1[cc][s], D1[cc], 0, 0, 0);
+                    D1[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bm1[cc][s], D1[cc], 0, 0, 0);
+                }
             }
         }
-        __syncthreads();  // every wave has read its tiles: reuse the space for the sums
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = 4 * v + k4;
-            T[((wave * 2) * 16 + i) * 17 + j16] = D0[v];
-            T[((wave * 2 + 1) * 16 + i) * 17 + j16] = D1[v];
-        }
-        __syncthreads();
-        {
+        for (int cc = 0; cc < NC; cc++) {
+            // every wave has read its tiles (cc = 0) or the previous chunk's sums: reuse the space
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 4 * v + k4;
+                T[((wave * 2) * 16 + i) * 17 + j16] = D0[cc][v];
+                T[((wave * 2 + 1) * 16 + i) * 17 + j16] = D1[cc][v];
+            }
+            __syncthreads();
             // thread: real column j = 4 (map) + q, l = l0 + tid % 32
-            const int j = tid >> 5, b = chunk * 4 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
+            const int j = tid >> 5, b = (chunk * NC + cc) * 4 + (j >> 2), g = (j >> 1) & 1, pp = j & 1;
             const int ll = tid & 31, par = ll & 1, i = ll >> 1;
             double sum = 0.0;
 #pragma unroll
@@ -4052,14 +4064,14 @@ static constexpr int N_ADJ_SHAPES = (int)(sizeof(ADJ_SHAPES) / sizeof(ADJ_SHAPES
 // shape id of the matrix-core adjoint: 256 ring pairs per block and 16 complex columns (spin 0) or
 // 8 maps (spin s) in FP32, 8 complex columns or 4 maps in FP64
 static constexpr int ADJ_SHAPE_MFMA = N_ADJ_SHAPES;
-// the FP32 spin-s matrix-core kernel with two chunks of 8 maps per block
+// the spin-s matrix-core kernels with two chunks of maps per block (8 maps a chunk in FP32, 4 in FP64)
 static constexpr int ADJ_SHAPE_MFMA2 = N_ADJ_SHAPES + 1;
 
 static bool adj_mfma_ok(const ShtPlan& p) { return p.prec == 4 || p.prec == 8; }
 
 static bool adj_shape_valid(const ShtPlan& p, int id, int pps) {
     if (id == ADJ_SHAPE_MFMA) return adj_mfma_ok(p);
-    if (id == ADJ_SHAPE_MFMA2) return p.prec == 4 && p.spin > 0;
+    if (id == ADJ_SHAPE_MFMA2) return adj_mfma_ok(p) && p.spin > 0;
     if (id < 0 || id >= N_ADJ_SHAPES) return false;
     const AdjShape& a = ADJ_SHAPES[id];
     return (p.spin == 0 ? a.s0 : a.sp) && 64 * a.nrt * a.nw >= pps;
@@ -4082,6 +4094,11 @@ static int adj_shape_rule(const ShtPlan& p, int ncol, bool many, int pps) {
     // nearly empty and half the blocks)
     if (p.prec == 4 && p.spin > 0 && ncol / 2 > 8 && (p.npairs >= 1024 || ncol / 2 >= 16)) return ADJ_SHAPE_MFMA2;
     if (p.prec == 4 && (p.spin == 0 ? ncol > 8 : ncol / 2 > 2)) return ADJ_SHAPE_MFMA;
+    // FP64 spin s likewise with chunks of 4 maps (whole adjoint: 64 maps at Nside 1024, lmax 2048,
+    // 203.3 against 159.0 ms; 16 at Nside 512, lmax 1535, 14.19 against 11.08 ms; 6 at Nside 512,
+    // lmax 1024, 3.51 against 2.77 ms; at Nside 256, 16 maps 1.29 against 1.07 ms but 5 maps 0.61
+    // against 0.77 ms)
+    if (p.prec == 8 && p.spin > 0 && ncol / 2 > 4 && (p.npairs >= 1024 || ncol / 2 >= 8)) return ADJ_SHAPE_MFMA2;
     if (p.prec == 8 && (p.spin == 0 ? ncol > 4 : ncol / 2 > 1)) return ADJ_SHAPE_MFMA;
     const int lvl = pps <= 64 ? 0 : pps <= 128 ? 1 : pps <= 256 ? 2 : pps <= 512 ? 3 : pps <= 1024 ? 4 : 5;
     if (many) {
@@ -4126,7 +4143,7 @@ static int launch_adj_shape(const ShtPlan& p, int id, int ncol, const void* phas
 static_assert(ADJM_PAIRS == ADJM_SP_PAIRS, "both matrix-core adjoints slice rings alike");
 static int adj_mfma_nslice(const ShtPlan& p) { return (p.npairs + ADJM_PAIRS - 1) / ADJM_PAIRS; }
 
-// nc: chunks of 8 maps per block of the FP32 spin-s kernel (1 or 2)
+// nc: chunks of maps per block of the spin-s kernels (1 or 2)
 template <typename T>
 static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* alm, long alm_stride,
                            double scale, void* part, hipStream_t st, int nc = 1) {
@@ -4156,9 +4173,14 @@ static int launch_adj_mfma(const ShtPlan& p, int ncol, const void* phase, void* 
             hipLaunchKernelGGL(k_leg_adj_s0_mfma64, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(256), 0,
                                st, d, ncol, (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
         } else {
-            const int nmap = ncol / 2, nchunk = (nmap + 3) / 4;
-            hipLaunchKernelGGL(k_leg_adj_sp_mfma64, dim3((unsigned)(nchunk * nslice * (p.mmax + 1))), dim3(512), 0,
-                               st, d, nmap, (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
+            const int nmap = ncol / 2, nchunk = (nmap + 4 * nc - 1) / (4 * nc);
+            const unsigned nb = (unsigned)(nchunk * nslice * (p.mmax + 1));
+            if (nc == 2)
+                hipLaunchKernelGGL(k_leg_adj_sp_mfma64<2>, dim3(nb), dim3(512), 0, st, d, nmap,
+                                   (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
+            else
+                hipLaunchKernelGGL(k_leg_adj_sp_mfma64<1>, dim3(nb), dim3(512), 0, st, d, nmap,
+                                   (const cplx<double>*)phase, (cplx<double>*)dst, dstride, osc, nslice);
         }
     }
     PS_LAUNCH_CHECK();
@@ -4765,7 +4787,7 @@ int sht_shape_candidates(const ShtPlan& p, int nmap, int stage, int* ids, int ma
                 if (adj_shape_valid(p, id, pps) && (cover <= 4 * pps || cover == 64)) add(id);
             }
             if (adj_mfma_ok(p)) add(ADJ_SHAPE_MFMA);
-            if (p.prec == 4 && p.spin > 0 && nmap > 8) add(ADJ_SHAPE_MFMA2);
+            if (p.spin > 0 && nmap > (p.prec == 4 ? 8 : 4)) add(ADJ_SHAPE_MFMA2);
             break;
         }
         default: set_error("launch shapes: stage must be 0..3"); return -1;
